@@ -1,0 +1,225 @@
+/*
+ * nlo_gfx950 — C ABI of the MI355X (gfx950) NLO BK right-hand side, Sobol /
+ * VEGAS sampling and rapidity driver.
+ *
+ * Conventions (all entry points):
+ *   - pointers named d_* are device pointers; the caller owns every buffer,
+ *     including scratch ("workspace"), whose size is queried first;
+ *   - every launch goes on the stream argument (hipStream_t); nothing
+ *     synchronises the host; there is no hidden global state except a
+ *     thread-local last-error string;
+ *   - return value: NLO_OK (0) or an NLO_ERR_* code; nlo_last_error() gives
+ *     the message.  The Python layer turns a non-zero code into RuntimeError,
+ *     as the reference's TORCH_CHECK failures do.
+ *
+ * Reference prefixes: S/ = torch_nlo_clean/src/nlo_torch/,
+ * K/ = S/custom_kernels/.  Each entry point names the reference interface it
+ * replaces.
+ */
+#ifndef NLO_GFX950_H
+#define NLO_GFX950_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NLO_OK 0
+#define NLO_ERR_INVALID 1     /* bad argument (shape, size, null pointer) */
+#define NLO_ERR_HIP 2         /* a HIP runtime call failed */
+#define NLO_ERR_WORKSPACE 3   /* workspace too small */
+#define NLO_ERR_UNSUPPORTED 4 /* configuration outside the implemented set */
+
+#define NLO_API_VERSION 1
+
+/* ------------------------------------------------------------------ misc */
+
+int nlo_api_version(void);
+/* Message of the last failed call on this host thread ("" if none). */
+const char* nlo_last_error(void);
+/* Gauss-Legendre nodes (ascending) and weights on [-1, 1], computed on the
+ * host by Newton iteration in long double.  Replaces the reference's
+ * numpy.polynomial.legendre.leggauss tables (K/quadrature.py:18). */
+int nlo_gauss_legendre(int32_t n, double* x, double* w);
+
+/* --------------------------------------------------------------- physics */
+
+/* BKConfig physics constants (S/bk/config.py:66-101) for the implemented
+ * production choice: RC_LO = BALITSKY_LO, RC_NLO = SMALLEST_NLO,
+ * RESUM_RC = RESUM_RC_SMALLEST, Order = NLO_RESUM_DLOG_SLOG, NF <= 3 (smooth
+ * coupling, S/coupling.py:81-97). */
+typedef struct nlo_bk_physics {
+    double nc;          /* NC */
+    double nf;          /* NF (0 or 3) */
+    double lambda_qcd;  /* LambdaQCD */
+    double c2;          /* C2 */
+    double ksub;        /* KSUB */
+    double minr;        /* MINR (double-log resummation only for r > 1.01 MINR) */
+    int32_t symmetrize; /* SYMMETRIZE_Z_Z2_INTEGRATION */
+    int32_t reserved;
+} nlo_bk_physics;
+
+/* ----------------------------------------------------------------- Sobol */
+
+#define NLO_SOBOL_MAXDIM 8
+#define NLO_SOBOL_BITS 30
+
+/* Scrambled Sobol state of torch.quasirandom.SobolEngine(dim, scramble=True,
+ * seed): direction = SobolEngine.sobolstate, shift = SobolEngine.shift,
+ * first = integer state of the float64 first point (float32(shift / 2^30)
+ * widened, torch/quasirandom.py _first_point).  Passed by value to kernels. */
+typedef struct nlo_sobol_table {
+    int32_t dim;
+    uint32_t direction[NLO_SOBOL_MAXDIM][NLO_SOBOL_BITS];
+    uint32_t shift[NLO_SOBOL_MAXDIM];
+    uint32_t first[NLO_SOBOL_MAXDIM];
+} nlo_sobol_table;
+
+#define NLO_SOBOL_U32 0 /* raw 30-bit integer states (row 0 = shift)        */
+#define NLO_SOBOL_F32 1 /* float(q) * 2^-30, SobolEngine.draw(dtype=float32) */
+#define NLO_SOBOL_F64 2 /* q * 2^-30, row 0 = first * 2^-30 (dtype=float64)  */
+
+/* Points [first_index, first_index + n) of the gray-code sequence, row-major
+ * n x dim.  Replaces SobolEngine.draw_base2 / make_sobol_points
+ * (S/dis/fixed_sobol.py:66-97). */
+int nlo_sobol_draw(const nlo_sobol_table* table, int64_t first_index, int64_t n,
+                   int32_t out_kind, void* d_out, hipStream_t stream);
+
+/* ----------------------------------------------------------------- VEGAS */
+
+/* Map unit-cube points through an importance grid: bin = min(floor(u B), B - 1),
+ * frac = u B - bin, x = lo + width (e[bin] + (e[bin+1] - e[bin]) frac),
+ * jac = prod_d B (e[bin+1] - e[bin]) width_d.  Bins are bit-exact (integer
+ * arithmetic on q = u 2^30 for Sobol input).  Replaces map_importance_grid
+ * (S/dis/fixed_sobol.py:100-113) and the sample step of vegas()
+ * (S/numerics/integration.py:392-396).  d_bin may be NULL; bins <= 255. */
+int nlo_vegas_map(int32_t dim, int32_t bins, int64_t n, const double* d_u,
+                  const double* d_edges, const double* d_lo, const double* d_width,
+                  double* d_x, double* d_jac, uint8_t* d_bin, hipStream_t stream);
+
+/* Deterministic per-bin sum |w| and hit count over n samples
+ * (hist, count: dim x bins, double).  Per-block 64-bit fixed-point sums
+ * (exact, order-free) followed by a fixed-order tree over blocks.  Replaces
+ * vegas_histogram (K/integration.cu:773-861). */
+size_t nlo_vegas_histogram_workspace(int32_t dim, int32_t bins, int64_t n);
+int nlo_vegas_histogram(int32_t dim, int32_t bins, int64_t n, const uint8_t* d_bin,
+                        const double* d_absw, double* d_hist, double* d_count,
+                        void* d_workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* Grid refinement for `rows` independent grids (rows x dim x (bins + 1)):
+ * importance = hist / max(count, 1), 3-point smoothing (2-point at the ends),
+ * + eps, cumulative sum, linspace targets, right-bisection, linear edge
+ * interpolation, edges 0 and 1 pinned.  Replaces _adapt_vegas_edges
+ * (S/numerics/integration.py:804-866) / vegas_adapt_edges (K/integration.cu:259-377). */
+int nlo_vegas_adapt(int32_t rows, int32_t dim, int32_t bins, const double* d_edges,
+                    const double* d_hist, const double* d_count, double eps,
+                    double* d_edges_out, hipStream_t stream);
+
+/* --------------------------------------------------------------- splines */
+
+/* Size in doubles (or floats) of one spline table for R r-nodes:
+ * [r_min, r_max, log r (R), N records (R-1) x {a,b,c,d}, S records (R-1) x {a,b,c,d}]. */
+#define NLO_SPLINE_WORDS(R) (2 + (R) + 8 * ((R) - 1))
+
+/* Natural cubic log-log splines of N (clamped to [0,1] / force_positive) and
+ * S = 1 - N on log r, one FP64 Thomas solve per row, plus an FP32 copy of the
+ * coefficients (d_table32 may be NULL).  Replaces
+ * evolution_loglog_splines_mixed (K/interpolation.cu:374-451) and
+ * _evolution_interpolators (S/bk/evolution.py:536-560). */
+int nlo_bk_spline_build(int32_t R, const double* d_r, const double* d_N, int32_t force_positive,
+                        double* d_table64, float* d_table32, hipStream_t stream);
+
+/* ------------------------------------------------------------- BK terms */
+
+/* NLO_PREC_FP64: the reference's K2/Kf formula (S/bk_kernels/k2.py, kf.py)
+ *   in FP64, FP64 sums: the plain FP64 path.
+ * NLO_PREC_FP32: FP32 integrand with K2/Kf rewritten without the 1/Delta
+ *   cancellation (Delta = (X Y')^2 - (X' Y)^2) and dipoles in N = 1 - S
+ *   form, so no FP64 band is needed; per-lane and cross-lane sums in FP64.
+ * NLO_PREC_FP64_STABLE: the rewritten kernels in FP64 (the accurate FP64
+ *   oracle on the GPU). */
+#define NLO_PREC_FP64 0
+#define NLO_PREC_FP32 1
+#define NLO_PREC_FP64_STABLE 2
+
+/* K1 on the production fixed rule for every parent r_i, i = 1..R-1, with
+ * N_i <= 0.99999 (others: 0): (R-1) log-r panels x GL(nr) in log z x GL(nt)
+ * in v, theta = pi v^2, FP64.  d_k1: R doubles.  Replaces
+ * mixed_fixed_grid_integrals (K/bk/k1/kernel.cu:2321-2418) /
+ * _batched_k1_component (S/bk/evolution.py:491-533). */
+int nlo_bk_k1(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+              const nlo_bk_physics* physics, int32_t nr, int32_t nt, double* d_k1,
+              hipStream_t stream);
+
+/* K2 + Kf VEGAS sums for every active parent over Sobol points
+ * [first_index, first_index + n) mapped through the parent's grid
+ * (d_edges: R x 4 x (bins + 1); row 0 unused) on the box
+ * (log z, log z') in [ln(r_0/2), ln(2 r_{R-1})], angles in [0, 2 pi]
+ * (S/bk/derivatives.py:424-431).  d_sums: R x 4 doubles
+ * (sum w, sum w^2, sum w K2-part, sum w Kf-part); inactive parents get 0.
+ * d_hist / d_count (R x 4 x bins, may be NULL) receive the adaptation
+ * histograms.  Replaces bk_nlo_mixed_vegas_summaries
+ * (K/bk/k2_kf/kernel.cu:877-1062) batched over parents. */
+size_t nlo_bk_nlo_workspace(int32_t R, int32_t bins, int64_t n, int32_t want_hist);
+int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+               const float* d_table32, const double* d_edges, int32_t bins,
+               const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision,
+               const nlo_bk_physics* physics, double* d_sums, double* d_hist, double* d_count,
+               void* d_workspace, size_t workspace_bytes, hipStream_t stream);
+
+/* dN/dy = K1 + sum_w / n for active parents, 0 elsewhere, non-finite -> 0
+ * (S/bk/evolution.py:340-351).  d_k2kf (R, may be NULL) receives sum_w / n. */
+int nlo_bk_rhs_finalize(int32_t R, const double* d_N, const double* d_k1, const double* d_sums,
+                        int64_t n, double* d_dndy, double* d_k2kf, hipStream_t stream);
+
+/* ------------------------------------------------------- rapidity driver */
+
+#define NLO_RK_EULER 0    /* N + h K(N)                                          */
+#define NLO_RK_MIDPOINT 1 /* N + h K(N + h/2 K(N))  (S/bk/evolution.py:265-276)  */
+#define NLO_RK_KUTTA3 2   /* (K1 + 4 K2 + K3)/6      (S/bk/evolution.py:190-236) */
+#define NLO_RK_CLASSIC4 3 /* classical fourth order                              */
+
+typedef struct nlo_bk_evolver_desc {
+    int32_t R;            /* r nodes */
+    int32_t bins;         /* VEGAS bins per dimension */
+    int64_t n_samples;    /* Sobol points per parent per right-hand side */
+    int64_t first_index;  /* first Sobol index of the production point set */
+    int32_t precision;    /* NLO_PREC_* */
+    int32_t rk_method;    /* NLO_RK_* */
+    double step;          /* rapidity step h */
+    int32_t force_positive; /* clamp N to [0, 1] after each step */
+    int32_t nr, nt;       /* K1 Gauss-Legendre orders (8, 24) */
+    int32_t nlo;          /* include K2 + Kf */
+    nlo_bk_physics physics;
+    nlo_sobol_table sobol;
+} nlo_bk_evolver_desc;
+
+typedef struct nlo_bk_evolver nlo_bk_evolver;
+
+/* One RK step per call, replayed from a hipGraph captured at creation.  The
+ * evolver state N (R doubles) lives in the workspace; nlo_bk_evolver_state
+ * returns its device address.  d_edges (R x 4 x (bins + 1)) is read by every
+ * step (frozen grids) and must outlive the evolver. */
+size_t nlo_bk_evolver_workspace(const nlo_bk_evolver_desc* desc);
+int nlo_bk_evolver_create(const nlo_bk_evolver_desc* desc, const double* d_r,
+                          const double* d_edges, void* d_workspace, size_t workspace_bytes,
+                          hipStream_t stream, nlo_bk_evolver** out);
+double* nlo_bk_evolver_state(nlo_bk_evolver* ev);
+/* Device int32 flag, non-zero once any step produced a non-finite N
+ * (S/bk/evolution.py:152-153 raises FloatingPointError). */
+int32_t* nlo_bk_evolver_nonfinite_flag(nlo_bk_evolver* ev);
+/* Last step's dN/dy stages: d_k1 (R) and d_k2kf (R) of the final stage. */
+const double* nlo_bk_evolver_k1(nlo_bk_evolver* ev);
+const double* nlo_bk_evolver_k2kf(nlo_bk_evolver* ev);
+int nlo_bk_evolver_step(nlo_bk_evolver* ev, hipStream_t stream);
+int nlo_bk_evolver_destroy(nlo_bk_evolver* ev);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

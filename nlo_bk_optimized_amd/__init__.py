@@ -1,0 +1,39 @@
+"""MI355X (gfx950) implementation of the NLO BK right-hand side and rapidity
+driver, drop-in for the reference's BK entry points.
+
+The numerics run in ``libnlo_gfx950.so`` (hand-written HIP kernels behind a C
+ABI, ``include/nlo_gfx950.h``); this package is the Python surface of the
+reference (S/__init__.py of torch_nlo_clean/src/nlo_torch).
+"""
+
+from nlo_bk_optimized_amd.bk import (
+    BKConfig,
+    BKOrder,
+    DNDYResult,
+    ICDataFile,
+    IntegrationMethod,
+    MV,
+    ResummationCoupling,
+    RunningCouplingLO,
+    RunningCouplingNLO,
+    compute_dndy,
+    solve_bk,
+)
+from nlo_bk_optimized_amd.dipole import DipoleTable, load_bk_table, save_bk_table
+
+__all__ = [
+    "BKConfig",
+    "BKOrder",
+    "DNDYResult",
+    "DipoleTable",
+    "ICDataFile",
+    "IntegrationMethod",
+    "MV",
+    "ResummationCoupling",
+    "RunningCouplingLO",
+    "RunningCouplingNLO",
+    "compute_dndy",
+    "load_bk_table",
+    "save_bk_table",
+    "solve_bk",
+]

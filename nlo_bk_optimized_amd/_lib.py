@@ -1,0 +1,173 @@
+"""ctypes binding of ``libnlo_gfx950.so`` (the C ABI in include/nlo_gfx950.h).
+
+PyTorch supplies device memory and the current HIP stream; every numeric
+result comes from the library's HIP kernels.  A missing library is an error:
+there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import torch
+
+LIBRARY = Path(__file__).resolve().parent / "libnlo_gfx950.so"
+
+NLO_OK = 0
+NLO_SOBOL_U32, NLO_SOBOL_F32, NLO_SOBOL_F64 = 0, 1, 2
+NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
+NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4 = 0, 1, 2, 3
+SOBOL_MAXDIM = 8
+SOBOL_BITS = 30
+
+
+def spline_words(R: int) -> int:
+    """NLO_SPLINE_WORDS(R): [r_min, r_max, ln r (R), N records, S records]."""
+
+    return 2 + R + 8 * (R - 1)
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [
+        ("nc", ctypes.c_double),
+        ("nf", ctypes.c_double),
+        ("lambda_qcd", ctypes.c_double),
+        ("c2", ctypes.c_double),
+        ("ksub", ctypes.c_double),
+        ("minr", ctypes.c_double),
+        ("symmetrize", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class SobolTable(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("direction", (ctypes.c_uint32 * SOBOL_BITS) * SOBOL_MAXDIM),
+        ("shift", ctypes.c_uint32 * SOBOL_MAXDIM),
+        ("first", ctypes.c_uint32 * SOBOL_MAXDIM),
+    ]
+
+
+class EvolverDesc(ctypes.Structure):
+    _fields_ = [
+        ("R", ctypes.c_int32),
+        ("bins", ctypes.c_int32),
+        ("n_samples", ctypes.c_int64),
+        ("first_index", ctypes.c_int64),
+        ("precision", ctypes.c_int32),
+        ("rk_method", ctypes.c_int32),
+        ("step", ctypes.c_double),
+        ("force_positive", ctypes.c_int32),
+        ("nr", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("nlo", ctypes.c_int32),
+        ("physics", Physics),
+        ("sobol", SobolTable),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_i64 = ctypes.c_int64
+_f64 = ctypes.c_double
+_size = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+SIGNATURES: dict[str, tuple] = {
+    "nlo_api_version": (ctypes.c_int, []),
+    "nlo_last_error": (ctypes.c_char_p, []),
+    "nlo_gauss_legendre": (ctypes.c_int, [_i32, _vp, _vp]),
+    "nlo_sobol_draw": (ctypes.c_int, [ctypes.POINTER(SobolTable), _i64, _i64, _i32, _vp, _vp]),
+    "nlo_vegas_map": (ctypes.c_int, [_i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "nlo_vegas_histogram_workspace": (_size, [_i32, _i32, _i64]),
+    "nlo_vegas_histogram": (ctypes.c_int, [_i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _size, _vp]),
+    "nlo_vegas_adapt": (ctypes.c_int, [_i32, _i32, _i32, _vp, _vp, _vp, _f64, _vp, _vp]),
+    "nlo_bk_spline_build": (ctypes.c_int, [_i32, _vp, _vp, _i32, _vp, _vp, _vp]),
+    "nlo_bk_k1": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(Physics), _i32, _i32, _vp, _vp]),
+    "nlo_bk_nlo_workspace": (_size, [_i32, _i32, _i64, _i32]),
+    "nlo_bk_nlo": (
+        ctypes.c_int,
+        [
+            _i32,
+            _vp,
+            _vp,
+            _vp,
+            _vp,
+            _vp,
+            _i32,
+            ctypes.POINTER(SobolTable),
+            _i64,
+            _i64,
+            _i32,
+            ctypes.POINTER(Physics),
+            _vp,
+            _vp,
+            _vp,
+            _vp,
+            _size,
+            _vp,
+        ],
+    ),
+    "nlo_bk_rhs_finalize": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "nlo_bk_evolver_workspace": (_size, [ctypes.POINTER(EvolverDesc)]),
+    "nlo_bk_evolver_create": (
+        ctypes.c_int,
+        [ctypes.POINTER(EvolverDesc), _vp, _vp, _vp, _size, _vp, ctypes.POINTER(_vp)],
+    ),
+    "nlo_bk_evolver_state": (_vp, [_vp]),
+    "nlo_bk_evolver_nonfinite_flag": (_vp, [_vp]),
+    "nlo_bk_evolver_k1": (_vp, [_vp]),
+    "nlo_bk_evolver_k2kf": (_vp, [_vp]),
+    "nlo_bk_evolver_step": (ctypes.c_int, [_vp, _vp]),
+    "nlo_bk_evolver_destroy": (ctypes.c_int, [_vp]),
+}
+
+_lib: ctypes.CDLL | None = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load the HIP library once; raise if it has not been built."""
+
+    global _lib
+    if _lib is None:
+        if not LIBRARY.exists():
+            raise RuntimeError(
+                f"{LIBRARY} is missing: build it with `python -m nlo_bk_optimized_amd.build` "
+                "(there is no CPU fallback)"
+            )
+        handle = ctypes.CDLL(str(LIBRARY))
+        for name, (restype, argtypes) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = restype
+            fn.argtypes = argtypes
+        _lib = handle
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    """Raise RuntimeError for a non-zero status, with the library's message."""
+
+    if status != NLO_OK:
+        message = lib().nlo_last_error().decode(errors="replace")
+        raise RuntimeError(f"{what or 'libnlo_gfx950'} failed (status {status}): {message}")
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def stream_handle(device: torch.device | None = None) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def require_cuda(t: torch.Tensor, dtype: torch.dtype, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
+    if t.dtype != dtype:
+        raise TypeError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")

@@ -1,0 +1,222 @@
+"""BK radial grid, VEGAS grid adaptation and rapidity stepping on the GPU.
+
+Entry points mirror S/bk/evolution.py:33-82 of the reference:
+``solve_bk(initial_condition, maxy, config, *, device, dtype, seed) -> DipoleTable``
+and ``compute_dndy(...) -> DNDYResult``.  The right-hand side
+(S/bk/evolution.py:354-488) is evaluated by libnlo_gfx950: K1 on the fixed
+production Gauss-Legendre rule, K2 + Kf as VEGAS sums over scrambled Sobol
+points (seed ``seed``) mapped through per-parent importance grids that are
+adapted on the initial condition and then frozen.  Each RK step is one
+hipGraph replay.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import torch
+
+from nlo_bk_optimized_amd import device as dev
+from nlo_bk_optimized_amd.bk.config import BKConfig, require_gpu_supported
+from nlo_bk_optimized_amd.bk.initial_conditions import MV, ICDataFile
+from nlo_bk_optimized_amd.dipole.table import DipoleTable
+
+
+@dataclass(frozen=True, slots=True)
+class DNDYResult:
+    r: torch.Tensor
+    K1: torch.Tensor
+    K2_Kf: torch.Tensor
+    N: torch.Tensor
+
+
+def _r_grid(config: BKConfig, *, device: torch.device | str, dtype: torch.dtype) -> torch.Tensor:
+    """r_i = MINR * step^i, step = (MAXR / MINR)^(1 / RPOINTS) (S/bk/evolution.py:584-592)."""
+
+    rind = torch.arange(config.RPOINTS + 1, dtype=dtype, device=device)
+    step = (config.MAXR / config.MINR) ** (1 / config.RPOINTS)
+    return config.MINR * torch.pow(rind.new_tensor(step), rind)
+
+
+def _resolve_device(device: torch.device | str | None) -> torch.device:
+    d = torch.device("cuda" if device is None else device)
+    if d.type != "cuda":
+        raise ValueError(
+            f"device={d}: the MI355X implementation runs on the GPU only (HIP device 'cuda')"
+        )
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device is available")
+    return d
+
+
+@dataclass
+class BKSetup:
+    """Grid, initial amplitude, Sobol table and adapted VEGAS grids."""
+
+    r: torch.Tensor  # float64 [R]
+    N0: torch.Tensor  # float64 [R]
+    table: object
+    edges: torch.Tensor  # float64 [R, 4, bins + 1]
+    n_samples: int
+
+
+def uniform_edges(R: int, bins: int, device: torch.device) -> torch.Tensor:
+    e = torch.linspace(0, 1, bins + 1, dtype=torch.float64, device=device)
+    return e.expand(R, 4, bins + 1).contiguous()
+
+
+def adapt_grids(
+    r: torch.Tensor,
+    N: torch.Tensor,
+    table,
+    config: BKConfig,
+    *,
+    iterations: int,
+    n_points: int,
+    precision: str,
+) -> torch.Tensor:
+    """VEGAS refinement of every parent's grid on amplitude N.
+
+    Iteration k histograms |w| over Sobol points [(k + 1) n, (k + 2) n) and
+    refines the edges as S/numerics/integration.py:804-866 does; the
+    production estimate later uses points [0, n).
+    """
+
+    R = r.numel()
+    edges = uniform_edges(R, config.MC_BINS, r.device)
+    if iterations == 0 or not config.Order.has_nlo_kernels:
+        return edges
+    t64, t32 = dev.spline_build(r, N, config.FORCE_POSITIVE_N)
+    eps = dev.dtype_epsilon(torch.float64)
+    for k in range(iterations):
+        _, hist, count = dev.bk_nlo(
+            r, N, t64, t32, edges, table, n_points, config,
+            first_index=(k + 1) * n_points, precision=precision, histogram=True,
+        )
+        edges = dev.vegas_adapt(edges, hist, count, eps)
+    return edges
+
+
+def prepare(
+    initial_condition: MV | ICDataFile,
+    config: BKConfig,
+    *,
+    device: torch.device,
+    seed: int | None,
+) -> BKSetup:
+    require_gpu_supported(config)
+    r = _r_grid(config, device="cpu", dtype=torch.float64).to(device)
+    N0 = initial_condition.dipole_amplitude(
+        r if not isinstance(initial_condition, ICDataFile) else r.to(initial_condition.r.device, initial_condition.r.dtype)
+    ).to(device=device, dtype=torch.float64)
+    table = dev.sobol_table(4, 0 if seed is None else int(seed))
+    n = int(config.MCINTPOINTS)
+    n_adapt = int(config.VEGAS_ADAPT_POINTS or n)
+    # adaptation blocks must start at multiples of 64 Sobol indices
+    n_adapt_aligned = (n_adapt + 63) // 64 * 64
+    if (config.VEGAS_ADAPT_ITERATIONS + 1) * max(n_adapt_aligned, n) > 2**30:
+        raise ValueError("MCINTPOINTS x adaptation iterations exceed the 2^30 Sobol points")
+    edges = adapt_grids(
+        r, N0, table, config,
+        iterations=config.VEGAS_ADAPT_ITERATIONS,
+        n_points=n_adapt_aligned,
+        precision=config.PRECISION,
+    )
+    return BKSetup(r=r, N0=N0, table=table, edges=edges, n_samples=n)
+
+
+def rhs(setup: BKSetup, N: torch.Tensor, config: BKConfig) -> tuple[torch.Tensor, torch.Tensor]:
+    """K1 and K2 + Kf at amplitude N (S/bk/evolution.py:354-488)."""
+
+    t64, t32 = dev.spline_build(setup.r, N, config.FORCE_POSITIVE_N)
+    k1 = dev.bk_k1(setup.r, N, t64, config)
+    if config.Order.has_nlo_kernels:
+        sums, _, _ = dev.bk_nlo(
+            setup.r, N, t64, t32, setup.edges, setup.table, setup.n_samples, config,
+            precision=config.PRECISION,
+        )
+        n = setup.n_samples
+    else:
+        sums = torch.zeros((N.numel(), 4), dtype=torch.float64, device=N.device)
+        n = 1
+    _, k2kf = dev.rhs_finalize(N, k1, sums, n)
+    return k1, k2kf
+
+
+def solve_bk(
+    initial_condition: MV | ICDataFile,
+    maxy: float,
+    config: BKConfig = BKConfig(),
+    *,
+    device: torch.device | str = "cuda",
+    dtype: torch.dtype = torch.float32,
+    seed: int | None = None,
+) -> DipoleTable:
+    """Evolve through full output steps and return the final in-memory table."""
+
+    if isinstance(initial_condition, ICDataFile):
+        dtype = initial_condition.r.dtype
+    d = _resolve_device(device)
+    setup = prepare(initial_condition, config, device=d, seed=seed)
+    steps = 0
+    y = 0.0
+    ys = [0.0]
+    tolerance = 1e-12 * max(1.0, abs(maxy), abs(config.DE_SOLVER_STEP))
+    while True:  # output rapidities as S/bk/evolution.py:108-159 produces them
+        y = y + config.DE_SOLVER_STEP
+        steps += 1
+        ys.append(y)
+        if y >= maxy - tolerance:
+            break
+    rk = "euler" if config.EULER_METHOD else config.RK_METHOD
+    evolver = dev.Evolver(
+        setup.r, setup.edges, setup.table, config,
+        n_samples=setup.n_samples, precision=config.PRECISION, rk_method=rk,
+        step=config.DE_SOLVER_STEP, nlo=config.Order.has_nlo_kernels,
+    )
+    try:
+        history = torch.empty((steps + 1, setup.r.numel()), dtype=torch.float64, device=d)
+        history[0].copy_(setup.N0)
+        evolver.state.copy_(setup.N0)
+        for t in range(steps):
+            evolver.step()
+            history[t + 1].copy_(evolver.state)
+        if int(evolver.flag.item()) != 0:
+            raise FloatingPointError("non-finite BK amplitude during the rapidity evolution")
+    finally:
+        evolver.close()
+    r_out = setup.r.to(dtype)
+    return DipoleTable(
+        r=r_out,
+        y=torch.tensor(ys, dtype=dtype, device=d),
+        N=history.to(dtype),
+        x0=initial_condition.x0,
+    )
+
+
+def compute_dndy(
+    initial_condition: MV | ICDataFile,
+    config: BKConfig = BKConfig(),
+    *,
+    device: torch.device | str = "cuda",
+    dtype: torch.dtype = torch.float32,
+    seed: int | None = None,
+) -> DNDYResult:
+    """Diagnostic columns at the initial rapidity (S/bk/evolution.py:61-82)."""
+
+    if isinstance(initial_condition, ICDataFile):
+        dtype = initial_condition.r.dtype
+    d = _resolve_device(device)
+    setup = prepare(initial_condition, config, device=d, seed=seed)
+    k1, k2kf = rhs(setup, setup.N0, config)
+    keep = torch.arange(setup.r.numel(), device=d) > 0
+    keep = keep & (setup.N0 <= 0.99999)
+    return DNDYResult(
+        r=setup.r[keep].to(dtype),
+        K1=k1[keep].to(dtype),
+        K2_Kf=k2kf[keep].to(dtype),
+        N=setup.N0[keep].to(dtype),
+    )
+
+
+__all__ = ["BKSetup", "DNDYResult", "adapt_grids", "compute_dndy", "prepare", "rhs", "solve_bk"]

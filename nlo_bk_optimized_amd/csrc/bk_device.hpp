@@ -1,0 +1,370 @@
+// Device-side BK physics: coupling, log-log splines held in LDS, the K1
+// integrand and the K2 + Kf integrand in FP64 (reference formula) and FP32
+// (cancellation-free formula).  Reference prefixes: S/ = torch_nlo_clean/src/
+// nlo_torch/, K/ = S/custom_kernels/.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace nlo {
+
+constexpr double kPi = 3.14159265358979323846;
+
+// Physics constants derived once on the host from nlo_bk_physics.
+struct BkConsts {
+    double nc, nf, ksub, minr;
+    double b0;          // (11 Nc - 2 Nf) / (12 pi)
+    double log_mu0;     // 2 / 0.2 * ln 2.5
+    double scale_num;   // 4 C2
+    double lambda_qcd;
+    double log_scale0;  // ln(4 C2 / Lambda^2)
+    double norm;        // 1 / (8 pi^4)
+    int symmetrize;
+};
+
+// -------------------------------------------------------------- coupling
+
+// S/coupling.py:81-97: alpha = 1 / (b0 * 0.2 * logaddexp(2/0.2 ln 2.5,
+// ln(4 C2 / (r^2 Lambda^2)) / 0.2)).
+__device__ __forceinline__ double alpha_s64(double r, const BkConsts& c) {
+    const double scale = c.scale_num / (r * r * c.lambda_qcd * c.lambda_qcd);
+    const double right = log(scale) / 0.2;
+    const double hi = fmax(c.log_mu0, right);
+    return 1.0 / (c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right))))));
+}
+
+// Same coupling from ln r (FP32): right = 5 (ln(4 C2 / Lambda^2) - 2 ln r).
+__device__ __forceinline__ float alpha_s32_log(float log_r, float b0, float log_mu0,
+                                               float log_scale0) {
+    const float right = 5.0f * (log_scale0 - 2.0f * log_r);
+    const float hi = fmaxf(log_mu0, right);
+    return 1.0f / (b0 * (0.2f * (hi + log1pf(expf(-fabsf(log_mu0 - right))))));
+}
+
+// --------------------------------------------------------------- splines
+
+// A log-log spline staged in LDS: nodes x[n] = ln r_i and per-interval
+// records {a, b, c, d} (structure of arrays would need four LDS reads).
+template <typename T>
+struct LdsSpline {
+    const T* x;
+    const T* rec;  // 4 (n - 1)
+    int n;
+    T x0, inv_h;   // ln r_0 and (n - 1) / (ln r_{n-1} - ln r_0)
+    T log_min, log_max;
+};
+
+// searchsorted(x, v, right=True) - 1 clamped to [0, n - 2]
+// (S/numerics/interpolation.py:55-56): the uniform-grid estimate, then exact
+// correction against the nodes (one step on the log-uniform evolution grid).
+template <typename T>
+__device__ __forceinline__ int spline_interval(const LdsSpline<T>& s, T v) {
+    int k = (int)((v - s.x0) * s.inv_h);
+    k = k < 0 ? 0 : (k > s.n - 2 ? s.n - 2 : k);
+    while (k > 0 && v < s.x[k]) --k;
+    while (k < s.n - 2 && v >= s.x[k + 1]) ++k;
+    return k;
+}
+
+template <typename T>
+__device__ __forceinline__ T spline_poly(const LdsSpline<T>& s, T v) {
+    const int k = spline_interval(s, v);
+    const T dx = v - s.x[k];
+    const T* c = s.rec + 4 * k;
+    return c[0] + dx * (c[1] + dx * (c[2] + dx * c[3]));
+}
+
+// _evaluate_loglog (S/bk/derivatives.py:761-772) on a distance d.
+__device__ __forceinline__ double loglog64(const LdsSpline<double>& s, double r_min, double r_max,
+                                          double d, double underflow, double overflow) {
+    const double ev = fmin(fmax(d, r_min), r_max);
+    double out = exp(spline_poly(s, log(ev)));
+    out = isfinite(out) ? out : 0.0;
+    out = d < r_min ? underflow : out;
+    return d > r_max ? overflow : out;
+}
+
+// Same from ln d: the clamp and the range tests done in log space.
+template <typename T>
+__device__ __forceinline__ T loglog_from_log(const LdsSpline<T>& s, T log_d, T underflow,
+                                            T overflow) {
+    const T lv = fmin(fmax(log_d, s.log_min), s.log_max);
+    T out = exp(spline_poly(s, lv));
+    out = isfinite(out) ? out : T(0);
+    out = log_d < s.log_min ? underflow : out;
+    return log_d > s.log_max ? overflow : out;
+}
+
+// ------------------------------------------------------- K1 (FP64 only)
+
+// Kernel_lo (S/bk_kernels/k1.py:20-155) for BALITSKY_LO, RESUM_RC_SMALLEST,
+// NLO_RESUM_DLOG_SLOG, times the dipole of _lo_integrand
+// (S/bk/derivatives.py:445-468).  X^2 = (r - z)^2 + 4 r z sin^2(theta / 2)
+// (the form K/bk/k1/kernel.cu:478-479 uses; it keeps X accurate near X -> 0).
+__device__ double k1_integrand64(double r, double z, double theta, double n_r, double alpha_r,
+                                 const LdsSpline<double>& sN, double r_min, double r_max,
+                                 const BkConsts& c) {
+    const double sh = sin(theta / 2.0);
+    const double xsq = (r - z) * (r - z) + 4.0 * r * z * sh * sh;
+    const bool invalid = xsq < 1e-40 || z < 1e-20 || r < 1e-20;
+    if (invalid) return 0.0;
+    const double X = sqrt(xsq);
+    const double Y = z;
+    const double n_x = fmin(loglog64(sN, r_min, r_max, X, 0.0, 1.0), 1.0);
+    const double n_y = fmin(loglog64(sN, r_min, r_max, Y, 0.0, 1.0), 1.0);
+    const double dipole = n_x + n_y - n_r - n_x * n_y;
+
+    const double a_x = alpha_s64(X, c);
+    const double a_y = alpha_s64(Y, c);
+    // alpha(min(X, Y, r)) without a fourth evaluation
+    const double a_min = fmin(X, Y) <= r ? (X <= Y ? a_x : a_y) : alpha_r;
+    double lo = c.nc / (2.0 * kPi * kPi) * alpha_r *
+                (r * r / (X * X * Y * Y) + (a_y / a_x - 1.0) / (Y * Y) + (a_x / a_y - 1.0) / (X * X));
+    lo = isfinite(lo) ? lo : 0.0;
+
+    const double lx = log(X / r), ly = log(Y / r);
+    const double xl = 4.0 * lx * ly;
+    const double as_x = sqrt(a_min * c.nc / kPi * fabs(xl));
+    double dlog;
+    if (as_x == 0.0) {
+        dlog = 1.0;
+    } else {
+        dlog = xl >= 0.0 ? j1(2.0 * as_x) / as_x : cyl_bessel_i1(2.0 * as_x) / as_x;
+    }
+    dlog = r > 1.01 * c.minr ? dlog : 1.0;
+    if (!isfinite(dlog)) return 0.0;
+
+    const double minxy = fmin(X, Y);
+    const double abar = a_min * c.nc / kPi;
+    const double ratio = r / minxy;
+    const double slog = exp(-abar * (11.0 / 12.0) * fabs(log(c.ksub * ratio * ratio)));
+    const double slog_exp = -abar * (11.0 / 12.0) * fabs(2.0 * log(sqrt(c.ksub) * r / minxy));
+    const double rxy = r / (X * Y);
+    const double lo_single = alpha_r * c.nc / (2.0 * kPi * kPi) * rxy * rxy;
+    const double k1fin = lo * alpha_r * c.nc / (4.0 * kPi) *
+                         (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
+    return (dlog * slog * lo - lo_single * slog_exp + k1fin) * dipole;
+}
+
+// -------------------------------------------------- K2 + Kf, FP64 direct
+
+struct NloValue64 {
+    double total, k2, kf;
+};
+
+__device__ __forceinline__ double kernel_nlo64(double r, double X, double Y, double X2, double Y2,
+                                               double Z, double z4) {
+    // S/bk_kernels/k2.py:24-36
+    const double a2 = (X * Y2) * (X * Y2), b2 = (X2 * Y) * (X2 * Y);
+    const double diff = a2 - b2;
+    const double rz = r * Z;
+    const double k = -2.0 / z4 + ((a2 + b2 - 4.0 * rz * rz) / (z4 * diff) + (r * r) * (r * r) / (a2 * diff) +
+                                  r * r / ((X * Y2 * Z) * (X * Y2 * Z))) *
+                                     2.0 * log(X * Y2 / (X2 * Y));
+    return isfinite(k) ? k : 0.0;
+}
+
+__device__ __forceinline__ double kernel_nlo_fermion64(double r, double X, double Y, double X2,
+                                                       double Y2, double Z, double z4, double nf,
+                                                       double nc) {
+    // S/bk_kernels/kf.py:28-38
+    const double a2 = (X * Y2) * (X * Y2), b2 = (X2 * Y) * (X2 * Y);
+    const double rz = r * Z;
+    double k = 2.0 / z4 - (a2 + b2 - rz * rz) / (z4 * (a2 - b2)) * 2.0 * log(X * Y2 / (X2 * Y));
+    k = k * nf / nc;
+    return isfinite(k) ? k : 0.0;
+}
+
+// _nlo_integrand (S/bk/derivatives.py:521-612) times exp(2 ln z + 2 ln z')
+// (S/bk/derivatives.py:335), with the reference's kernels and S dipoles.
+__device__ NloValue64 nlo_integrand64(double r, double lz, double lz2, double tz, double tz2,
+                                      const LdsSpline<double>& sS, double r_min, double r_max,
+                                      const BkConsts& c) {
+    NloValue64 out{0.0, 0.0, 0.0};
+    const double z = exp(lz), z2 = exp(lz2);
+    const double s1 = sin(tz / 2.0), s2 = sin(tz2 / 2.0), s3 = sin((tz - tz2) / 2.0);
+    const double xsq = (r - z) * (r - z) + 4.0 * r * z * s1 * s1;
+    const double x2sq = (r - z2) * (r - z2) + 4.0 * r * z2 * s2 * s2;
+    const double zsq = (z - z2) * (z - z2) + 4.0 * z * z2 * s3 * s3;
+    const double X = sqrt(xsq), X2 = sqrt(x2sq), Z = sqrt(zsq);
+    if (X < 1e-20 || z < 1e-20 || X2 < 1e-20 || z2 < 1e-20 || Z < 1e-20) return out;
+    const double Y = z, Y2 = z2;
+    const double S_X = loglog64(sS, r_min, r_max, X, 1.0, 0.0);
+    const double S_Y = loglog64(sS, r_min, r_max, Y, 1.0, 0.0);
+    const double S_X2 = loglog64(sS, r_min, r_max, X2, 1.0, 0.0);
+    const double S_Y2 = loglog64(sS, r_min, r_max, Y2, 1.0, 0.0);
+    const double S_Z = loglog64(sS, r_min, r_max, Z, 1.0, 0.0);
+    const double dmin = fmin(fmin(fmin(r, X), Y), fmin(fmin(X2, Y2), Z));
+    const double as = alpha_s64(dmin, c);
+    const double z4 = Z * Z * Z * Z;
+    double k = kernel_nlo64(r, X, Y, X2, Y2, Z, z4);
+    double ks = kernel_nlo64(r, X2, Y2, X, Y, Z, z4);
+    double dip = -(S_X * S_Z * S_Y2 - S_X * S_Y);
+    double dips = -(S_X2 * S_Z * S_Y - S_X2 * S_Y2);
+    if (fabs(k) > 1e10 && fabs(dip) < 1e-10) { k = 0.0; dip = 0.0; }
+    if (fabs(ks) > 1e10 && fabs(dips) < 1e-10) { ks = 0.0; dips = 0.0; }
+    double k2r = c.symmetrize ? (k * dip + ks * dips) / 2.0 : k * dip;
+    double kfr = 0.0;
+    if (c.nf > 0.0) {
+        const double kf = kernel_nlo_fermion64(r, X, Y, X2, Y2, Z, z4, c.nf, c.nc);
+        const double kfs = kernel_nlo_fermion64(r, X2, Y2, X, Y, Z, z4, c.nf, c.nc);
+        const double df = S_Y * (S_X2 - S_X);
+        const double dfs = S_Y2 * (S_X - S_X2);
+        kfr = -(c.symmetrize ? (kf * df + kfs * dfs) / 2.0 : kf * df);
+    }
+    const double scale = (as * c.nc) * (as * c.nc) * c.norm * exp(2.0 * lz + 2.0 * lz2);
+    k2r *= scale;
+    kfr *= scale;
+    out.k2 = isfinite(k2r) ? k2r : 0.0;
+    out.kf = isfinite(kfr) ? kfr : 0.0;
+    out.total = out.k2 + out.kf;
+    return out;
+}
+
+// ------------------------------------- K2 + Kf, cancellation-free kernels
+
+// g(L) = L coth L - 1 (even).  |L| < 1: L^2 P(L^2) / Q(L^2) with
+// L cosh L - sinh L = L^3 P(L^2), sinh L = L Q(L^2), both series positive;
+// otherwise |L| + h(|L|) - 1.
+template <typename T>
+__device__ __forceinline__ T coth_minus_one(T L);
+
+template <>
+__device__ __forceinline__ double coth_minus_one<double>(double L) {
+    L = fabs(L);
+    if (L < 1.0) {
+        const double y = L * L;
+        const double p =
+            ((((((((((1.0 / 1175091669949317120000.0) * y + 1.0 / 2554547108585472000.0) * y +
+                    1.0 / 6758061133824000.0) * y + 1.0 / 22230464256000.0) * y +
+                  1.0 / 93405312000.0) * y + 1.0 / 518918400.0) * y + 1.0 / 3991680.0) * y +
+               1.0 / 45360.0) * y + 1.0 / 840.0) * y + 1.0 / 30.0) * y + 1.0 / 3.0;
+        const double q =
+            (((((((((((1.0 / 25852016738884976640000.0) * y + 1.0 / 51090942171709440000.0) * y +
+                     1.0 / 121645100408832000.0) * y + 1.0 / 355687428096000.0) * y +
+                   1.0 / 1307674368000.0) * y + 1.0 / 6227020800.0) * y + 1.0 / 39916800.0) * y +
+                1.0 / 362880.0) * y + 1.0 / 5040.0) * y + 1.0 / 120.0) * y + 1.0 / 6.0) * y + 1.0;
+        return y * p / q;
+    }
+    return L + 2.0 * L / expm1(2.0 * L) - 1.0;
+}
+
+template <>
+__device__ __forceinline__ float coth_minus_one<float>(float L) {
+    L = fabsf(L);
+    if (L < 1.0f) {
+        const float y = L * L;
+        const float p = ((((1.0f / 518918400.0f) * y + 1.0f / 3991680.0f) * y + 1.0f / 45360.0f) * y +
+                         1.0f / 840.0f) * y + 1.0f / 30.0f;
+        const float q = ((((1.0f / 39916800.0f) * y + 1.0f / 362880.0f) * y + 1.0f / 5040.0f) * y +
+                         1.0f / 120.0f) * y + 1.0f / 6.0f;
+        return y * (p * y + 1.0f / 3.0f) / (q * y + 1.0f);
+    }
+    return L + 2.0f * L / expm1f(2.0f * L) - 1.0f;
+}
+
+// ln(X / Y) with X^2 = (r - z)^2 + 4 r z sin^2(theta / 2), Y = z:
+// (X^2 - z^2) / z^2 = rho (rho - 2 cos theta), rho = r / z.
+template <typename T>
+__device__ __forceinline__ T log_x_over_y(T r, T z, T log_z, T cos_t, T xsq);
+
+template <>
+__device__ __forceinline__ double log_x_over_y<double>(double r, double z, double log_z,
+                                                       double cos_t, double xsq) {
+    const double rho = r / z;
+    const double u = rho * (rho - 2.0 * cos_t);
+    return fabs(u) < 0.5 ? 0.5 * log1p(u) : 0.5 * log(xsq) - log_z;
+}
+
+template <>
+__device__ __forceinline__ float log_x_over_y<float>(float r, float z, float log_z, float cos_t,
+                                                     float xsq) {
+    const float rho = r / z;
+    const float u = rho * (rho - 2.0f * cos_t);
+    return fabsf(u) < 0.5f ? 0.5f * log1pf(u) : 0.5f * logf(xsq) - log_z;
+}
+
+struct NloValue32 {
+    float total, k2, kf;
+};
+
+// The K2 + Kf integrand of S/bk/derivatives.py:521-612 with the kernels in
+// cancellation-free form (multiplied by J = z^2 z'^2):
+//   K2 J  = 2 A^2 g - 4 h B^2 + h C^2 + 2 L D^2
+//   K2s J = 2 A^2 g - 4 h' D^2 + h' C^2 - 2 L B^2
+//   Kf J  = (-2 A^2 g + h B^2) Nf / Nc,  Kfs J = (-2 A^2 g + h' D^2) Nf / Nc
+// L = ln(X Y' / (X' Y)), g = L coth L - 1, h = 2L / (e^{2L} - 1),
+// h' = h e^{2L}, A = z z' / Z^2, B = r z' / (X' Z), C = r^2 / (X X'),
+// D = r z / (X Z); dipoles in N = 1 - S form.  Every quantity is a ratio of
+// distances, so FP32 neither overflows nor cancels; no FP64 band is needed.
+template <typename T>
+struct NloStable {
+    T total, k2, kf;
+};
+
+template <typename T>
+__device__ __forceinline__ NloStable<T> nlo_integrand_stable(T r, T log_r, T lz, T lz2, T tz, T tz2,
+                                                             const LdsSpline<T>& sN, T b0, T log_mu0,
+                                                             T log_scale0, T nf_over_nc, T nc,
+                                                             T norm, bool symmetrize) {
+    NloStable<T> out{T(0), T(0), T(0)};
+    const T z = exp(lz), z2 = exp(lz2);
+    const T c1 = cos(tz), c2 = cos(tz2);
+    const T s1 = sin(T(0.5) * tz), s2 = sin(T(0.5) * tz2), s3 = sin(T(0.5) * (tz - tz2));
+    const T xsq = (r - z) * (r - z) + T(4) * r * z * s1 * s1;
+    const T x2sq = (r - z2) * (r - z2) + T(4) * r * z2 * s2 * s2;
+    const T zsq = (z - z2) * (z - z2) + T(4) * z * z2 * s3 * s3;
+    const T tiny = sizeof(T) == 4 ? T(1e-37) : T(1e-40);
+    if (!(xsq >= tiny) || !(x2sq >= tiny) || !(zsq >= tiny)) return out;
+    const T X = sqrt(xsq), X2 = sqrt(x2sq), Z = sqrt(zsq);
+    const T lxy = log_x_over_y<T>(r, z, lz, c1, xsq);
+    const T lx2y2 = log_x_over_y<T>(r, z2, lz2, c2, x2sq);
+    const T log_x = lz + lxy, log_x2 = lz2 + lx2y2, log_zz = T(0.5) * log(zsq);
+    // N at the five distances: r_min..r_max, N -> 0 below and 1 above.
+    const T n_x = loglog_from_log(sN, log_x, T(0), T(1));
+    const T n_y = loglog_from_log(sN, lz, T(0), T(1));
+    const T n_x2 = loglog_from_log(sN, log_x2, T(0), T(1));
+    const T n_y2 = loglog_from_log(sN, lz2, T(0), T(1));
+    const T n_z = loglog_from_log(sN, log_zz, T(0), T(1));
+    const T sx = T(1) - n_x, sx2 = T(1) - n_x2;
+    T dip = sx * (n_z + n_y2 - n_y - n_z * n_y2);
+    T dips = sx2 * (n_z + n_y - n_y2 - n_z * n_y);
+
+    const T L = lxy - lx2y2;
+    const T g = coth_minus_one<T>(L);
+    const T e2l = exp(T(2) * L);
+    const T h = L == T(0) ? T(1) : T(2) * L / expm1(T(2) * L);
+    const T hm = h * e2l;
+    const T A = z * z2 / zsq, B = r * z2 / (X2 * Z), C = r * r / (X * X2), D = r * z / (X * Z);
+    const T A2g = T(2) * A * A * g;
+    T k = A2g - T(4) * h * B * B + h * C * C + T(2) * L * D * D;
+    T ks = A2g - T(4) * hm * D * D + hm * C * C - T(2) * L * B * B;
+    // cut of S/bk/derivatives.py:589-594: |K2| > 1e10 with |dipole| < 1e-10
+    const T cut = T(1e10) * exp(T(2) * (lz + lz2));
+    if (fabs(k) > cut && fabs(dip) < T(1e-10)) { k = T(0); dip = T(0); }
+    if (fabs(ks) > cut && fabs(dips) < T(1e-10)) { ks = T(0); dips = T(0); }
+    T k2r = symmetrize ? (k * dip + ks * dips) * T(0.5) : k * dip;
+    T kfr = T(0);
+    if (nf_over_nc > T(0)) {
+        const T kf = (-A2g + h * B * B) * nf_over_nc;
+        const T kfs = (-A2g + hm * D * D) * nf_over_nc;
+        const T df = (T(1) - n_y) * (n_x - n_x2);
+        const T dfs = (T(1) - n_y2) * (n_x2 - n_x);
+        kfr = -(symmetrize ? (kf * df + kfs * dfs) * T(0.5) : kf * df);
+    }
+    const T log_min = fmin(fmin(fmin(log_r, log_x), lz), fmin(fmin(log_x2, lz2), log_zz));
+    const T right = T(5) * (log_scale0 - T(2) * log_min);
+    const T hi = fmax(log_mu0, right);
+    const T as = T(1) / (b0 * (T(0.2) * (hi + log1p(exp(-fabs(log_mu0 - right))))));
+    const T scale = (as * nc) * (as * nc) * norm;
+    k2r *= scale;
+    kfr *= scale;
+    out.k2 = isfinite(k2r) ? k2r : T(0);
+    out.kf = isfinite(kfr) ? kfr : T(0);
+    out.total = out.k2 + out.kf;
+    return out;
+}
+
+}  // namespace nlo

@@ -1,0 +1,46 @@
+// Internal launch helpers shared by the C ABI entry points and the rapidity
+// driver (which captures the same launches into a hipGraph).
+#pragma once
+
+#include "bk_device.hpp"
+
+namespace nlo {
+
+constexpr int kMaxR = 2048;       // r nodes (LDS: FP64 spline + Thomas scratch)
+constexpr int kMaxGaussR = 32;    // K1 radial GL order
+constexpr int kMaxGaussT = 64;    // K1 angular GL order
+
+struct NloPlan {
+    int spt;              // samples per lane
+    int bpp;              // blocks per parent
+    size_t part_bytes;    // per-block sums
+    size_t hist_bytes;    // per-block histograms (0 without histogram)
+    size_t workspace;     // total workspace
+};
+
+BkConsts make_consts(const nlo_bk_physics& p);
+int validate_physics(const nlo_bk_physics* p);
+// Raises the dynamic-LDS limit of the kernels that may need more than 64 KiB
+// (once per process; must run before any stream capture).
+int init_kernel_attributes();
+
+NloPlan nlo_plan(int R, int bins, int64_t n, bool hist);
+
+int launch_spline_build(int R, const double* r, const double* N, int force_positive, double* t64,
+                        float* t32, hipStream_t stream);
+int launch_k1(int R, const double* r, const double* N, const double* t64, const BkConsts& c, int nr,
+              int nt, double* k1, hipStream_t stream);
+int launch_nlo(int R, const double* r, const double* N, const double* t64, const float* t32,
+               const double* edges, int bins, const nlo_sobol_table& sobol, int64_t first, int64_t n,
+               int precision, const BkConsts& c, double* sums, double* hist, double* count, void* ws,
+               size_t ws_bytes, hipStream_t stream);
+int launch_rhs_finalize(int R, const double* N, const double* k1, const double* sums, int64_t n,
+                        double* dndy, double* k2kf, hipStream_t stream);
+int launch_rk_stage(int R, const double* N, const double* K0, const double* K1, const double* K2,
+                    double a0, double a1, double a2, double h, double* out, hipStream_t stream);
+int launch_rk_final(int R, double* N, const double* K0, const double* K1, const double* K2,
+                    const double* K3, double b0, double b1, double b2, double b3, double h,
+                    int force_positive, int32_t* flag, hipStream_t stream);
+int launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t stream);
+
+}  // namespace nlo

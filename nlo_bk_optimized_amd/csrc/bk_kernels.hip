@@ -1,0 +1,763 @@
+// BK right-hand side on gfx950: spline build, K1 fixed rule, K2 + Kf VEGAS
+// sums with in-kernel Sobol points, deterministic reductions, RHS finalize and
+// RK stage updates.  Reference prefixes: S/ = torch_nlo_clean/src/nlo_torch/,
+// K/ = S/custom_kernels/.
+#include <cmath>
+#include <mutex>
+#include <type_traits>
+
+#include "bk_device.hpp"
+#include "bk_internal.hpp"
+
+namespace nlo {
+
+BkConsts make_consts(const nlo_bk_physics& p) {
+    BkConsts c{};
+    c.nc = p.nc;
+    c.nf = p.nf;
+    c.ksub = p.ksub;
+    c.minr = p.minr;
+    c.b0 = (11.0 * p.nc - 2.0 * p.nf) / (12.0 * kPi);
+    c.log_mu0 = 2.0 / 0.2 * std::log(2.5);
+    c.scale_num = 4.0 * p.c2;
+    c.lambda_qcd = p.lambda_qcd;
+    c.log_scale0 = std::log(4.0 * p.c2 / (p.lambda_qcd * p.lambda_qcd));
+    c.norm = 1.0 / (8.0 * kPi * kPi * kPi * kPi);
+    c.symmetrize = p.symmetrize;
+    return c;
+}
+
+int validate_physics(const nlo_bk_physics* p) {
+    NLO_REQUIRE(p, "null physics");
+    NLO_REQUIRE(p->nc > 0 && p->lambda_qcd > 0 && p->c2 > 0 && p->ksub > 0 && p->minr > 0,
+                "physics constants must be positive");
+    if (!(p->nf == 0.0 || p->nf == 3.0)) {
+        set_error("NF=%g: only the smooth coupling (NF <= 3, S/coupling.py:20-21) is implemented",
+                  p->nf);
+        return NLO_ERR_UNSUPPORTED;
+    }
+    return NLO_OK;
+}
+
+namespace {
+
+// ------------------------------------------------------------ LDS staging
+
+template <typename T>
+__device__ void stage_spline(const T* table, int R, int row, T* x, T* rec, LdsSpline<T>& s) {
+    // table: [r_min, r_max, ln r (R), N records 4 (R-1), S records 4 (R-1)]
+    const T* lx = table + 2;
+    const T* src = table + 2 + R + (size_t)row * 4 * (R - 1);
+    for (int i = threadIdx.x; i < R; i += blockDim.x) x[i] = lx[i];
+    for (int i = threadIdx.x; i < 4 * (R - 1); i += blockDim.x) rec[i] = src[i];
+    s.x = x;
+    s.rec = rec;
+    s.n = R;
+    s.x0 = lx[0];
+    s.inv_h = T(R - 1) / (lx[R - 1] - lx[0]);
+    s.log_min = lx[0];
+    s.log_max = lx[R - 1];
+}
+
+// -------------------------------------------------------------- splines
+
+// S/numerics/interpolation.py:27-48, :124-149 and S/bk/evolution.py:548-560:
+// one wave; lanes 0 and 1 run the Thomas sweeps of the N and S rows.
+__global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* __restrict__ r,
+                                                          const double* __restrict__ N,
+                                                          int force_positive, double* __restrict__ t64,
+                                                          float* __restrict__ t32) {
+    extern __shared__ double sh[];
+    double* x = sh;
+    double* ly = x + R;       // 2 R
+    double* up = ly + 2 * R;  // 2 R
+    double* so = up + 2 * R;  // 2 R
+    for (int i = threadIdx.x; i < R; i += blockDim.x) {
+        x[i] = log(r[i]);
+        double v = N[i] > 1.0 ? 1.0 : N[i];
+        if (force_positive && v < 0.0) v = 0.0;
+        ly[i] = v > 0.0 ? log(v) : -40.0;
+        double s = 1.0 - N[i];
+        if (s < 0.0) s = 0.0;
+        if (force_positive && s > 1.0) s = 1.0;
+        ly[R + i] = s > 0.0 ? log(s) : -40.0;
+    }
+    __syncthreads();
+    const int m = R - 2;
+    if (threadIdx.x < 2) {
+        const double* y = ly + threadIdx.x * R;
+        double* u = up + threadIdx.x * R;
+        double* sol = so + threadIdx.x * R;
+        if (m == 1) {
+            sol[0] = 6.0 * ((y[2] - y[1]) / (x[2] - x[1]) - (y[1] - y[0]) / (x[1] - x[0])) /
+                     (2.0 * ((x[1] - x[0]) + (x[2] - x[1])));
+        } else if (m > 1) {
+            double h0 = x[1] - x[0], h1 = x[2] - x[1];
+            u[0] = h1 / (2.0 * (h0 + h1));
+            sol[0] = 6.0 * ((y[2] - y[1]) / h1 - (y[1] - y[0]) / h0) / (2.0 * (h0 + h1));
+            for (int i = 1; i < m; ++i) {
+                const double hi = x[i + 1] - x[i], hi1 = x[i + 2] - x[i + 1];
+                const double den = 2.0 * (hi + hi1) - hi * u[i - 1];
+                if (i < m - 1) u[i] = hi1 / den;
+                sol[i] = (6.0 * ((y[i + 2] - y[i + 1]) / hi1 - (y[i + 1] - y[i]) / hi) - hi * sol[i - 1]) / den;
+            }
+            for (int i = m - 2; i >= 0; --i) sol[i] = sol[i] - u[i] * sol[i + 1];
+        }
+    }
+    __syncthreads();
+    const int iv = R - 1;
+    for (int item = threadIdx.x; item < 2 * iv; item += blockDim.x) {
+        const int row = item / iv, i = item - row * iv;
+        const double* y = ly + row * R;
+        const double* sol = so + row * R;
+        const double h = x[i + 1] - x[i];
+        const double sl = i == 0 ? 0.0 : sol[i - 1];
+        const double sr = i == iv - 1 ? 0.0 : sol[i];
+        const double a = y[i];
+        const double b = (y[i + 1] - y[i]) / h - h * (2.0 * sl + sr) / 6.0;
+        const double c = sl / 2.0;
+        const double d = (sr - sl) / (6.0 * h);
+        const size_t o = 2 + R + (size_t)row * 4 * iv + 4 * (size_t)i;
+        t64[o] = a;
+        t64[o + 1] = b;
+        t64[o + 2] = c;
+        t64[o + 3] = d;
+        if (t32) {
+            t32[o] = (float)a;
+            t32[o + 1] = (float)b;
+            t32[o + 2] = (float)c;
+            t32[o + 3] = (float)d;
+        }
+    }
+    for (int i = threadIdx.x; i < R; i += blockDim.x) {
+        t64[2 + i] = x[i];
+        if (t32) t32[2 + i] = (float)x[i];
+    }
+    if (threadIdx.x == 0) {
+        t64[0] = r[0];
+        t64[1] = r[R - 1];
+        if (t32) {
+            t32[0] = (float)r[0];
+            t32[1] = (float)r[R - 1];
+        }
+    }
+}
+
+// ------------------------------------------------------------------- K1
+
+struct GaussRule {
+    int nr, nt;
+    double xr[kMaxGaussR], wr[kMaxGaussR];
+    double xt[kMaxGaussT], wt[kMaxGaussT];
+};
+
+__device__ __forceinline__ double block_sum_256(double v, double* red) {
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    return t;
+}
+
+// K/bk/k1/kernel.cu:2040-2092 (production rule) for parent p = blockIdx.x:
+// panels [x_i, x_{i+1}] of the log-r grid, GL(nr) in ln z, GL(nt) in v with
+// theta = pi v^2, weight half_width w_r * w_t pi v, integrand * 2 z^2.
+__global__ void __launch_bounds__(256) k1_kernel(int R, const double* __restrict__ r,
+                                                 const double* __restrict__ N,
+                                                 const double* __restrict__ t64, BkConsts c,
+                                                 GaussRule g, double* __restrict__ k1) {
+    extern __shared__ double sh[];
+    __shared__ double red[4];
+    const int p = blockIdx.x;
+    if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
+        if (threadIdx.x == 0) k1[p] = 0.0;
+        return;
+    }
+    LdsSpline<double> s;
+    stage_spline<double>(t64, R, 0, sh, sh + R, s);
+    __syncthreads();
+    const double r_min = t64[0], r_max = t64[1];
+    const double rp = r[p];
+    const double n_r = fmin(loglog64(s, r_min, r_max, rp, 0.0, 1.0), 1.0);
+    const double a_r = alpha_s64(rp, c);
+    const int per_panel = g.nr * g.nt;
+    const int nodes = (R - 1) * per_panel;
+    double total = 0.0;
+    for (int k = threadIdx.x; k < nodes; k += blockDim.x) {
+        const int panel = k / per_panel;
+        const int loc = k - panel * per_panel;
+        const int i = loc / g.nt, j = loc - i * g.nt;
+        const double lower = s.x[panel], upper = s.x[panel + 1];
+        const double center = (lower + upper) / 2.0, half = (upper - lower) / 2.0;
+        const double log_z = center + half * g.xr[i];
+        const double z = exp(log_z);
+        const double v = (g.xt[j] + 1.0) / 2.0;
+        const double theta = kPi * v * v;
+        const double w = (half * g.wr[i]) * (g.wt[j] * kPi * v);
+        total += k1_integrand64(rp, z, theta, n_r, a_r, s, r_min, r_max, c) * (2.0 * z * z) * w;
+    }
+    total = block_sum_256(total, red);
+    if (threadIdx.x == 0) k1[p] = total;
+}
+
+// ------------------------------------------------------------- K2 + Kf
+
+// LDS layout of the NLO kernel (bytes), shared by host sizing and device.
+struct NloLayout {
+    size_t x, rec, edges, dir, absw, bins, hist, cnt, total;
+};
+
+__host__ __device__ inline NloLayout nlo_layout(int R, int bins, int spt, bool hist, size_t tsize) {
+    NloLayout L{};
+    size_t o = 0;
+    L.x = o;
+    o = align_up(o + (size_t)R * tsize, 16);
+    L.rec = o;
+    o = align_up(o + (size_t)4 * (R - 1) * tsize, 16);
+    L.edges = o;
+    o = align_up(o + (size_t)4 * (bins + 1) * tsize, 16);
+    L.dir = o;
+    o = align_up(o + (size_t)4 * 30 * sizeof(uint32_t), 16);
+    if (hist) {
+        L.absw = o;
+        o = align_up(o + (size_t)256 * spt * sizeof(double), 16);
+        L.bins = o;
+        o = align_up(o + (size_t)256 * spt * sizeof(uint32_t), 16);
+        L.hist = o;
+        o = align_up(o + (size_t)4 * bins * sizeof(unsigned long long), 16);
+        L.cnt = o;
+        o = align_up(o + (size_t)4 * bins * sizeof(unsigned int), 16);
+    }
+    L.total = o;
+    return L;
+}
+
+struct NloArgs {
+    int R, bins, spt, bpp;
+    int64_t n, first;
+    const double* r;
+    const double* N;
+    const double* t64;
+    const float* t32;
+    const double* edges;
+    double* part;       // (R - 1) x bpp x 4
+    double* part_hist;  // (R - 1) x bpp x 4 x bins
+    double* part_cnt;
+    BkConsts c;
+    uint32_t dir[4][30];
+    uint32_t shift[4];
+    uint32_t first_q[4];
+};
+
+// Per (parent, block): spt samples per lane.  Sample ordinal
+// s = (blk * spt * 4 + j * 4 + wave) * 64 + lane, Sobol index first + s, so
+// lanes of a wave hold 64 consecutive indices and the gray-code state splits
+// into a wave-uniform high part and a per-lane low part.
+template <int MODE, bool HIST>
+__global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
+    using T = typename std::conditional<MODE == NLO_PREC_FP32, float, double>::type;
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[4][4];
+    __shared__ double wmax[4];
+    const int p = blockIdx.y + 1;
+    const int blk = blockIdx.x;
+    const size_t pidx = (size_t)(p - 1) * a.bpp + blk;
+    const NloLayout lay = nlo_layout(a.R, a.bins, a.spt, HIST, sizeof(T));
+    if (!(a.N[p] <= 0.99999)) {
+        if (threadIdx.x < 4) a.part[pidx * 4 + threadIdx.x] = 0.0;
+        if (HIST)
+            for (int k = threadIdx.x; k < 4 * a.bins; k += blockDim.x) {
+                a.part_hist[pidx * 4 * a.bins + k] = 0.0;
+                a.part_cnt[pidx * 4 * a.bins + k] = 0.0;
+            }
+        return;
+    }
+    T* sx = reinterpret_cast<T*>(smem + lay.x);
+    T* srec = reinterpret_cast<T*>(smem + lay.rec);
+    T* E = reinterpret_cast<T*>(smem + lay.edges);
+    uint32_t* dir = reinterpret_cast<uint32_t*>(smem + lay.dir);
+    LdsSpline<T> spl;
+    // FP64 reference formula reads the S spline (row 1); the stable forms read N (row 0).
+    const int row = MODE == NLO_PREC_FP64 ? 1 : 0;
+    if constexpr (MODE == NLO_PREC_FP32) stage_spline<float>(a.t32, a.R, row, sx, srec, spl);
+    else stage_spline<double>(a.t64, a.R, row, sx, srec, spl);
+    const int ne = 4 * (a.bins + 1);
+    const double* ep = a.edges + (size_t)p * ne;
+    for (int i = threadIdx.x; i < ne; i += blockDim.x) E[i] = (T)ep[i];
+    for (int i = threadIdx.x; i < 120; i += blockDim.x) dir[i] = a.dir[i / 30][i % 30];
+    unsigned long long* hh = nullptr;
+    unsigned int* hc = nullptr;
+    double* st_w = nullptr;
+    uint32_t* st_b = nullptr;
+    if (HIST) {
+        hh = reinterpret_cast<unsigned long long*>(smem + lay.hist);
+        hc = reinterpret_cast<unsigned int*>(smem + lay.cnt);
+        st_w = reinterpret_cast<double*>(smem + lay.absw);
+        st_b = reinterpret_cast<uint32_t*>(smem + lay.bins);
+        for (int k = threadIdx.x; k < 4 * a.bins; k += blockDim.x) {
+            hh[k] = 0ull;
+            hc[k] = 0u;
+        }
+    }
+    __syncthreads();
+
+    const double r0 = a.r[0], rmax = a.r[a.R - 1];
+    const double rp = a.r[p];
+    // nlo_vegas_bounds (S/bk/derivatives.py:424-431)
+    const double lo_r = log(0.5 * r0);
+    const double w_r = log(2.0 * rmax) - lo_r;
+    const T lo[4] = {(T)lo_r, (T)lo_r, T(0), T(0)};
+    const T wd[4] = {(T)w_r, (T)w_r, (T)(2.0 * kPi), (T)(2.0 * kPi)};
+    T vol = T(1);
+    for (int d = 0; d < 4; ++d) vol = vol * wd[d];
+    const T r_t = (T)rp;
+    const T log_r_t = (T)log(rp);
+    const double r_min64 = a.t64[0], r_max64 = a.t64[1];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const uint32_t glo0 = (uint32_t)(lane ^ (lane >> 1));
+    uint32_t vlo[4][2];
+    for (int d = 0; d < 4; ++d) {
+        uint32_t v0 = 0, v1;
+        for (int b = 0; b < 6; ++b)
+            if ((glo0 >> b) & 1u) v0 ^= dir[d * 30 + b];
+        v1 = v0 ^ dir[d * 30 + 5];
+        vlo[d][0] = v0 ^ a.shift[d];
+        vlo[d][1] = v1 ^ a.shift[d];
+    }
+
+    double s1 = 0.0, s2 = 0.0, sk2 = 0.0, skf = 0.0;
+    const int64_t hi0 = (a.first >> 6) + (int64_t)blk * a.spt * 4 + wave;
+    for (int j = 0; j < a.spt; ++j) {
+        const int64_t hi = hi0 + (int64_t)j * 4;
+        const int64_t sample = (hi << 6) + lane - a.first;
+        const bool valid = sample < a.n;
+        // wave-uniform high part of the gray code (bits 6..29)
+        uint32_t vhi[4] = {0u, 0u, 0u, 0u};
+        {
+            uint64_t g = (uint64_t)hi ^ ((uint64_t)hi >> 1);
+            for (int b = 6; g && b < 30; ++b, g >>= 1)
+                if (g & 1u)
+                    for (int d = 0; d < 4; ++d) vhi[d] ^= dir[d * 30 + b];
+        }
+        const int par = (int)(hi & 1);
+        T x[4];
+        T jac = T(1);
+        uint32_t packed = 0u;
+        for (int d = 0; d < 4; ++d) {
+            uint32_t q = vhi[d] ^ vlo[d][par];
+            if (hi == 0 && lane == 0) q = a.first_q[d];
+            // bin = min(floor(u B), B - 1), frac = u B - bin with u = q 2^-30
+            const uint64_t scaled = (uint64_t)q * (uint64_t)a.bins;
+            uint32_t b = (uint32_t)(scaled >> 30);
+            b = b > (uint32_t)(a.bins - 1) ? (uint32_t)(a.bins - 1) : b;
+            const T frac = (T)(double)(scaled - ((uint64_t)b << 30)) * (T)0x1p-30;
+            const T left = E[d * (a.bins + 1) + b], right = E[d * (a.bins + 1) + b + 1];
+            const T w = right - left;
+            x[d] = lo[d] + wd[d] * (left + w * frac);
+            jac = jac * ((T)a.bins * w);
+            packed |= b << (8 * d);
+        }
+        jac = vol * jac;
+        double wv = 0.0, k2 = 0.0, kf = 0.0;
+        if (valid) {
+            if constexpr (MODE == NLO_PREC_FP64) {
+                const NloValue64 f = nlo_integrand64(rp, x[0], x[1], x[2], x[3], spl, r_min64,
+                                                     r_max64, a.c);
+                wv = f.total * (double)jac;
+                k2 = f.k2 * (double)jac;
+                kf = f.kf * (double)jac;
+            } else {
+                const NloStable<T> f = nlo_integrand_stable<T>(
+                    r_t, log_r_t, x[0], x[1], x[2], x[3], spl, (T)a.c.b0, (T)a.c.log_mu0,
+                    (T)a.c.log_scale0, (T)(a.c.nf / a.c.nc), (T)a.c.nc, (T)a.c.norm,
+                    a.c.symmetrize != 0);
+                const T wt = f.total * jac;
+                wv = (double)wt;
+                k2 = (double)(f.k2 * jac);
+                kf = (double)(f.kf * jac);
+            }
+            s1 += wv;
+            s2 += wv * wv;
+            sk2 += k2;
+            skf += kf;
+        }
+        if (HIST) {
+            st_w[threadIdx.x + j * 256] = valid ? fabs(wv) : 0.0;
+            st_b[threadIdx.x + j * 256] = valid ? packed : 0xffffffffu;
+        }
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    sk2 = wave_sum(sk2);
+    skf = wave_sum(skf);
+    if (lane == 0) {
+        red[wave][0] = s1;
+        red[wave][1] = s2;
+        red[wave][2] = sk2;
+        red[wave][3] = skf;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
+        a.part[pidx * 4 + threadIdx.x] = t;
+    }
+    if (HIST) {
+        // block max of |w| -> fixed-point scale; integer LDS sums are exact
+        double m = 0.0;
+        for (int j = 0; j < a.spt; ++j) m = fmax(m, st_w[threadIdx.x + j * 256]);
+        m = wave_max(m);
+        if (lane == 0) wmax[wave] = m;
+        __syncthreads();
+        m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        const int e = m > 0.0 ? ilogb(m) : 0;
+        const int sc = 50 - e;  // m 2^sc < 2^51, <= 2^12 terms stay below 2^63
+        for (int j = 0; j < a.spt; ++j) {
+            const uint32_t pk = st_b[threadIdx.x + j * 256];
+            if (pk == 0xffffffffu) continue;
+            const unsigned long long v =
+                (unsigned long long)rint(ldexp(st_w[threadIdx.x + j * 256], sc));
+            for (int d = 0; d < 4; ++d) {
+                const int b = (pk >> (8 * d)) & 0xff;
+                atomicAdd(&hh[d * a.bins + b], v);
+                atomicAdd(&hc[d * a.bins + b], 1u);
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < 4 * a.bins; k += blockDim.x) {
+            a.part_hist[pidx * 4 * a.bins + k] = ldexp((double)hh[k], -sc);
+            a.part_cnt[pidx * 4 * a.bins + k] = (double)hc[k];
+        }
+    }
+}
+
+// Fixed-order sum of the per-block partials of each parent.
+__global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bins, int hist,
+                                                         const double* __restrict__ part,
+                                                         const double* __restrict__ ph,
+                                                         const double* __restrict__ pc,
+                                                         double* __restrict__ sums,
+                                                         double* __restrict__ hsum,
+                                                         double* __restrict__ csum) {
+    const int p = blockIdx.x;  // 0 .. R-1
+    if (p == 0) {
+        if (threadIdx.x < 4) sums[threadIdx.x] = 0.0;
+        if (hist)
+            for (int k = threadIdx.x; k < 4 * bins; k += blockDim.x) {
+                hsum[k] = 0.0;
+                csum[k] = 0.0;
+            }
+        return;
+    }
+    const size_t base = (size_t)(p - 1) * bpp;
+    if (threadIdx.x < 4) {
+        double t = 0.0;
+        for (int b = 0; b < bpp; ++b) t += part[(base + b) * 4 + threadIdx.x];
+        sums[(size_t)p * 4 + threadIdx.x] = t;
+    }
+    if (hist) {
+        for (int k = threadIdx.x; k < 4 * bins; k += blockDim.x) {
+            double th = 0.0, tc = 0.0;
+            for (int b = 0; b < bpp; ++b) {
+                th += ph[(base + b) * 4 * bins + k];
+                tc += pc[(base + b) * 4 * bins + k];
+            }
+            hsum[(size_t)p * 4 * bins + k] = th;
+            csum[(size_t)p * 4 * bins + k] = tc;
+        }
+    }
+}
+
+// S/bk/evolution.py:340-351 plus the active set of :501.
+__global__ void rhs_finalize_kernel(int R, const double* __restrict__ N, const double* __restrict__ k1,
+                                    const double* __restrict__ sums, double inv_n,
+                                    double* __restrict__ dndy, double* __restrict__ k2kf) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= R) return;
+    const bool active = p >= 1 && N[p] <= 0.99999;
+    const double nlo = active ? sums[(size_t)p * 4] * inv_n : 0.0;
+    double v = active ? k1[p] + nlo : 0.0;
+    dndy[p] = isfinite(v) ? v : 0.0;
+    if (k2kf) k2kf[p] = nlo;
+}
+
+// out = N + h (a0 K0 + a1 K1 + a2 K2)
+__global__ void rk_stage_kernel(int R, const double* __restrict__ N, const double* __restrict__ K0,
+                                const double* __restrict__ K1, const double* __restrict__ K2,
+                                double a0, double a1, double a2, double h, double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    double acc = a0 * K0[i];
+    if (K1) acc += a1 * K1[i];
+    if (K2) acc += a2 * K2[i];
+    out[i] = N[i] + h * acc;
+}
+
+// N <- N + h sum_s b_s K_s, clamp to [0, 1] when forced, non-finite flag
+// (S/bk/evolution.py:150-153).
+__global__ void rk_final_kernel(int R, double* __restrict__ N, const double* __restrict__ K0,
+                                const double* __restrict__ K1, const double* __restrict__ K2,
+                                const double* __restrict__ K3, double b0, double b1, double b2,
+                                double b3, double h, int force_positive, int32_t* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= R) return;
+    double acc = b0 * K0[i];
+    if (K1) acc += b1 * K1[i];
+    if (K2) acc += b2 * K2[i];
+    if (K3) acc += b3 * K3[i];
+    double v = N[i] + h * acc;
+    if (force_positive) v = fmin(fmax(v, 0.0), 1.0);
+    if (!isfinite(v)) flag[0] = 1;
+    N[i] = v;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int32_t v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------ internal launches
+
+int init_kernel_attributes() {
+    static std::once_flag once;
+    static int status = NLO_OK;
+    std::call_once(once, [] {
+        int dev = 0, max_lds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess) {
+            (void)hipGetLastError();
+            return;
+        }
+        const int dyn = max_lds - 2048;  // room for the kernels' static LDS
+        if (dyn <= 65536) return;
+        const void* fns[] = {
+            (const void*)spline_build_kernel,
+            (const void*)k1_kernel,
+            (const void*)nlo_kernel<NLO_PREC_FP64, false>,
+            (const void*)nlo_kernel<NLO_PREC_FP64, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP32, false>,
+            (const void*)nlo_kernel<NLO_PREC_FP32, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false>,
+            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, true>,
+        };
+        for (const void* f : fns) {
+            if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess)
+                (void)hipGetLastError();
+        }
+    });
+    return status;
+}
+
+int launch_spline_build(int R, const double* r, const double* N, int force_positive, double* t64,
+                        float* t32, hipStream_t stream) {
+    const size_t smem = (size_t)7 * R * sizeof(double);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    hipLaunchKernelGGL(spline_build_kernel, dim3(1), dim3(64), smem, stream, R, r, N, force_positive,
+                       t64, t32);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_k1(int R, const double* r, const double* N, const double* t64, const BkConsts& c, int nr,
+              int nt, double* k1, hipStream_t stream) {
+    GaussRule g{};
+    g.nr = nr;
+    g.nt = nt;
+    if (nlo_gauss_legendre(nr, g.xr, g.wr) != NLO_OK) return NLO_ERR_INVALID;
+    if (nlo_gauss_legendre(nt, g.xt, g.wt) != NLO_OK) return NLO_ERR_INVALID;
+    const size_t smem = ((size_t)R + 4 * (size_t)(R - 1)) * sizeof(double);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    hipLaunchKernelGGL(k1_kernel, dim3(R), dim3(256), smem, stream, R, r, N, t64, c, g, k1);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+NloPlan nlo_plan(int R, int bins, int64_t n, bool hist) {
+    NloPlan pl{};
+    // Enough blocks to fill 256 CUs several times over; each lane evaluates
+    // spt samples (at most 8 when the histogram stages them in LDS).
+    const int64_t parents = R - 1;
+    int spt = 16;
+    while (spt > 1 && parents * ((n + 256 * spt - 1) / (256 * spt)) < 8192) spt /= 2;
+    if (hist && spt > 8) spt = 8;
+    pl.spt = spt;
+    pl.bpp = (int)((n + 256 * spt - 1) / (256 * spt));
+    pl.part_bytes = (size_t)parents * pl.bpp * 4 * sizeof(double);
+    pl.hist_bytes = hist ? (size_t)parents * pl.bpp * 4 * bins * sizeof(double) : 0;
+    pl.workspace = align_up(pl.part_bytes, 256) + 2 * align_up(pl.hist_bytes, 256);
+    return pl;
+}
+
+int launch_nlo(int R, const double* r, const double* N, const double* t64, const float* t32,
+               const double* edges, int bins, const nlo_sobol_table& sobol, int64_t first, int64_t n,
+               int precision, const BkConsts& c, double* sums, double* hist, double* count,
+               void* ws, size_t ws_bytes, hipStream_t stream) {
+    const bool want_hist = hist != nullptr;
+    const NloPlan pl = nlo_plan(R, bins, n, want_hist);
+    if (ws_bytes < pl.workspace || (!ws && pl.workspace)) {
+        set_error("nlo_bk_nlo: workspace %zu bytes < %zu", ws_bytes, pl.workspace);
+        return NLO_ERR_WORKSPACE;
+    }
+    NloArgs a{};
+    a.R = R;
+    a.bins = bins;
+    a.spt = pl.spt;
+    a.bpp = pl.bpp;
+    a.n = n;
+    a.first = first;
+    a.r = r;
+    a.N = N;
+    a.t64 = t64;
+    a.t32 = t32;
+    a.edges = edges;
+    char* w = static_cast<char*>(ws);
+    a.part = reinterpret_cast<double*>(w);
+    a.part_hist = want_hist ? reinterpret_cast<double*>(w + align_up(pl.part_bytes, 256)) : nullptr;
+    a.part_cnt = want_hist ? reinterpret_cast<double*>(w + align_up(pl.part_bytes, 256) +
+                                                       align_up(pl.hist_bytes, 256))
+                           : nullptr;
+    a.c = c;
+    for (int d = 0; d < 4; ++d) {
+        for (int b = 0; b < 30; ++b) a.dir[d][b] = sobol.direction[d][b];
+        a.shift[d] = sobol.shift[d];
+        a.first_q[d] = sobol.first[d];
+    }
+    const size_t tsize = precision == NLO_PREC_FP32 ? sizeof(float) : sizeof(double);
+    const size_t smem = nlo_layout(R, bins, pl.spt, want_hist, tsize).total;
+    const dim3 grid(pl.bpp, R - 1);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+#define NLO_PICK(MODE, H) \
+    hipLaunchKernelGGL((nlo_kernel<MODE, H>), grid, dim3(256), smem, stream, a)
+    if (precision == NLO_PREC_FP64) {
+        if (want_hist) NLO_PICK(NLO_PREC_FP64, true);
+        else NLO_PICK(NLO_PREC_FP64, false);
+    } else if (precision == NLO_PREC_FP32) {
+        if (want_hist) NLO_PICK(NLO_PREC_FP32, true);
+        else NLO_PICK(NLO_PREC_FP32, false);
+    } else {
+        if (want_hist) NLO_PICK(NLO_PREC_FP64_STABLE, true);
+        else NLO_PICK(NLO_PREC_FP64_STABLE, false);
+    }
+#undef NLO_PICK
+    NLO_LAUNCH_CHECK();
+    hipLaunchKernelGGL(nlo_reduce_kernel, dim3(R), dim3(256), 0, stream, R, pl.bpp, bins,
+                       want_hist ? 1 : 0, a.part, a.part_hist, a.part_cnt, sums, hist, count);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_rhs_finalize(int R, const double* N, const double* k1, const double* sums, int64_t n,
+                        double* dndy, double* k2kf, hipStream_t stream) {
+    hipLaunchKernelGGL(rhs_finalize_kernel, dim3((R + 255) / 256), dim3(256), 0, stream, R, N, k1,
+                       sums, 1.0 / (double)n, dndy, k2kf);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_rk_stage(int R, const double* N, const double* K0, const double* K1, const double* K2,
+                    double a0, double a1, double a2, double h, double* out, hipStream_t stream) {
+    hipLaunchKernelGGL(rk_stage_kernel, dim3((R + 255) / 256), dim3(256), 0, stream, R, N, K0, K1, K2,
+                       a0, a1, a2, h, out);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_rk_final(int R, double* N, const double* K0, const double* K1, const double* K2,
+                    const double* K3, double b0, double b1, double b2, double b3, double h,
+                    int force_positive, int32_t* flag, hipStream_t stream) {
+    hipLaunchKernelGGL(rk_final_kernel, dim3((R + 255) / 256), dim3(256), 0, stream, R, N, K0, K1, K2,
+                       K3, b0, b1, b2, b3, h, force_positive, flag);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+}  // namespace nlo
+
+// ------------------------------------------------------------------- C ABI
+
+extern "C" {
+
+int nlo_bk_spline_build(int32_t R, const double* d_r, const double* d_N, int32_t force_positive,
+                        double* d_table64, float* d_table32, hipStream_t stream) {
+    NLO_REQUIRE(R >= 3 && R <= nlo::kMaxR, "nlo_bk_spline_build: R=%d outside [3, %d]", R, nlo::kMaxR);
+    NLO_REQUIRE(d_r && d_N && d_table64, "nlo_bk_spline_build: null pointer");
+    const int rc = nlo::launch_spline_build(R, d_r, d_N, force_positive, d_table64, d_table32, stream);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+int nlo_bk_k1(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+              const nlo_bk_physics* physics, int32_t nr, int32_t nt, double* d_k1,
+              hipStream_t stream) {
+    NLO_REQUIRE(R >= 3 && R <= nlo::kMaxR, "nlo_bk_k1: R=%d outside [3, %d]", R, nlo::kMaxR);
+    NLO_REQUIRE(nr >= 1 && nr <= nlo::kMaxGaussR && nt >= 1 && nt <= nlo::kMaxGaussT,
+                "nlo_bk_k1: GL orders (%d, %d) outside [1, %d] x [1, %d]", nr, nt, nlo::kMaxGaussR,
+                nlo::kMaxGaussT);
+    NLO_REQUIRE(d_r && d_N && d_table64 && d_k1, "nlo_bk_k1: null pointer");
+    int rc = nlo::validate_physics(physics);
+    if (rc != NLO_OK) return rc;
+    rc = nlo::launch_k1(R, d_r, d_N, d_table64, nlo::make_consts(*physics), nr, nt, d_k1, stream);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+size_t nlo_bk_nlo_workspace(int32_t R, int32_t bins, int64_t n, int32_t want_hist) {
+    if (R < 3 || bins < 2 || n < 1) return 0;
+    return nlo::nlo_plan(R, bins, n, want_hist != 0).workspace;
+}
+
+int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+               const float* d_table32, const double* d_edges, int32_t bins,
+               const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision,
+               const nlo_bk_physics* physics, double* d_sums, double* d_hist, double* d_count,
+               void* d_workspace, size_t workspace_bytes, hipStream_t stream) {
+    NLO_REQUIRE(R >= 3 && R <= nlo::kMaxR, "nlo_bk_nlo: R=%d outside [3, %d]", R, nlo::kMaxR);
+    NLO_REQUIRE(bins >= 2 && bins <= 255, "nlo_bk_nlo: bins=%d outside [2, 255]", bins);
+    NLO_REQUIRE(sobol && sobol->dim == 4, "nlo_bk_nlo: a 4-dimensional Sobol table is required");
+    NLO_REQUIRE(n >= 1 && first_index >= 0 && first_index % 64 == 0 &&
+                    first_index + n <= (int64_t(1) << 30),
+                "nlo_bk_nlo: sample range [%lld, %lld) invalid (first must be a multiple of 64, "
+                "end <= 2^30)",
+                (long long)first_index, (long long)(first_index + n));
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32 ||
+                    precision == NLO_PREC_FP64_STABLE,
+                "nlo_bk_nlo: unknown precision %d", precision);
+    NLO_REQUIRE(d_r && d_N && d_table64 && d_edges && d_sums, "nlo_bk_nlo: null pointer");
+    NLO_REQUIRE(precision != NLO_PREC_FP32 || d_table32, "nlo_bk_nlo: FP32 mode needs d_table32");
+    NLO_REQUIRE((d_hist == nullptr) == (d_count == nullptr),
+                "nlo_bk_nlo: d_hist and d_count must be given together");
+    int rc = nlo::validate_physics(physics);
+    if (rc != NLO_OK) return rc;
+    rc = nlo::launch_nlo(R, d_r, d_N, d_table64, d_table32, d_edges, bins, *sobol, first_index, n,
+                         precision, nlo::make_consts(*physics), d_sums, d_hist, d_count, d_workspace,
+                         workspace_bytes, stream);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+int nlo_bk_rhs_finalize(int32_t R, const double* d_N, const double* d_k1, const double* d_sums,
+                        int64_t n, double* d_dndy, double* d_k2kf, hipStream_t stream) {
+    NLO_REQUIRE(R >= 1 && n >= 1, "nlo_bk_rhs_finalize: R=%d n=%lld invalid", R, (long long)n);
+    NLO_REQUIRE(d_N && d_k1 && d_sums && d_dndy, "nlo_bk_rhs_finalize: null pointer");
+    const int rc = nlo::launch_rhs_finalize(R, d_N, d_k1, d_sums, n, d_dndy, d_k2kf, stream);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,376 @@
+// Scrambled Sobol generation, VEGAS map, deterministic histogram and grid
+// refinement for gfx950.  HBM-streaming integer/byte work: no MFMA, coalesced
+// wave64 stores, one point per lane.
+#include <cmath>
+#include <cstring>
+
+#include "common.hpp"
+
+namespace nlo {
+
+namespace {
+
+// ------------------------------------------------------------------ Sobol
+
+// One wave covers 64 consecutive sample indices n = hi * 64 + lane.  Then
+// gray(n) = (gray(hi) << 6) ^ (lane ^ (lane >> 1) ^ ((hi & 1) << 5)), so the
+// state splits into a wave-uniform part (direction bits 6..29, scalar ALU)
+// and a per-lane part (bits 0..5) fixed for the whole launch.
+__global__ void __launch_bounds__(256) sobol_draw_kernel(nlo_sobol_table t, int64_t first,
+                                                          int64_t n, int32_t kind, void* out) {
+    const int lane = threadIdx.x & 63;
+    const int dim = t.dim;
+    const int64_t wave_global = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
+    const int64_t hi_begin = first >> 6;
+    const int64_t hi_end = (first + n + 63) >> 6;
+    for (int64_t hi = hi_begin + wave_global; hi < hi_end; hi += waves) {
+        const uint64_t idx = ((uint64_t)hi << 6) | (uint64_t)lane;
+        const int64_t row = (int64_t)idx - first;
+        const bool live = row >= 0 && row < n;
+        const uint64_t ghi = (uint64_t)hi ^ ((uint64_t)hi >> 1);
+        const uint32_t glo = (uint32_t)(lane ^ (lane >> 1)) ^ ((uint32_t)(hi & 1) << 5);
+        for (int d = 0; d < dim; ++d) {
+            uint32_t q = t.shift[d];
+            uint64_t g = ghi;
+            for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
+                if (g & 1u) q ^= t.direction[d][b];
+#pragma unroll
+            for (int b = 0; b < 6; ++b)
+                if ((glo >> b) & 1u) q ^= t.direction[d][b];
+            if (!live) continue;
+            const size_t o = (size_t)row * dim + d;
+            if (kind == NLO_SOBOL_U32) {
+                static_cast<uint32_t*>(out)[o] = q;
+            } else if (kind == NLO_SOBOL_F32) {
+                static_cast<float*>(out)[o] = (float)q * 0x1p-30f;
+            } else {
+                const uint32_t qq = idx == 0 ? t.first[d] : q;
+                static_cast<double*>(out)[o] = (double)qq * 0x1p-30;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ VEGAS
+
+// map_importance_grid (S/dis/fixed_sobol.py:100-113) in IEEE order, without
+// contraction, so bins and coordinates equal the FP64 reference bit for bit.
+__global__ void __launch_bounds__(256) vegas_map_kernel(int dim, int bins, int64_t n,
+                                                         const double* __restrict__ u,
+                                                         const double* __restrict__ edges,
+                                                         const double* __restrict__ lo,
+                                                         const double* __restrict__ width,
+                                                         double* __restrict__ x,
+                                                         double* __restrict__ jac,
+                                                         uint8_t* __restrict__ bin) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        double j = 1.0, vol = 1.0;
+        for (int d = 0; d < dim; ++d) {
+            const double scaled = __dmul_rn(u[i * dim + d], (double)bins);
+            int b = (int)floor(scaled);
+            b = b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
+            const double frac = __dsub_rn(scaled, (double)b);
+            const double left = edges[d * (bins + 1) + b];
+            const double right = edges[d * (bins + 1) + b + 1];
+            const double w = __dsub_rn(right, left);
+            x[i * dim + d] = __dadd_rn(lo[d], __dmul_rn(width[d], __dadd_rn(left, __dmul_rn(w, frac))));
+            j = __dmul_rn(j, __dmul_rn((double)bins, w));
+            vol = __dmul_rn(vol, width[d]);
+            if (bin) bin[i * dim + d] = (uint8_t)b;
+        }
+        jac[i] = __dmul_rn(vol, j);
+    }
+}
+
+constexpr int kHistThreads = 256;
+constexpr int kHistChunk = 4096;  // samples per block: 2^12, sums stay below 2^63
+
+// Per-block histogram.  Each block first finds the largest |w| of its chunk,
+// then adds every |w| as a 64-bit fixed-point integer scaled so that the
+// chunk total cannot exceed 2^63: integer addition is exact and
+// order-independent, so LDS atomics give the same bits every run.
+__global__ void __launch_bounds__(kHistThreads) vegas_hist_block_kernel(
+    int dim, int bins, int64_t n, const uint8_t* __restrict__ bin, const double* __restrict__ absw,
+    double* __restrict__ part_hist, double* __restrict__ part_count) {
+    extern __shared__ unsigned long long smem[];
+    unsigned long long* h = smem;                                      // dim * bins
+    unsigned int* c = reinterpret_cast<unsigned int*>(smem + dim * bins);  // dim * bins
+    __shared__ double wave_max_v[kHistThreads / 64];
+    const int64_t begin = (int64_t)blockIdx.x * kHistChunk;
+    const int64_t end = begin + kHistChunk < n ? begin + kHistChunk : n;
+    for (int k = threadIdx.x; k < dim * bins; k += blockDim.x) {
+        h[k] = 0ull;
+        c[k] = 0u;
+    }
+    double m = 0.0;
+    for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) m = fmax(m, fabs(absw[i]));
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) wave_max_v[threadIdx.x >> 6] = m;
+    __syncthreads();
+    m = 0.0;
+    for (int k = 0; k < kHistThreads / 64; ++k) m = fmax(m, wave_max_v[k]);
+    const int e = m > 0.0 ? ilogb(m) : 0;
+    const int s = 50 - e;  // m * 2^s < 2^51; 4096 terms < 2^63
+    for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
+        const unsigned long long v = (unsigned long long)rint(ldexp(fabs(absw[i]), s));
+        for (int d = 0; d < dim; ++d) {
+            const int b = min((int)bin[i * dim + d], bins - 1);
+            atomicAdd(&h[d * bins + b], v);
+            atomicAdd(&c[d * bins + b], 1u);
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < dim * bins; k += blockDim.x) {
+        part_hist[(size_t)blockIdx.x * dim * bins + k] = ldexp((double)h[k], -s);
+        part_count[(size_t)blockIdx.x * dim * bins + k] = (double)c[k];
+    }
+}
+
+// Fixed-order sum of the per-block partials: one thread per (dim, bin).
+__global__ void vegas_hist_reduce_kernel(int values, int blocks, const double* __restrict__ ph,
+                                         const double* __restrict__ pc, double* __restrict__ hist,
+                                         double* __restrict__ count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= values) return;
+    double sh = 0.0, sc = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        sh += ph[(size_t)b * values + k];
+        sc += pc[(size_t)b * values + k];
+    }
+    hist[k] = sh;
+    count[k] = sc;
+}
+
+// Smoothed importance of bin i (S/numerics/integration.py:828-833): 3-point
+// mean inside, 2-point mean at the ends, plus eps.
+__device__ __forceinline__ double vegas_importance(const double* h, const double* c, int i) {
+    return __ddiv_rn(h[i], fmax(c[i], 1.0));
+}
+
+__device__ __forceinline__ double vegas_smoothed(const double* h, const double* c, int bins, int i,
+                                                 double eps) {
+    double v;
+    if (i == 0) {
+        v = __ddiv_rn(__dadd_rn(vegas_importance(h, c, 0), vegas_importance(h, c, 1)), 2.0);
+    } else if (i == bins - 1) {
+        v = __ddiv_rn(
+            __dadd_rn(vegas_importance(h, c, bins - 2), vegas_importance(h, c, bins - 1)), 2.0);
+    } else {
+        v = __ddiv_rn(__dadd_rn(__dadd_rn(vegas_importance(h, c, i - 1), vegas_importance(h, c, i)),
+                                vegas_importance(h, c, i + 1)),
+                      3.0);
+    }
+    return __dadd_rn(v, eps);
+}
+
+// _adapt_vegas_edges (S/numerics/integration.py:835-866) for one (row, dim)
+// per thread, in the reference's operation order (no contraction).  The
+// linspace targets increase, so searchsorted(cumulative, target, right) - 1
+// is found by a forward sweep over the cumulative sums: no scratch memory.
+__global__ void vegas_adapt_kernel(int rows, int dim, int bins, const double* __restrict__ edges,
+                                   const double* __restrict__ hist,
+                                   const double* __restrict__ count, double eps,
+                                   double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= rows * dim) return;
+    const double* e = edges + (size_t)k * (bins + 1);
+    const double* hh = hist + (size_t)k * bins;
+    const double* cc = count + (size_t)k * bins;
+    double* o = out + (size_t)k * (bins + 1);
+    double total = 0.0;
+    for (int i = 0; i < bins; ++i) total = __dadd_rn(total, vegas_smoothed(hh, cc, bins, i, eps));
+    const double step = __ddiv_rn(total, (double)bins);
+    // j: largest index with cumulative[j] <= target; cum_j = cumulative[j];
+    // sm_j = smoothed[j] (cumulative[j + 1] = cum_j + sm_j).
+    int j = 0;
+    double cum_j = 0.0;
+    double sm_j = vegas_smoothed(hh, cc, bins, 0, eps);
+    double cum_prev = 0.0, sm_prev = sm_j;  // values at j - 1 (valid when j == bins)
+    for (int t = 0; t <= bins; ++t) {
+        const double target =
+            t < (bins + 1) / 2 ? __dmul_rn((double)t, step) : fma(-step, (double)(bins - t), total);
+        while (j < bins) {
+            const double cum_next = __dadd_rn(cum_j, sm_j);
+            if (!(target >= cum_next)) break;
+            cum_prev = cum_j;
+            sm_prev = sm_j;
+            cum_j = cum_next;
+            ++j;
+            if (j < bins) sm_j = vegas_smoothed(hh, cc, bins, j, eps);
+        }
+        const int b = j < bins ? j : bins - 1;
+        const double cb = j < bins ? cum_j : cum_prev;
+        const double sb = j < bins ? sm_j : sm_prev;
+        const double frac = __ddiv_rn(__dsub_rn(target, cb), sb);
+        o[t] = __dadd_rn(e[b], __dmul_rn(frac, __dsub_rn(e[b + 1], e[b])));
+    }
+    o[0] = 0.0;
+    o[bins] = 1.0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------ error state
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+}
+
+void clear_error() { g_error.clear(); }
+
+const char* last_error() { return g_error.c_str(); }
+
+int launch_vegas_adapt(int rows, int dim, int bins, const double* edges, const double* hist,
+                       const double* count, double eps, double* out, hipStream_t stream) {
+    const int items = rows * dim;
+    hipLaunchKernelGGL(vegas_adapt_kernel, dim3((items + 63) / 64), dim3(64), 0, stream, rows, dim,
+                       bins, edges, hist, count, eps, out);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+}  // namespace nlo
+
+// ------------------------------------------------------------------- C ABI
+
+extern "C" {
+
+int nlo_api_version(void) { return NLO_API_VERSION; }
+
+const char* nlo_last_error(void) { return nlo::last_error(); }
+
+int nlo_gauss_legendre(int32_t n, double* x, double* w) {
+    NLO_REQUIRE(n >= 1 && n <= 512, "nlo_gauss_legendre: n=%d outside [1, 512]", n);
+    NLO_REQUIRE(x && w, "nlo_gauss_legendre: null output");
+    // Newton iteration on P_n from the Chebyshev-like initial guess
+    // cos(pi (i + 0.75) / (n + 0.5)), three-term recurrence in long double.
+    for (int i = 0; i < n; ++i) {
+        long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+        long double dp = 1.0L;
+        for (int it = 0; it < 100; ++it) {
+            long double p0 = 1.0L, p1 = z;
+            for (int k = 2; k <= n; ++k) {
+                const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / k;
+                p0 = p1;
+                p1 = p2;
+            }
+            if (n == 1) p0 = 1.0L;
+            dp = n * (z * p1 - p0) / (z * z - 1.0L);
+            const long double dz = p1 / dp;
+            z -= dz;
+            if (fabsl(dz) < 1e-30L) break;
+        }
+        long double p0 = 1.0L, p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            const long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0L);
+        // ascending order: node i (largest first from the guess) goes to n-1-i
+        x[n - 1 - i] = (double)z;
+        w[n - 1 - i] = (double)(2.0L / ((1.0L - z * z) * dp * dp));
+    }
+    nlo::clear_error();
+    return NLO_OK;
+}
+
+int nlo_sobol_draw(const nlo_sobol_table* table, int64_t first_index, int64_t n, int32_t out_kind,
+                   void* d_out, hipStream_t stream) {
+    NLO_REQUIRE(table, "nlo_sobol_draw: null table");
+    NLO_REQUIRE(table->dim >= 1 && table->dim <= NLO_SOBOL_MAXDIM,
+                "nlo_sobol_draw: dim=%d outside [1, %d]", table->dim, NLO_SOBOL_MAXDIM);
+    NLO_REQUIRE(first_index >= 0 && n >= 0 && first_index + n <= (int64_t(1) << 30),
+                "nlo_sobol_draw: indices [%lld, %lld) outside [0, 2^30)", (long long)first_index,
+                (long long)(first_index + n));
+    NLO_REQUIRE(out_kind == NLO_SOBOL_U32 || out_kind == NLO_SOBOL_F32 || out_kind == NLO_SOBOL_F64,
+                "nlo_sobol_draw: unknown out_kind %d", out_kind);
+    NLO_REQUIRE(d_out || n == 0, "nlo_sobol_draw: null output");
+    if (n == 0) return NLO_OK;
+    const int64_t groups = ((first_index + n + 63) >> 6) - (first_index >> 6);
+    const int64_t waves_per_block = 4;
+    int64_t blocks = (groups + waves_per_block - 1) / waves_per_block;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(nlo::sobol_draw_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, *table,
+                       first_index, n, out_kind, d_out);
+    NLO_LAUNCH_CHECK();
+    nlo::clear_error();
+    return NLO_OK;
+}
+
+int nlo_vegas_map(int32_t dim, int32_t bins, int64_t n, const double* d_u, const double* d_edges,
+                  const double* d_lo, const double* d_width, double* d_x, double* d_jac,
+                  uint8_t* d_bin, hipStream_t stream) {
+    NLO_REQUIRE(dim >= 1 && dim <= 16, "nlo_vegas_map: dim=%d outside [1, 16]", dim);
+    NLO_REQUIRE(bins >= 2 && bins <= 255, "nlo_vegas_map: bins=%d outside [2, 255]", bins);
+    NLO_REQUIRE(n >= 0, "nlo_vegas_map: negative n");
+    NLO_REQUIRE(n == 0 || (d_u && d_edges && d_lo && d_width && d_x && d_jac),
+                "nlo_vegas_map: null pointer");
+    if (n == 0) return NLO_OK;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(nlo::vegas_map_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, dim, bins,
+                       n, d_u, d_edges, d_lo, d_width, d_x, d_jac, d_bin);
+    NLO_LAUNCH_CHECK();
+    nlo::clear_error();
+    return NLO_OK;
+}
+
+size_t nlo_vegas_histogram_workspace(int32_t dim, int32_t bins, int64_t n) {
+    if (dim < 1 || bins < 1 || n < 0) return 0;
+    const int64_t blocks = (n + nlo::kHistChunk - 1) / nlo::kHistChunk;
+    return (size_t)(blocks > 0 ? blocks : 1) * dim * bins * 2 * sizeof(double);
+}
+
+int nlo_vegas_histogram(int32_t dim, int32_t bins, int64_t n, const uint8_t* d_bin,
+                        const double* d_absw, double* d_hist, double* d_count, void* d_workspace,
+                        size_t workspace_bytes, hipStream_t stream) {
+    NLO_REQUIRE(dim >= 1 && dim <= 16, "nlo_vegas_histogram: dim=%d outside [1, 16]", dim);
+    NLO_REQUIRE(bins >= 2 && bins <= 255, "nlo_vegas_histogram: bins=%d outside [2, 255]", bins);
+    NLO_REQUIRE(n >= 1, "nlo_vegas_histogram: n must be positive");
+    NLO_REQUIRE(d_bin && d_absw && d_hist && d_count, "nlo_vegas_histogram: null pointer");
+    const size_t need = nlo_vegas_histogram_workspace(dim, bins, n);
+    if (workspace_bytes < need || !d_workspace) {
+        nlo::set_error("nlo_vegas_histogram: workspace %zu bytes < %zu", workspace_bytes, need);
+        return NLO_ERR_WORKSPACE;
+    }
+    const int blocks = (int)((n + nlo::kHistChunk - 1) / nlo::kHistChunk);
+    double* ph = static_cast<double*>(d_workspace);
+    double* pc = ph + (size_t)blocks * dim * bins;
+    const size_t smem = (size_t)dim * bins * (sizeof(unsigned long long) + sizeof(unsigned int)) + 8;
+    hipLaunchKernelGGL(nlo::vegas_hist_block_kernel, dim3(blocks), dim3(nlo::kHistThreads), smem, stream,
+                       dim, bins, n, d_bin, d_absw, ph, pc);
+    NLO_LAUNCH_CHECK();
+    const int values = dim * bins;
+    hipLaunchKernelGGL(nlo::vegas_hist_reduce_kernel, dim3((values + 255) / 256), dim3(256), 0, stream,
+                       values, blocks, ph, pc, d_hist, d_count);
+    NLO_LAUNCH_CHECK();
+    nlo::clear_error();
+    return NLO_OK;
+}
+
+int nlo_vegas_adapt(int32_t rows, int32_t dim, int32_t bins, const double* d_edges,
+                    const double* d_hist, const double* d_count, double eps, double* d_edges_out,
+                    hipStream_t stream) {
+    NLO_REQUIRE(rows >= 1 && dim >= 1 && bins >= 2 && bins <= 4096,
+                "nlo_vegas_adapt: rows=%d dim=%d bins=%d invalid", rows, dim, bins);
+    NLO_REQUIRE(d_edges && d_hist && d_count && d_edges_out, "nlo_vegas_adapt: null pointer");
+    NLO_REQUIRE(d_edges != d_edges_out, "nlo_vegas_adapt: in-place refinement is not supported");
+    const int rc = nlo::launch_vegas_adapt(rows, dim, bins, d_edges, d_hist, d_count, eps,
+                                           d_edges_out, stream);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,339 @@
+"""Torch-facing wrappers of the C ABI (one function per entry point).
+
+Each wrapper checks shapes and dtypes on the host, allocates outputs and
+workspace with torch on the tensors' device, and launches on the current
+stream.  Reference prefixes: S/ = torch_nlo_clean/src/nlo_torch/,
+K/ = S/custom_kernels/.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from nlo_bk_optimized_amd import _lib
+from nlo_bk_optimized_amd._lib import check, lib, ptr, require_cuda, stream_handle
+
+# ----------------------------------------------------------------- Sobol
+
+
+def sobol_table(dim: int, seed: int) -> _lib.SobolTable:
+    """Scrambled direction numbers of ``SobolEngine(dim, scramble=True, seed=seed)``.
+
+    torch computes the scramble (S/dis/fixed_sobol.py:87-93 draws from the
+    same engine); the library runs the gray-code recurrence.  ``first`` is the
+    integer form of the engine's float64 first point, float32(shift / 2^30)
+    widened (torch/quasirandom.py ``_first_point``).
+    """
+
+    if not 1 <= dim <= _lib.SOBOL_MAXDIM:
+        raise ValueError(f"dim must lie in [1, {_lib.SOBOL_MAXDIM}]")
+    engine = torch.quasirandom.SobolEngine(dim, scramble=True, seed=seed)
+    direction = engine.sobolstate.numpy().astype(np.uint32)
+    shift = engine.shift.numpy().astype(np.uint32)
+    first = (shift.astype(np.float64) / 2.0**30).astype(np.float32).astype(np.float64)
+    first_q = np.minimum(np.round(first * 2.0**30), 2.0**30).astype(np.uint32)
+    t = _lib.SobolTable()
+    t.dim = dim
+    for d in range(dim):
+        for b in range(_lib.SOBOL_BITS):
+            t.direction[d][b] = int(direction[d, b])
+        t.shift[d] = int(shift[d])
+        t.first[d] = int(first_q[d])
+    return t
+
+
+def sobol_draw(
+    table: _lib.SobolTable,
+    n: int,
+    *,
+    first_index: int = 0,
+    kind: str = "f64",
+    device: torch.device | str = "cuda",
+) -> torch.Tensor:
+    """Points [first_index, first_index + n) as SobolEngine draws them."""
+
+    kinds = {"u32": (_lib.NLO_SOBOL_U32, torch.int32), "f32": (_lib.NLO_SOBOL_F32, torch.float32),
+             "f64": (_lib.NLO_SOBOL_F64, torch.float64)}
+    if kind not in kinds:
+        raise ValueError(f"kind must be one of {sorted(kinds)}")
+    code, dtype = kinds[kind]
+    out = torch.empty((n, table.dim), dtype=dtype, device=device)
+    check(
+        lib().nlo_sobol_draw(ctypes.byref(table), first_index, n, code, ptr(out), stream_handle(out.device)),
+        "nlo_sobol_draw",
+    )
+    return out
+
+
+# ----------------------------------------------------------------- VEGAS
+
+
+def vegas_map(
+    u: torch.Tensor, edges: torch.Tensor, lo: torch.Tensor, width: torch.Tensor
+) -> tuple[torch.Tensor, torch.Tensor, torch.Tensor]:
+    """(x, jacobian, bins) of unit-cube points u[n, dim] through edges[dim, B + 1]."""
+
+    require_cuda(u, torch.float64, "u")
+    require_cuda(edges, torch.float64, "edges")
+    require_cuda(lo, torch.float64, "lo")
+    require_cuda(width, torch.float64, "width")
+    n, dim = u.shape
+    bins = edges.shape[1] - 1
+    if edges.shape != (dim, bins + 1) or lo.shape != (dim,) or width.shape != (dim,):
+        raise ValueError("edges must be [dim, bins + 1] and lo, width [dim]")
+    x = torch.empty_like(u)
+    jac = torch.empty(n, dtype=torch.float64, device=u.device)
+    b = torch.empty((n, dim), dtype=torch.uint8, device=u.device)
+    check(
+        lib().nlo_vegas_map(dim, bins, n, ptr(u), ptr(edges), ptr(lo), ptr(width), ptr(x), ptr(jac),
+                            ptr(b), stream_handle(u.device)),
+        "nlo_vegas_map",
+    )
+    return x, jac, b
+
+
+def vegas_histogram(bins_idx: torch.Tensor, absw: torch.Tensor, bins: int) -> tuple[torch.Tensor, torch.Tensor]:
+    """Deterministic per-bin sum |w| and hit counts, each [dim, bins]."""
+
+    require_cuda(bins_idx, torch.uint8, "bins_idx")
+    require_cuda(absw, torch.float64, "absw")
+    n, dim = bins_idx.shape
+    if absw.shape != (n,):
+        raise ValueError("absw must hold one weight per sample")
+    hist = torch.empty((dim, bins), dtype=torch.float64, device=absw.device)
+    count = torch.empty_like(hist)
+    nbytes = lib().nlo_vegas_histogram_workspace(dim, bins, n)
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=absw.device)
+    check(
+        lib().nlo_vegas_histogram(dim, bins, n, ptr(bins_idx), ptr(absw), ptr(hist), ptr(count),
+                                  ptr(ws), nbytes, stream_handle(absw.device)),
+        "nlo_vegas_histogram",
+    )
+    return hist, count
+
+
+def vegas_adapt(edges: torch.Tensor, hist: torch.Tensor, count: torch.Tensor, eps: float) -> torch.Tensor:
+    """Refined edges for grids [..., dim, B + 1] (S/numerics/integration.py:804-866)."""
+
+    require_cuda(edges, torch.float64, "edges")
+    require_cuda(hist, torch.float64, "hist")
+    require_cuda(count, torch.float64, "count")
+    bins = edges.shape[-1] - 1
+    dim = edges.shape[-2]
+    rows = edges.numel() // (dim * (bins + 1))
+    if hist.shape != edges.shape[:-1] + (bins,) or count.shape != hist.shape:
+        raise ValueError("hist and count must be [..., dim, bins] matching edges")
+    out = torch.empty_like(edges)
+    check(
+        lib().nlo_vegas_adapt(rows, dim, bins, ptr(edges), ptr(hist), ptr(count), eps, ptr(out),
+                              stream_handle(edges.device)),
+        "nlo_vegas_adapt",
+    )
+    return out
+
+
+# ------------------------------------------------------------------- BK
+
+
+def physics(config) -> _lib.Physics:
+    p = _lib.Physics()
+    p.nc = float(config.NC)
+    p.nf = float(config.NF)
+    p.lambda_qcd = float(config.LambdaQCD)
+    p.c2 = float(config.C2)
+    p.ksub = float(config.KSUB)
+    p.minr = float(config.MINR)
+    p.symmetrize = int(bool(config.SYMMETRIZE_Z_Z2_INTEGRATION))
+    return p
+
+
+def spline_build(r: torch.Tensor, N: torch.Tensor, force_positive: bool = True) -> tuple[torch.Tensor, torch.Tensor]:
+    """FP64 and FP32 log-log spline tables of N and S = 1 - N."""
+
+    require_cuda(r, torch.float64, "r")
+    require_cuda(N, torch.float64, "N")
+    R = r.numel()
+    t64 = torch.empty(_lib.spline_words(R), dtype=torch.float64, device=r.device)
+    t32 = torch.empty(_lib.spline_words(R), dtype=torch.float32, device=r.device)
+    check(
+        lib().nlo_bk_spline_build(R, ptr(r), ptr(N), int(force_positive), ptr(t64), ptr(t32),
+                                  stream_handle(r.device)),
+        "nlo_bk_spline_build",
+    )
+    return t64, t32
+
+
+def bk_k1(r: torch.Tensor, N: torch.Tensor, t64: torch.Tensor, config) -> torch.Tensor:
+    """K1 for every r node (0 for index 0 and N > 0.99999)."""
+
+    R = r.numel()
+    out = torch.empty(R, dtype=torch.float64, device=r.device)
+    ph = physics(config)
+    check(
+        lib().nlo_bk_k1(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
+                        config.K1_ANGULAR_ORDER, ptr(out), stream_handle(r.device)),
+        "nlo_bk_k1",
+    )
+    return out
+
+
+PRECISION_CODES = {"fp64": _lib.NLO_PREC_FP64, "fp32": _lib.NLO_PREC_FP32,
+                   "fp64_stable": _lib.NLO_PREC_FP64_STABLE}
+
+
+def bk_nlo(
+    r: torch.Tensor,
+    N: torch.Tensor,
+    t64: torch.Tensor,
+    t32: torch.Tensor,
+    edges: torch.Tensor,
+    table: _lib.SobolTable,
+    n: int,
+    config,
+    *,
+    first_index: int = 0,
+    precision: str = "fp64",
+    histogram: bool = False,
+):
+    """Per-parent (sum w, sum w^2, sum w K2, sum w Kf) [R, 4] (+ hist, count)."""
+
+    R = r.numel()
+    bins = edges.shape[-1] - 1
+    require_cuda(edges, torch.float64, "edges")
+    if edges.shape != (R, 4, bins + 1):
+        raise ValueError("edges must be [R, 4, bins + 1]")
+    sums = torch.empty((R, 4), dtype=torch.float64, device=r.device)
+    hist = torch.empty((R, 4, bins), dtype=torch.float64, device=r.device) if histogram else None
+    count = torch.empty_like(hist) if histogram else None
+    nbytes = lib().nlo_bk_nlo_workspace(R, bins, n, int(histogram))
+    ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=r.device)
+    ph = physics(config)
+    check(
+        lib().nlo_bk_nlo(R, ptr(r), ptr(N), ptr(t64), ptr(t32), ptr(edges), bins, ctypes.byref(table),
+                         first_index, n, PRECISION_CODES[precision], ctypes.byref(ph), ptr(sums),
+                         ptr(hist), ptr(count), ptr(ws), nbytes, stream_handle(r.device)),
+        "nlo_bk_nlo",
+    )
+    return sums, hist, count
+
+
+def rhs_finalize(N: torch.Tensor, k1: torch.Tensor, sums: torch.Tensor, n: int) -> tuple[torch.Tensor, torch.Tensor]:
+    """dN/dy and the K2 + Kf part (S/bk/evolution.py:340-351)."""
+
+    R = N.numel()
+    dndy = torch.empty(R, dtype=torch.float64, device=N.device)
+    k2kf = torch.empty(R, dtype=torch.float64, device=N.device)
+    check(
+        lib().nlo_bk_rhs_finalize(R, ptr(N), ptr(k1), ptr(sums), n, ptr(dndy), ptr(k2kf),
+                                  stream_handle(N.device)),
+        "nlo_bk_rhs_finalize",
+    )
+    return dndy, k2kf
+
+
+RK_CODES = {"euler": _lib.NLO_RK_EULER, "rk2": _lib.NLO_RK_MIDPOINT, "kutta3": _lib.NLO_RK_KUTTA3,
+            "rk4": _lib.NLO_RK_CLASSIC4}
+
+
+class Evolver:
+    """One captured hipGraph per RK step (C ABI ``nlo_bk_evolver_*``)."""
+
+    def __init__(
+        self,
+        r: torch.Tensor,
+        edges: torch.Tensor,
+        table: _lib.SobolTable,
+        config,
+        *,
+        n_samples: int,
+        precision: str,
+        rk_method: str,
+        step: float,
+        nlo: bool = True,
+        first_index: int = 0,
+    ) -> None:
+        require_cuda(r, torch.float64, "r")
+        require_cuda(edges, torch.float64, "edges")
+        self.r = r
+        self.edges = edges
+        d = _lib.EvolverDesc()
+        d.R = r.numel()
+        d.bins = edges.shape[-1] - 1
+        d.n_samples = n_samples
+        d.first_index = first_index
+        d.precision = PRECISION_CODES[precision]
+        d.rk_method = RK_CODES[rk_method]
+        d.step = step
+        d.force_positive = int(bool(config.FORCE_POSITIVE_N))
+        d.nr = config.K1_RADIAL_ORDER
+        d.nt = config.K1_ANGULAR_ORDER
+        d.nlo = int(nlo)
+        d.physics = physics(config)
+        d.sobol = table
+        self.desc = d
+        nbytes = lib().nlo_bk_evolver_workspace(ctypes.byref(d))
+        if nbytes == 0:
+            raise ValueError("invalid evolver description")
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=r.device)
+        handle = ctypes.c_void_p()
+        check(
+            lib().nlo_bk_evolver_create(ctypes.byref(d), ptr(r), ptr(edges), ptr(self.workspace), nbytes,
+                                        stream_handle(r.device), ctypes.byref(handle)),
+            "nlo_bk_evolver_create",
+        )
+        self.handle = handle
+        base = self.workspace.data_ptr()
+        R = d.R
+
+        def view(address: int, count: int, dtype: torch.dtype) -> torch.Tensor:
+            offset = address - base
+            size = torch.empty((), dtype=dtype).element_size()
+            if offset < 0 or offset % size or offset + count * size > nbytes:
+                raise RuntimeError("evolver buffer outside its workspace")
+            return self.workspace[offset : offset + count * size].view(dtype)
+
+        self.state = view(lib().nlo_bk_evolver_state(handle), R, torch.float64)
+        self.k1 = view(lib().nlo_bk_evolver_k1(handle), R, torch.float64)
+        self.k2kf = view(lib().nlo_bk_evolver_k2kf(handle), R, torch.float64)
+        self.flag = view(lib().nlo_bk_evolver_nonfinite_flag(handle), 1, torch.int32)
+
+    def step(self) -> None:
+        check(lib().nlo_bk_evolver_step(self.handle, stream_handle(self.r.device)), "nlo_bk_evolver_step")
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            torch.cuda.current_stream(self.r.device).synchronize()
+            lib().nlo_bk_evolver_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dtype_epsilon(dtype: torch.dtype) -> float:
+    return float(torch.finfo(dtype).eps)
+
+
+__all__ = [
+    "Evolver",
+    "PRECISION_CODES",
+    "RK_CODES",
+    "bk_k1",
+    "bk_nlo",
+    "dtype_epsilon",
+    "physics",
+    "rhs_finalize",
+    "sobol_draw",
+    "sobol_table",
+    "spline_build",
+    "vegas_adapt",
+    "vegas_histogram",
+    "vegas_map",
+]
+

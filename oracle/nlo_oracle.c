@@ -1,0 +1,676 @@
+/*
+ * FP64 CPU oracle for the NLO BK right-hand side (test infrastructure only;
+ * see nlo_oracle.h).  Each function names the reference lines it restates.
+ * S/ = torch_nlo_clean/src/nlo_torch/, K/ = S/custom_kernels/.
+ */
+#define _DEFAULT_SOURCE 1 /* j1() from the C library */
+#include "nlo_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------- coupling */
+
+/* S/coupling.py:81-97 (_smooth_alpha_s): alpha = 1 / (b0 * 0.2 *
+ * logaddexp(2/0.2 ln 2.5, ln(4 C2 / (r^2 Lambda^2)) / 0.2)). */
+double orc_alpha_s(double r, const orc_bk_params* p) {
+    const double b0 = (11.0 * p->nc - 2.0 * p->nf) / (12.0 * M_PI);
+    const double left = 2.0 / 0.2 * log(2.5);
+    const double scale = 4.0 * p->c2 / (r * r * p->lambda_qcd * p->lambda_qcd);
+    const double right = log(scale) / 0.2;
+    const double hi = left > right ? left : right;
+    const double lae = hi + log1p(exp(-fabs(left - right)));
+    return 1.0 / (b0 * (0.2 * lae));
+}
+
+/* S/bk/initial_conditions.py:29-35. The constant e is truncated to 2.7182818. */
+double orc_mv(double r, double qs0sqr, double gamma, double ec, double lambda_qcd) {
+    const double exponent =
+        pow(r * r * qs0sqr, gamma) / 4.0 * log(1.0 / (r * lambda_qcd) + ec * 2.7182818);
+    return exponent < 1e-5 ? exponent : 1.0 - exp(-exponent);
+}
+
+/* S/bk/evolution.py:584-592: r_i = MINR * step^i, step = (MAXR/MINR)^(1/RPOINTS). */
+void orc_r_grid(double minr, double maxr, int rpoints, double* out) {
+    const double step = pow(maxr / minr, 1.0 / (double)rpoints);
+    for (int i = 0; i <= rpoints; ++i) out[i] = minr * pow(step, (double)i);
+}
+
+/* ----------------------------------------------------------------- splines */
+
+/* S/numerics/interpolation.py:27-48 and :124-149 (natural spline, Thomas). */
+void orc_natural_spline(int n, const double* x, const double* y, double* a, double* b,
+                        double* c, double* d) {
+    const int m = n - 2; /* interior second derivatives */
+    double* h = (double*)malloc(sizeof(double) * (size_t)(n - 1));
+    double* sec = (double*)calloc((size_t)n, sizeof(double));
+    double* upper = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    double* sol = (double*)malloc(sizeof(double) * (size_t)(m > 0 ? m : 1));
+    for (int i = 0; i < n - 1; ++i) h[i] = x[i + 1] - x[i];
+    if (m == 1) {
+        const double diag = 2.0 * (h[0] + h[1]);
+        const double rhs = 6.0 * ((y[2] - y[1]) / h[1] - (y[1] - y[0]) / h[0]);
+        sol[0] = rhs / diag;
+    } else if (m > 1) {
+        /* row i: off-diagonal h[i] couples unknowns i-1, i (interior index). */
+        upper[0] = h[1] / (2.0 * (h[0] + h[1]));
+        sol[0] = 6.0 * ((y[2] - y[1]) / h[1] - (y[1] - y[0]) / h[0]) / (2.0 * (h[0] + h[1]));
+        for (int i = 1; i < m; ++i) {
+            const double diag = 2.0 * (h[i] + h[i + 1]);
+            const double off = h[i];
+            const double rhs =
+                6.0 * ((y[i + 2] - y[i + 1]) / h[i + 1] - (y[i + 1] - y[i]) / h[i]);
+            const double den = diag - off * upper[i - 1];
+            if (i < m - 1) upper[i] = h[i + 1] / den;
+            sol[i] = (rhs - off * sol[i - 1]) / den;
+        }
+        for (int i = m - 2; i >= 0; --i) sol[i] = sol[i] - upper[i] * sol[i + 1];
+    }
+    for (int i = 0; i < m; ++i) sec[i + 1] = sol[i];
+    for (int i = 0; i < n - 1; ++i) {
+        a[i] = y[i];
+        b[i] = (y[i + 1] - y[i]) / h[i] - h[i] * (2.0 * sec[i] + sec[i + 1]) / 6.0;
+        c[i] = sec[i] / 2.0;
+        d[i] = (sec[i + 1] - sec[i]) / (6.0 * h[i]);
+    }
+    free(h);
+    free(sec);
+    free(upper);
+    free(sol);
+}
+
+/* S/bk/evolution.py:548-560 (clamps) + S/numerics/interpolation.py:98-103
+ * (log y with the -40 floor). */
+void orc_evolution_splines(int n, const double* r, const double* N, int force_positive,
+                           double* x_out, double* coef_n, double* coef_s) {
+    double* ly = (double*)malloc(sizeof(double) * (size_t)n);
+    const int m = n - 1;
+    for (int i = 0; i < n; ++i) x_out[i] = log(r[i]);
+    for (int i = 0; i < n; ++i) {
+        double v = N[i] > 1.0 ? 1.0 : N[i];
+        if (force_positive && v < 0.0) v = 0.0;
+        ly[i] = v > 0.0 ? log(v) : -40.0;
+    }
+    orc_natural_spline(n, x_out, ly, coef_n, coef_n + m, coef_n + 2 * m, coef_n + 3 * m);
+    for (int i = 0; i < n; ++i) {
+        double s = 1.0 - N[i];
+        if (s < 0.0) s = 0.0;
+        if (force_positive && s > 1.0) s = 1.0;
+        ly[i] = s > 0.0 ? log(s) : -40.0;
+    }
+    orc_natural_spline(n, x_out, ly, coef_s, coef_s + m, coef_s + 2 * m, coef_s + 3 * m);
+    free(ly);
+}
+
+/* S/numerics/interpolation.py:54-60: interval = searchsorted(x, v, right) - 1,
+ * clamped to [0, n - 2]. */
+double orc_spline_eval(const orc_spline* s, double v) {
+    int lo = 0, hi = s->n;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (v >= s->x[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    int k = lo - 1;
+    if (k < 0) k = 0;
+    if (k > s->n - 2) k = s->n - 2;
+    const double dx = v - s->x[k];
+    return s->a[k] + dx * (s->b[k] + dx * (s->c[k] + dx * s->d[k]));
+}
+
+/* S/bk/derivatives.py:761-772 (_evaluate_loglog). */
+double orc_loglog_eval(const orc_spline* s, double value, double underflow, double overflow) {
+    double ev = value < s->r_min ? s->r_min : value;
+    ev = ev > s->r_max ? s->r_max : ev;
+    double out = exp(orc_spline_eval(s, log(ev)));
+    if (!isfinite(out)) out = 0.0;
+    if (value < s->r_min) out = underflow;
+    if (value > s->r_max) out = overflow;
+    return out;
+}
+
+/* ------------------------------------------------------------------ Bessel */
+
+/* J1 from the C library (glibc j1). */
+double orc_bessel_j1(double x) { return j1(x); }
+
+/* I1 by its power series sum_k (x/2)^(2k+1) / (k! (k+1)!), all terms positive,
+ * accumulated in long double (no cancellation). */
+double orc_bessel_i1(double x) {
+    const long double h = (long double)fabs(x) / 2.0L;
+    const long double h2 = h * h;
+    long double term = h, sum = h;
+    for (int k = 1; k < 400; ++k) {
+        term *= h2 / ((long double)k * (long double)(k + 1));
+        sum += term;
+        if (term < sum * 1e-22L) break;
+    }
+    return x < 0 ? -(double)sum : (double)sum;
+}
+
+/* ---------------------------------------------------------------- K1 (LO) */
+
+static double sqr(double v) { return v * v; }
+
+static double dist_sq(double r, double z, double theta, int cos_form) {
+    if (cos_form) return r * r + z * z - 2.0 * r * z * cos(theta);
+    const double s = sin(theta / 2.0);
+    return (r - z) * (r - z) + 4.0 * r * z * s * s;
+}
+
+/* S/bk_kernels/k1.py:20-155 for RC_LO = BALITSKY_LO, RESUM_RC = SMALLEST,
+ * Order = NLO_RESUM_DLOG_SLOG (dlog = 0 in k1fin). */
+static double kernel_lo_impl(double r, double z, double theta, const orc_bk_params* p,
+                             double* magnitude) {
+    const double Y = z;
+    const double Xsqr = dist_sq(r, z, theta, p->geometry_cos);
+    int invalid = Xsqr <= 0.0;
+    const double X = sqrt(Xsqr > 0.0 ? Xsqr : 0.0);
+    invalid = invalid || X < 1e-20 || Y < 1e-20;
+    const double sX = invalid ? 1.0 : X;
+    const double sY = invalid ? 1.0 : Y;
+    double mn = sX < sY ? sX : sY;
+    mn = mn < r ? mn : r;
+    const double a_r = orc_alpha_s(r, p);
+    const double a_Y = orc_alpha_s(sY, p);
+    const double a_X = orc_alpha_s(sX, p);
+    double lo = p->nc / (2.0 * M_PI * M_PI) * a_r *
+                (r * r / (sX * sX * sY * sY) + (a_Y / a_X - 1.0) / (sY * sY) +
+                 (a_X / a_Y - 1.0) / (sX * sX));
+    if (!isfinite(lo)) lo = 0.0;
+    if (invalid) lo = 0.0;
+
+    const double a_min = orc_alpha_s(mn, p);
+    const double xl = 4.0 * log(sX / r) * log(sY / r);
+    const double as_x = sqrt(a_min * p->nc / M_PI * fabs(xl));
+    const double safe = as_x != 0.0 ? as_x : 1.0;
+    double dlog = xl >= 0.0 ? orc_bessel_j1(2.0 * as_x) / safe : orc_bessel_i1(2.0 * as_x) / safe;
+    if (as_x == 0.0) dlog = 1.0;
+    if (!(r > 1.01 * p->minr)) dlog = 1.0;
+    const int resum_valid = isfinite(dlog);
+
+    const double minxy = sX < sY ? sX : sY;
+    const double abar = a_min * p->nc / M_PI;
+    const double A1 = 11.0 / 12.0;
+    const double slog = exp(-abar * A1 * fabs(log(p->ksub * sqr(r / minxy))));
+    const double slog_exp = -abar * A1 * fabs(2.0 * log(sqrt(p->ksub) * r / minxy));
+    const double lo_single = a_r * p->nc / (2.0 * M_PI * M_PI) * sqr(r / (sX * sY));
+    const double subtract = lo_single * slog_exp;
+    const double k1fin = lo * a_r * p->nc / (4.0 * M_PI) *
+                         (67.0 / 9.0 - M_PI * M_PI / 3.0 - 10.0 / 9.0 * p->nf / p->nc);
+    const double result = dlog * slog * lo - subtract + k1fin;
+    if (magnitude) {
+        const double lo_abs = p->nc / (2.0 * M_PI * M_PI) * a_r *
+                              (r * r / (sX * sX * sY * sY) + fabs(a_Y / a_X - 1.0) / (sY * sY) +
+                               fabs(a_X / a_Y - 1.0) / (sX * sX));
+        *magnitude = (invalid || !resum_valid)
+                         ? 0.0
+                         : fabs(dlog * slog) * lo_abs + fabs(subtract) + fabs(k1fin) +
+                               fabs(lo_abs * a_r * p->nc / (4.0 * M_PI) *
+                                    (67.0 / 9.0 - M_PI * M_PI / 3.0 - 10.0 / 9.0 * p->nf / p->nc));
+    }
+    return (invalid || !resum_valid) ? 0.0 : result;
+}
+
+double orc_kernel_lo(double r, double z, double theta, const orc_bk_params* p) {
+    return kernel_lo_impl(r, z, theta, p, NULL);
+}
+
+/* Magnitude of Kernel_lo with every additive piece taken in absolute value
+ * (used by tests to scale tolerances where the reference formula cancels). */
+double orc_kernel_lo_magnitude(double r, double z, double theta, const orc_bk_params* p) {
+    double m = 0.0;
+    (void)kernel_lo_impl(r, z, theta, p, &m);
+    return m;
+}
+
+/* S/bk/derivatives.py:434-468 (_lo_integrand without kinematical constraint). */
+double orc_k1_integrand(double r, double z, double theta, const orc_spline* sN,
+                        const orc_bk_params* p) {
+    const double Xsqr = dist_sq(r, z, theta, p->geometry_cos);
+    const int invalid = Xsqr < 1e-40 || z < 1e-20 || r < 1e-20;
+    const double X = sqrt(Xsqr > 0.0 ? Xsqr : 0.0);
+    const double sX = invalid ? 1.0 : X;
+    const double sY = invalid ? 1.0 : z;
+    double nX = orc_loglog_eval(sN, sX, 0.0, 1.0);
+    double nY = orc_loglog_eval(sN, sY, 0.0, 1.0);
+    double nr = orc_loglog_eval(sN, r, 0.0, 1.0);
+    nX = nX < 1.0 ? nX : 1.0;
+    nY = nY < 1.0 ? nY : 1.0;
+    nr = nr < 1.0 ? nr : 1.0;
+    const double dip = nX + nY - nr - nX * nY;
+    const double v = orc_kernel_lo(r, z, theta, p) * dip;
+    return invalid ? 0.0 : v;
+}
+
+double orc_k1_integrand_magnitude(double r, double z, double theta, const orc_spline* sN,
+                                  const orc_bk_params* p) {
+    const double Xsqr = dist_sq(r, z, theta, p->geometry_cos);
+    const int invalid = Xsqr < 1e-40 || z < 1e-20 || r < 1e-20;
+    if (invalid) return 0.0;
+    const double X = sqrt(Xsqr > 0.0 ? Xsqr : 0.0);
+    double nX = orc_loglog_eval(sN, X, 0.0, 1.0);
+    double nY = orc_loglog_eval(sN, z, 0.0, 1.0);
+    double nr = orc_loglog_eval(sN, r, 0.0, 1.0);
+    nX = nX < 1.0 ? nX : 1.0;
+    nY = nY < 1.0 ? nY : 1.0;
+    nr = nr < 1.0 ? nr : 1.0;
+    return orc_kernel_lo_magnitude(r, z, theta, p) * (nX + nY + nr + nX * nY);
+}
+
+/* K/bk/k1/kernel.cu:2040-2092: sum over panels [x_i, x_{i+1}] of the log-r
+ * grid, GL(nr) in log z and GL(nt) in v in [0, 1] with theta = pi v^2,
+ * integrand * 2 z^2 * (half_width w_r) * (w_t pi v). */
+void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN, int nr,
+                  const double* gr_x, const double* gr_w, int nt, const double* gt_x,
+                  const double* gt_w, const orc_bk_params* p, double* out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int pi = 0; pi < parents; ++pi) {
+        const double r = parent_r[pi];
+        long double total = 0.0L;
+        for (int panel = 0; panel < sN->n - 1; ++panel) {
+            const double lower = sN->x[panel], upper = sN->x[panel + 1];
+            const double center = (lower + upper) / 2.0, half = (upper - lower) / 2.0;
+            for (int i = 0; i < nr; ++i) {
+                const double log_z = center + half * gr_x[i];
+                const double z = exp(log_z);
+                const double wr = half * gr_w[i];
+                for (int j = 0; j < nt; ++j) {
+                    const double v = (gt_x[j] + 1.0) / 2.0;
+                    const double theta = M_PI * v * v;
+                    const double wt = gt_w[j] * M_PI * v;
+                    total += (long double)(orc_k1_integrand(r, z, theta, sN, p) * (2.0 * z * z) *
+                                           wr * wt);
+                }
+            }
+        }
+        out[pi] = (double)total;
+    }
+}
+
+/* ----------------------------------------------------------- K2, Kf (NLO) */
+
+/* S/bk_kernels/k2.py:8-39. */
+double orc_kernel_nlo(double r, double X, double Y, double X2, double Y2, double zmz2) {
+    if (X < 1e-20 || X2 < 1e-20 || Y < 1e-20 || Y2 < 1e-20 || zmz2 < 1e-20) return 0.0;
+    const double XY2sq = sqr(X * Y2), X2Ysq = sqr(X2 * Y);
+    const double diff = XY2sq - X2Ysq;
+    const double z4 = pow(zmz2, 4.0);
+    double k = -2.0 / z4;
+    k = k + ((XY2sq + X2Ysq - 4.0 * sqr(r * zmz2)) / (z4 * diff) + pow(r, 4.0) / (XY2sq * diff) +
+             r * r / sqr(X * Y2 * zmz2)) *
+                2.0 * log(X * Y2 / (X2 * Y));
+    return isfinite(k) ? k : 0.0;
+}
+
+/* Rounding-error scale of Kernel_nlo / Kernel_nlo_fermion: every additive
+ * term in absolute value, the Delta-divided ones amplified by the relative
+ * size of Delta = (X Y2)^2 - (X2 Y)^2 (its subtraction loses that many digits). */
+static double kernel_nlo_scale(double r, double X, double Y, double X2, double Y2, double zmz2,
+                               int fermion) {
+    if (X < 1e-20 || X2 < 1e-20 || Y < 1e-20 || Y2 < 1e-20 || zmz2 < 1e-20) return 0.0;
+    const double XY2sq = sqr(X * Y2), X2Ysq = sqr(X2 * Y);
+    const double diff = XY2sq - X2Ysq;
+    const double rel = fabs(diff) / (XY2sq + X2Ysq);
+    const double amp = 1.0 / (rel > 1e-300 ? rel : 1e-300);
+    const double z4 = pow(zmz2, 4.0);
+    const double L = fabs(2.0 * log(X * Y2 / (X2 * Y)));
+    double m;
+    if (fermion) {
+        m = 2.0 / z4 + fabs((XY2sq + X2Ysq + sqr(r * zmz2)) / (z4 * diff)) * L * amp;
+    } else {
+        m = 2.0 / z4 + r * r / sqr(X * Y2 * zmz2) * L +
+            (fabs((XY2sq + X2Ysq + 4.0 * sqr(r * zmz2)) / (z4 * diff)) +
+             fabs(pow(r, 4.0) / (XY2sq * diff))) * L * amp;
+    }
+    return isfinite(m) ? m : 0.0;
+}
+
+/* S/bk_kernels/kf.py:10-41. */
+double orc_kernel_nlo_fermion(double r, double X, double Y, double X2, double Y2, double zmz2,
+                              double nf, double nc) {
+    if (X < 1e-20 || X2 < 1e-20 || Y < 1e-20 || Y2 < 1e-20 || zmz2 < 1e-20) return 0.0;
+    const double XY2sq = sqr(X * Y2), X2Ysq = sqr(X2 * Y);
+    double k = 2.0 / pow(zmz2, 4.0);
+    k = k - ((XY2sq + X2Ysq - sqr(r * zmz2)) / (pow(zmz2, 4.0) * (XY2sq - X2Ysq)) * 2.0 *
+             log(X * Y2 / (X2 * Y)));
+    k = k * nf / nc;
+    return isfinite(k) ? k : 0.0;
+}
+
+/* Kernels times J = z^2 z'^2 in a form free of the 1/Delta cancellation.
+ * With a = X Y', b = X' Y, L = ln(a / b), Delta = a^2 - b^2 = b^2 (e^{2L} - 1):
+ *   (a^2 + b^2) 2L / Delta = g(L) + 1,  g(L) = L coth L - 1,
+ *   2L / Delta = h(L) / b^2,             h(L) = 2L / (e^{2L} - 1),  h(-L) = h(L) e^{2L},
+ * so (S/bk_kernels/k2.py:24-36, kf.py:28-38)
+ *   K2 J  = 2 A^2 g - 4 h B^2 + h C^2 + 2 L D^2
+ *   K2s J = 2 A^2 g - 4 h(-L) D^2 + h(-L) C^2 - 2 L B^2
+ *   Kf J  = (-2 A^2 g + h B^2) Nf / Nc
+ *   Kfs J = (-2 A^2 g + h(-L) D^2) Nf / Nc
+ * with the scale-free ratios A = z z'/Z^2, B = r z'/(X' Z), C = r^2/(X X'),
+ * D = r z/(X Z).  g uses L^2 P(L^2) / Q(L^2) for |L| < 1, where
+ * L cosh L - sinh L = L^3 P(L^2), sinh L = L Q(L^2) (positive series). */
+static const double kGP[11] = {1.0 / 3.0,
+                               1.0 / 30.0,
+                               1.0 / 840.0,
+                               1.0 / 45360.0,
+                               1.0 / 3991680.0,
+                               1.0 / 518918400.0,
+                               1.0 / 93405312000.0,
+                               1.0 / 22230464256000.0,
+                               1.0 / 6758061133824000.0,
+                               1.0 / 2554547108585472000.0,
+                               1.0 / 1175091669949317120000.0};
+static const double kGQ[12] = {1.0,
+                               1.0 / 6.0,
+                               1.0 / 120.0,
+                               1.0 / 5040.0,
+                               1.0 / 362880.0,
+                               1.0 / 39916800.0,
+                               1.0 / 6227020800.0,
+                               1.0 / 1307674368000.0,
+                               1.0 / 355687428096000.0,
+                               1.0 / 121645100408832000.0,
+                               1.0 / 51090942171709440000.0,
+                               1.0 / 25852016738884976640000.0};
+
+double orc_coth_minus_one(double L) {
+    L = fabs(L); /* even function */
+    if (L < 1.0) {
+        const double y = L * L;
+        double pp = kGP[10], qq = kGQ[11];
+        for (int i = 9; i >= 0; --i) pp = pp * y + kGP[i];
+        for (int i = 10; i >= 0; --i) qq = qq * y + kGQ[i];
+        return y * pp / qq;
+    }
+    return L + 2.0 * L / expm1(2.0 * L) - 1.0;
+}
+
+double orc_h_ratio(double L) { return L == 0.0 ? 1.0 : 2.0 * L / expm1(2.0 * L); }
+
+/* ln(X / Y) for X^2 = (r - z)^2 + 4 r z sin^2(theta/2), Y = z:
+ * (X^2 - z^2) / z^2 = rho (rho - 2 cos theta), rho = r / z. */
+double orc_log_x_over_y(double r, double z, double theta, double xsq) {
+    const double rho = r / z;
+    const double u = rho * (rho - 2.0 * cos(theta));
+    if (fabs(u) < 0.5) return 0.5 * log1p(u);
+    return 0.5 * log(xsq) - log(z);
+}
+
+void orc_nlo_kernels_stable(double r, double z, double theta_z, double z2, double theta_z2,
+                            double nf, double nc, double* out /* kJ, ksJ, kfJ, kfsJ */) {
+    const double s1 = sin(theta_z / 2.0), s2 = sin(theta_z2 / 2.0);
+    const double s3 = sin((theta_z - theta_z2) / 2.0);
+    const double Xsq = (r - z) * (r - z) + 4.0 * r * z * s1 * s1;
+    const double X2sq = (r - z2) * (r - z2) + 4.0 * r * z2 * s2 * s2;
+    const double Zsq = (z - z2) * (z - z2) + 4.0 * z * z2 * s3 * s3;
+    const double X = sqrt(Xsq), X2 = sqrt(X2sq), Z = sqrt(Zsq);
+    const double L = orc_log_x_over_y(r, z, theta_z, Xsq) - orc_log_x_over_y(r, z2, theta_z2, X2sq);
+    const double g = orc_coth_minus_one(L);
+    const double h = orc_h_ratio(L);
+    const double hm = h * exp(2.0 * L); /* h(-L), no cancellation for either sign */
+    const double A = z * z2 / Zsq, B = r * z2 / (X2 * Z), C = r * r / (X * X2), D = r * z / (X * Z);
+    out[0] = 2.0 * A * A * g - 4.0 * h * B * B + h * C * C + 2.0 * L * D * D;
+    out[1] = 2.0 * A * A * g - 4.0 * hm * D * D + hm * C * C - 2.0 * L * B * B;
+    out[2] = (-2.0 * A * A * g + h * B * B) * nf / nc;
+    out[3] = (-2.0 * A * A * g + hm * D * D) * nf / nc;
+}
+
+/* S/bk/derivatives.py:490-612 (_nlo_integrand, eager branch) times the
+ * exp(2 log z + 2 log z') Jacobian of S/bk/derivatives.py:322-335. */
+static double nlo_impl(double r, double log_z, double log_z2, double theta_z,
+                       double theta_z2, const orc_spline* sS, const orc_bk_params* p,
+                       double* k2_part, double* kf_part, double* magnitude) {
+    const double z = exp(log_z), z2 = exp(log_z2);
+    const double Xsqr = dist_sq(r, z, theta_z, p->geometry_cos);
+    const double X2sqr = dist_sq(r, z2, theta_z2, p->geometry_cos);
+    const double zzsqr = dist_sq(z, z2, theta_z - theta_z2, p->geometry_cos);
+    const int negative = Xsqr < 0.0 || X2sqr < 0.0 || zzsqr < 0.0;
+    const double X = sqrt(Xsqr > 0.0 ? Xsqr : 0.0);
+    const double X2 = sqrt(X2sqr > 0.0 ? X2sqr : 0.0);
+    const double zz = sqrt(zzsqr > 0.0 ? zzsqr : 0.0);
+    const int invalid =
+        negative || X < 1e-20 || z < 1e-20 || X2 < 1e-20 || z2 < 1e-20 || zz < 1e-20;
+    if (k2_part) *k2_part = 0.0;
+    if (kf_part) *kf_part = 0.0;
+    if (magnitude) *magnitude = 0.0;
+    if (invalid) return 0.0;
+    const double Y = z, Y2 = z2;
+    const double S_X = orc_loglog_eval(sS, X, 1.0, 0.0);
+    const double S_Y = orc_loglog_eval(sS, Y, 1.0, 0.0);
+    const double S_X2 = orc_loglog_eval(sS, X2, 1.0, 0.0);
+    const double S_Y2 = orc_loglog_eval(sS, Y2, 1.0, 0.0);
+    const double S_zz = orc_loglog_eval(sS, zz, 1.0, 0.0);
+
+    double mn = r;
+    const double cand[5] = {X, Y, X2, Y2, zz};
+    for (int i = 0; i < 5; ++i) mn = cand[i] < mn ? cand[i] : mn;
+    const double as = orc_alpha_s(mn, p);
+
+    const double J = exp(2.0 * log_z + 2.0 * log_z2);
+    double k, ks, kf_st = 0.0, kfs_st = 0.0, cut_limit = 1e10;
+    if (p->stable) {
+        double kk[4];
+        orc_nlo_kernels_stable(r, z, theta_z, z2, theta_z2, p->nf, p->nc, kk);
+        k = kk[0];
+        ks = kk[1];
+        kf_st = kk[2];
+        kfs_st = kk[3];
+        cut_limit = 1e10 * J; /* |K2| > 1e10 <=> |K2 J| > 1e10 J */
+    } else {
+        k = orc_kernel_nlo(r, X, Y, X2, Y2, zz);
+        ks = orc_kernel_nlo(r, X2, Y2, X, Y, zz);
+    }
+    double dip = -(S_X * S_zz * S_Y2 - S_X * S_Y);
+    double dips = -(S_X2 * S_zz * S_Y - S_X2 * S_Y2);
+    if (p->stable) {
+        /* dipole combinations in N = 1 - S form (no 1 - S cancellation) */
+        const double N_Y = 1.0 - S_Y, N_Y2 = 1.0 - S_Y2, N_zz = 1.0 - S_zz;
+        dip = S_X * (N_zz + N_Y2 - N_Y - N_zz * N_Y2);
+        dips = S_X2 * (N_zz + N_Y - N_Y2 - N_zz * N_Y);
+    }
+    if (fabs(k) > cut_limit && fabs(dip) < 1e-10) { k = 0.0; dip = 0.0; }
+    if (fabs(ks) > cut_limit && fabs(dips) < 1e-10) { ks = 0.0; dips = 0.0; }
+    double k2r = p->symmetrize ? (k * dip + ks * dips) / 2.0 : k * dip;
+    double kfr = 0.0;
+    double mag = (kernel_nlo_scale(r, X, Y, X2, Y2, zz, 0) *
+                      (fabs(S_X * S_zz * S_Y2) + fabs(S_X * S_Y)) +
+                  kernel_nlo_scale(r, X2, Y2, X, Y, zz, 0) *
+                      (fabs(S_X2 * S_zz * S_Y) + fabs(S_X2 * S_Y2)));
+    if (p->nf > 0.0) {
+        const double kf = p->stable ? kf_st : orc_kernel_nlo_fermion(r, X, Y, X2, Y2, zz, p->nf, p->nc);
+        const double kfs = p->stable ? kfs_st : orc_kernel_nlo_fermion(r, X2, Y2, X, Y, zz, p->nf, p->nc);
+        const double df = S_Y * (S_X2 - S_X);
+        const double dfs = S_Y2 * (S_X - S_X2);
+        kfr = -(p->symmetrize ? (kf * df + kfs * dfs) / 2.0 : kf * df);
+        mag += p->nf / p->nc *
+               (kernel_nlo_scale(r, X, Y, X2, Y2, zz, 1) * fabs(S_Y) * (fabs(S_X2) + fabs(S_X)) +
+                kernel_nlo_scale(r, X2, Y2, X, Y, zz, 1) * fabs(S_Y2) * (fabs(S_X) + fabs(S_X2)));
+    }
+    const double scale = sqr(as * p->nc) / (8.0 * pow(M_PI, 4.0)) * (p->stable ? 1.0 : J);
+    k2r *= scale;
+    kfr *= scale;
+    if (!isfinite(k2r)) k2r = 0.0;
+    if (!isfinite(kfr)) kfr = 0.0;
+    if (k2_part) *k2_part = k2r;
+    if (kf_part) *kf_part = kfr;
+    if (magnitude) *magnitude = isfinite(mag * scale) ? mag * scale : 0.0;
+    return k2r + kfr;
+}
+
+double orc_nlo_integrand_log(double r, double log_z, double log_z2, double theta_z,
+                             double theta_z2, const orc_spline* sS, const orc_bk_params* p,
+                             double* k2_part, double* kf_part) {
+    return nlo_impl(r, log_z, log_z2, theta_z, theta_z2, sS, p, k2_part, kf_part, NULL);
+}
+
+/* Sum of the absolute values of the kernel x dipole products of the NLO
+ * integrand (tolerance scale for tests where the reference formula cancels). */
+double orc_nlo_integrand_magnitude(double r, double log_z, double log_z2, double theta_z,
+                                   double theta_z2, const orc_spline* sS,
+                                   const orc_bk_params* p) {
+    double m = 0.0;
+    (void)nlo_impl(r, log_z, log_z2, theta_z, theta_z2, sS, p, NULL, NULL, &m);
+    return m;
+}
+
+/* ------------------------------------------------------------------- Sobol */
+
+/* Gray-code Sobol (SobolEngine): q_n = shift ^ XOR_{b in gray(n)} v_b. */
+void orc_sobol_u32(int dim, const uint32_t* direction, const uint32_t* shift, int64_t first,
+                   int64_t n, uint32_t* out) {
+    for (int64_t i = 0; i < n; ++i) {
+        const uint64_t idx = (uint64_t)(first + i);
+        const uint64_t g = idx ^ (idx >> 1);
+        for (int d = 0; d < dim; ++d) {
+            uint32_t q = shift[d];
+            for (int b = 0; b < 30; ++b)
+                if ((g >> b) & 1u) q ^= direction[d * 30 + b];
+            out[i * dim + d] = q;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------- VEGAS */
+
+/* u = q / 2^30; bin = min(floor(u B), B - 1); frac = u B - bin
+ * (S/dis/fixed_sobol.py:100-113); x = lo + width (edge_l + (edge_r - edge_l) frac),
+ * jac = prod_d B (edge_r - edge_l) * width_d. */
+static void map_one(int dim, int bins, const uint32_t* q, const double* edges, const double* lo,
+                    const double* width, double* x, double* jac, int32_t* bin) {
+    double j = 1.0;
+    for (int d = 0; d < dim; ++d) {
+        const double u = (double)q[d] / 1073741824.0;
+        const double scaled = u * (double)bins;
+        int b = (int)floor(scaled);
+        if (b > bins - 1) b = bins - 1;
+        const double frac = scaled - (double)b;
+        const double left = edges[d * (bins + 1) + b], right = edges[d * (bins + 1) + b + 1];
+        const double w = right - left;
+        x[d] = lo[d] + width[d] * (left + w * frac);
+        j *= (double)bins * w;
+        if (bin) bin[d] = b;
+    }
+    double vol = 1.0;
+    for (int d = 0; d < dim; ++d) vol *= width[d];
+    *jac = vol * j;
+}
+
+void orc_vegas_map(int dim, int bins, int64_t n, const uint32_t* q, const double* edges,
+                   const double* lo, const double* width, double* x, double* jac, int32_t* bin) {
+    for (int64_t i = 0; i < n; ++i)
+        map_one(dim, bins, q + i * dim, edges, lo, width, x + i * dim, jac + i,
+                bin ? bin + i * dim : NULL);
+}
+
+/* S/numerics/integration.py:828-866 (eager branch). */
+void orc_vegas_adapt(int dim, int bins, const double* edges, const double* hist,
+                     const double* count, double eps, double* out) {
+    double* imp = (double*)malloc(sizeof(double) * (size_t)bins);
+    double* sm = (double*)malloc(sizeof(double) * (size_t)bins);
+    double* cum = (double*)malloc(sizeof(double) * (size_t)(bins + 1));
+    for (int d = 0; d < dim; ++d) {
+        const double* h = hist + d * bins;
+        const double* c = count + d * bins;
+        const double* e = edges + d * (bins + 1);
+        double* o = out + d * (bins + 1);
+        for (int i = 0; i < bins; ++i) imp[i] = h[i] / (c[i] > 1.0 ? c[i] : 1.0);
+        for (int i = 1; i < bins - 1; ++i) sm[i] = (imp[i - 1] + imp[i] + imp[i + 1]) / 3.0;
+        sm[0] = (imp[0] + imp[1]) / 2.0;
+        sm[bins - 1] = (imp[bins - 2] + imp[bins - 1]) / 2.0;
+        for (int i = 0; i < bins; ++i) sm[i] += eps;
+        cum[0] = 0.0;
+        for (int i = 0; i < bins; ++i) cum[i + 1] = cum[i] + sm[i];
+        const double total = cum[bins];
+        for (int t = 0; t <= bins; ++t) {
+            /* torch.linspace(0, total, bins + 1) on the CPU: i * step for the
+             * lower half, fma(-step, steps - 1 - i, end) for the upper half
+             * (matched bit for bit against torch.linspace). */
+            const double step = total / (double)bins;
+            const double target = t < (bins + 1) / 2 ? (double)t * step
+                                                     : fma(-step, (double)(bins - t), total);
+            int lo = 0, hi = bins + 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) / 2;
+                if (target >= cum[mid]) lo = mid + 1;
+                else hi = mid;
+            }
+            int k = lo - 1;
+            if (k < 0) k = 0;
+            if (k > bins - 1) k = bins - 1;
+            const double frac = (target - cum[k]) / sm[k];
+            o[t] = e[k] + frac * (e[k + 1] - e[k]);
+        }
+        o[0] = 0.0;
+        o[bins] = 1.0;
+    }
+    free(imp);
+    free(sm);
+    free(cum);
+}
+
+/* Per-parent VEGAS sums of the K2 + Kf integrand (S/numerics/integration.py:
+ * 392-428 with Sobol points in place of torch.rand / torch.randint). */
+void orc_bk_nlo_sums(int parents, const double* parent_r, const orc_spline* sS, int bins,
+                     const double* edges, const double* lo, const double* width,
+                     const uint32_t* direction, const uint32_t* shift, const uint32_t* first_q,
+                     int64_t first, int64_t n, const orc_bk_params* p, double* sums, double* hist,
+                     double* count) {
+    const int dim = 4;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int pi = 0; pi < parents; ++pi) {
+        const double* e = edges + (size_t)pi * dim * (bins + 1);
+        long double s = 0, s2 = 0, sk2 = 0, skf = 0;
+        long double* lh = hist ? (long double*)calloc((size_t)dim * bins, sizeof(long double)) : NULL;
+        double* lc = count ? count + (size_t)pi * dim * bins : NULL;
+        if (lc) memset(lc, 0, sizeof(double) * (size_t)dim * bins);
+        for (int64_t i = 0; i < n; ++i) {
+            uint32_t q[4];
+            const int64_t idx = first + i;
+            if (idx == 0 && first_q) {
+                for (int d = 0; d < dim; ++d) q[d] = first_q[d];
+            } else {
+                orc_sobol_u32(dim, direction, shift, idx, 1, q);
+            }
+            double x[4], jac;
+            int32_t b[4];
+            map_one(dim, bins, q, e, lo, width, x, &jac, b);
+            double k2, kf;
+            const double f = orc_nlo_integrand_log(parent_r[pi], x[0], x[1], x[2], x[3], sS, p,
+                                                   &k2, &kf);
+            const double w = f * jac;
+            s += w;
+            s2 += (long double)w * w;
+            sk2 += (long double)(k2 * jac);
+            skf += (long double)(kf * jac);
+            if (lh)
+                for (int d = 0; d < dim; ++d) lh[d * bins + b[d]] += fabs(w);
+            if (lc)
+                for (int d = 0; d < dim; ++d) lc[d * bins + b[d]] += 1.0;
+        }
+        sums[4 * pi + 0] = (double)s;
+        sums[4 * pi + 1] = (double)s2;
+        sums[4 * pi + 2] = (double)sk2;
+        sums[4 * pi + 3] = (double)skf;
+        if (lh) {
+            for (int k = 0; k < dim * bins; ++k) hist[(size_t)pi * dim * bins + k] = (double)lh[k];
+            free(lh);
+        }
+    }
+}

@@ -1,0 +1,126 @@
+/*
+ * FP64 CPU oracle for the NLO BK right-hand side and rapidity driver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
+ * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * may load it.  The product library (nlo_bk_optimized_amd/) never links,
+ * loads or calls it.
+ *
+ * Every function restates the reference algorithm from its source text and
+ * cites the reference file:line it follows (prefix S/ = torch_nlo_clean/src/
+ * nlo_torch/, K/ = S/custom_kernels/).  Arithmetic is IEEE double; sums over
+ * samples are accumulated in long double so that the oracle's own rounding
+ * is far below the 1e-10 parity tolerance.
+ */
+#ifndef NLO_ORACLE_H
+#define NLO_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Physics choices of BKConfig (S/bk/config.py:66-101) used on the hot path:
+ * Balitsky LO coupling, smallest-distance NLO coupling and resummation
+ * coupling, NLO_RESUM_DLOG_SLOG order, smooth coupling for NF <= 3. */
+typedef struct orc_bk_params {
+    double nc;          /* NC */
+    double nf;          /* NF */
+    double lambda_qcd;  /* LambdaQCD */
+    double c2;          /* C2 */
+    double ksub;        /* KSUB */
+    double minr;        /* MINR (double-log resummation switch r > 1.01 MINR) */
+    int symmetrize;     /* SYMMETRIZE_Z_Z2_INTEGRATION */
+    int geometry_cos;   /* 1: X^2 = r^2 + z^2 - 2 r z cos(theta) (torch code);
+                           0: X^2 = (r - z)^2 + 4 r z sin^2(theta / 2) */
+    int stable;         /* 0: K2/Kf exactly as S/bk_kernels/k2.py, kf.py;
+                           1: the same kernels rewritten without the 1/Delta
+                              cancellation (see orc_nlo_kernels_stable) */
+} orc_bk_params;
+
+/* Natural cubic spline of log(y) against log(r) (S/numerics/interpolation.py:95-121). */
+typedef struct orc_spline {
+    int n;              /* nodes */
+    double r_min;       /* r_grid[0]  (clamp/underflow limit) */
+    double r_max;       /* r_grid[-1] (clamp/overflow limit)  */
+    const double* x;    /* log r, n values */
+    const double* a;    /* n - 1 coefficients each */
+    const double* b;
+    const double* c;
+    const double* d;
+} orc_spline;
+
+double orc_alpha_s(double r, const orc_bk_params* p);
+double orc_mv(double r, double qs0sqr, double gamma, double ec, double lambda_qcd);
+void orc_r_grid(double minr, double maxr, int rpoints, double* out);
+
+void orc_natural_spline(int n, const double* x, const double* y,
+                        double* a, double* b, double* c, double* d);
+/* Evolution splines of N and S = 1 - N (S/bk/evolution.py:548-560).
+ * x_out: n log r; coef_n / coef_s: 4 * (n - 1) values laid out a|b|c|d. */
+void orc_evolution_splines(int n, const double* r, const double* N, int force_positive,
+                           double* x_out, double* coef_n, double* coef_s);
+double orc_spline_eval(const orc_spline* s, double log_value);
+double orc_loglog_eval(const orc_spline* s, double value, double underflow, double overflow);
+
+double orc_bessel_j1(double x);
+double orc_bessel_i1(double x);
+
+double orc_kernel_lo(double r, double z, double theta, const orc_bk_params* p);
+double orc_k1_integrand(double r, double z, double theta, const orc_spline* sN,
+                        const orc_bk_params* p);
+/* Tolerance scales: the same expressions with every additive piece in
+ * absolute value (only tests use them). */
+double orc_kernel_lo_magnitude(double r, double z, double theta, const orc_bk_params* p);
+double orc_k1_integrand_magnitude(double r, double z, double theta, const orc_spline* sN,
+                                  const orc_bk_params* p);
+double orc_nlo_integrand_magnitude(double r, double log_z, double log_z2, double theta_z,
+                                   double theta_z2, const orc_spline* sS,
+                                   const orc_bk_params* p);
+double orc_kernel_nlo(double r, double X, double Y, double X2, double Y2, double zmz2);
+double orc_kernel_nlo_fermion(double r, double X, double Y, double X2, double Y2,
+                              double zmz2, double nf, double nc);
+/* Returns the full integrand (with the exp(2 log z + 2 log z') Jacobian of
+ * S/bk/derivatives.py:335); k2_part / kf_part receive the two pieces. */
+double orc_nlo_integrand_log(double r, double log_z, double log_z2, double theta_z,
+                             double theta_z2, const orc_spline* sS, const orc_bk_params* p,
+                             double* k2_part, double* kf_part);
+
+/* K1 on the fixed production rule: (n-1) log-r panels x GL(nr) x GL(nt) in
+ * v with theta = pi v^2 (K/bk/k1/kernel.cu:2040-2092). */
+void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN,
+                  int nr, const double* gr_x, const double* gr_w,
+                  int nt, const double* gt_x, const double* gt_w,
+                  const orc_bk_params* p, double* out);
+
+/* Scrambled Sobol integer states (gray-code order, SobolEngine semantics). */
+void orc_sobol_u32(int dim, const uint32_t* direction /* dim x 30 */, const uint32_t* shift,
+                   int64_t first, int64_t n, uint32_t* out /* n x dim */);
+
+/* VEGAS map of integer Sobol states through an importance grid
+ * (S/dis/fixed_sobol.py:100-113, S/numerics/integration.py:392-396). */
+void orc_vegas_map(int dim, int bins, int64_t n, const uint32_t* q, const double* edges,
+                   const double* lo, const double* width, double* x, double* jac,
+                   int32_t* bin);
+
+/* Grid refinement (S/numerics/integration.py:804-866). eps: dtype epsilon. */
+void orc_vegas_adapt(int dim, int bins, const double* edges, const double* hist,
+                     const double* count, double eps, double* out);
+
+/* Per-parent K2 + Kf VEGAS sums over Sobol points [first, first + n):
+ * sums[4 p + k] = (sum w, sum w^2, sum w_K2, sum w_Kf).  hist / count
+ * (parents x 4 x bins, may be NULL) receive sum |w| and hits per bin. */
+void orc_bk_nlo_sums(int parents, const double* parent_r, const orc_spline* sS, int bins,
+                     const double* edges /* parents x 4 x (bins + 1) */,
+                     const double* lo, const double* width, const uint32_t* direction,
+                     const uint32_t* shift, const uint32_t* first_q, int64_t first, int64_t n,
+                     const orc_bk_params* p, double* sums, double* hist, double* count);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
