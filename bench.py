@@ -1,0 +1,236 @@
+"""Benchmark: full NLO BK evolution on one MI355X (the 256-node workload).
+
+Workload: 256 log-spaced r in [1e-6, 1e2] GeV^-1; MV initial condition
+(Qs0^2 = 0.2 GeV^2, gamma = 1, Lambda = 0.241 GeV, e_c = 1), Nf = 3, Balitsky
+LO / smallest-distance NLO coupling, double- and single-log resummation;
+K2 + Kf by VEGAS over 2^20 scrambled-Sobol points (seed 0) per parent per
+right-hand side through 50-bin grids adapted 5 times on the initial
+condition and then frozen; K1 on the 8 x 24 Gauss-Legendre rule per log-r
+panel; midpoint RK2 with dY = 0.1.
+
+A "step" is one RK2 rapidity step (two right-hand sides) over all parents,
+replayed from its hipGraph; the default 100 steps are the whole evolution
+Y: 0 -> 10.  ``value`` is integrand samples per second of the FP32 path
+(FP32 integrand, FP64 accumulation), counting only active parents
+(N <= 0.99999), with all inputs resident in HBM.  The same evolution in
+plain FP64 (the reference's K2/Kf formula) and in FP64 with the FP32 path's
+formula is run for the speedup and max |dN| fields.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+METRIC = "NLO-BK evolution wall time + integrand samples/s, FP32 vs FP64 at fixed error"
+
+# Algorithmic flops per K2 + Kf sample of the FP32 path (VEGAS map, the
+# cancellation-free integrand and the accumulation), counted on the source
+# with add/sub/mul/div/sqrt/exp/log/log1p/expm1/sin/cos = 1, fma = 2,
+# compares and selects = 0; itemised in DESIGN.md.
+FLOPS_PER_SAMPLE = 263
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
+FP64_PEAK_TFLOPS = 78.6
+
+
+def parse() -> argparse.Namespace:
+    p = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--samples-log2", type=int, default=20)
+    p.add_argument("--rpoints", type=int, default=255)
+    p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--kernel-launches", type=int, default=5)
+    return p.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only (got --gpus {args.gpus}): the BK right-hand side "
+            "shards over parents with one all-gather of N per RK stage, which this build "
+            "does not launch; run with --gpus 1",
+            file=sys.stderr,
+        )
+        return 2
+
+    import torch
+
+    from nlo_bk_optimized_amd import MV, BKConfig
+    from nlo_bk_optimized_amd import device as dev
+    from nlo_bk_optimized_amd.bk import evolution as evo
+
+    if not torch.cuda.is_available():
+        print("no HIP device available", file=sys.stderr)
+        return 1
+
+    n = 1 << args.samples_log2
+    base = dict(
+        RPOINTS=args.rpoints, MINR=1e-6, MAXR=1e2, MCINTPOINTS=n, MC_BINS=50,
+        VEGAS_ADAPT_ITERATIONS=5, DE_SOLVER_STEP=0.1, RK_METHOD="rk2",
+    )
+    ic = MV(qs0sqr=0.2, x0=0.01, ec=1.0, lambdaqcd=0.241, gamma=1.0)
+    device = torch.device("cuda")
+
+    def log(msg: str) -> None:
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    def evolve(precision: str, steps: int, warmup: int):
+        config = BKConfig(PRECISION=precision, **base)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        setup = evo.prepare(ic, config, device=device, seed=0)
+        torch.cuda.synchronize()
+        t_adapt = time.perf_counter() - t0
+        ev = dev.Evolver(setup.r, setup.edges, setup.table, config, n_samples=n, precision=precision,
+                         rk_method="rk2", step=config.DE_SOLVER_STEP)
+        ev.state.copy_(setup.N0)
+        for _ in range(warmup):
+            ev.step()
+        ev.state.copy_(setup.N0)
+        torch.cuda.synchronize()
+        count0 = int(ev.active_evaluations.item())
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ev.step()
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        evaluations = int(ev.active_evaluations.item()) - count0
+        flag = int(ev.flag.item())
+        N = ev.state.clone()
+        ev.close()
+        return dict(setup=setup, seconds=elapsed, adapt_seconds=t_adapt, evaluations=evaluations,
+                    N=N, nonfinite=flag, config=config)
+
+    log(f"fp32 evolution: {args.steps} steps, 2^{args.samples_log2} samples/parent/RHS")
+    fp32 = evolve("fp32", args.steps, args.warmup)
+    samples = fp32["evaluations"] * n
+    value = samples / fp32["seconds"]
+    log(f"fp32: {fp32['seconds']:.3f} s, {value:.4e} samples/s, adaptation {fp32['adapt_seconds']:.3f} s")
+
+    # Dominant kernel: the K2 + Kf integrand launch, timed with HIP events on
+    # the stream it is launched on (torch's current stream).
+    setup = fp32["setup"]
+    config = fp32["config"]
+    t64, t32 = dev.spline_build(setup.r, setup.N0)
+    active0 = int(((torch.arange(setup.r.numel(), device=device) >= 1) & (setup.N0 <= 0.99999)).sum())
+    for _ in range(2):
+        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32")
+    stream = torch.cuda.current_stream()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(args.kernel_launches):
+        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32")
+    stop.record(stream)
+    stop.synchronize()
+    launch_ms = start.elapsed_time(stop) / args.kernel_launches
+    achieved = FLOPS_PER_SAMPLE * active0 * n / (launch_ms * 1e-3) / 1e12
+    log(f"nlo fp32 launch: {launch_ms:.3f} ms for {active0} parents x {n} samples -> {achieved:.2f} TFLOP/s")
+
+    extra: dict = {}
+    if not args.no_fp64:
+        log("fp64 (reference K2/Kf formula) evolution")
+        fp64 = evolve("fp64", args.steps, 0)
+        log(f"fp64: {fp64['seconds']:.3f} s")
+        log("fp64 (FP32 path's formula) evolution")
+        fp64s = evolve("fp64_stable", args.steps, 0)
+        log(f"fp64_stable: {fp64s['seconds']:.3f} s")
+        extra = {
+            "fp64_samples_per_s": fp64["evaluations"] * n / fp64["seconds"],
+            "fp64_evolution_s": fp64["seconds"],
+            "speedup_fp32_vs_fp64": fp64["seconds"] / fp32["seconds"],
+            "max_abs_dN_fp32_vs_fp64_same_formula": float((fp32["N"] - fp64s["N"]).abs().max()),
+            "max_abs_dN_fp32_vs_fp64_reference_formula": float((fp32["N"] - fp64["N"]).abs().max()),
+            "max_abs_dN_fp64_formulas": float((fp64s["N"] - fp64["N"]).abs().max()),
+        }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # The FP64 oracle (C restatement of the reference CPU algorithm) on a
+        # bounded sample of the same right-hand side: 64 parents x 2^17 Sobol
+        # points of K2 + Kf (reference formula) plus their K1, at the initial
+        # condition with the adapted grids, on every host core OpenMP uses.
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        import numpy as np
+
+        from oracle import oracle as O
+
+        r = setup.r.cpu().numpy()
+        N0 = setup.N0.cpu().numpy()
+        edges = setup.edges.cpu().numpy()
+        sN, sS = O.evolution_splines(r, N0)
+        lo, width = O.nlo_bounds(r)
+        active = np.nonzero((np.arange(r.size) >= 1) & (N0 <= 0.99999))[0]
+        pick = active[np.linspace(0, active.size - 1, min(64, active.size)).astype(int)]
+        m = 1 << 17
+        table = O.sobol_table(4, 0)
+        t0 = time.perf_counter()
+        O.bk_nlo_sums(r[pick], sS, edges[pick], lo, width, table, 0, m, O.params(minr=1e-6))
+        O.k1_fixed(r[pick], sN, p=O.params(minr=1e-6))
+        cpu_s = time.perf_counter() - t0
+        cpu = {
+            "value": pick.size * m / cpu_s,
+            "unit": "samples/s",
+            "cores": O.num_threads(),
+            "kind": "port",
+            "sample": f"{pick.size} parents x 2^17 Sobol points of K2+Kf (FP64 reference formula) "
+                      f"plus their K1 (8x24 GL per panel) at Y=0, {cpu_s:.2f} s wall",
+        }
+        log(f"cpu baseline: {cpu['value']:.4e} samples/s on {cpu['cores']} threads")
+
+    ms_per_step = fp32["seconds"] / args.steps * 1e3
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"NLO BK evolution: {args.rpoints + 1} r in [1e-6,1e2], MV Qs0^2=0.2 gamma=1, "
+                        f"2^{args.samples_log2} Sobol points/parent/RHS, VEGAS 50 bins adapted x5 then "
+                        f"frozen, RK2 dY=0.1, {args.steps} steps",
+            "r_points": args.rpoints + 1,
+            "samples_per_parent": n,
+            "rk": "midpoint RK2",
+            "dY": 0.1,
+            "accumulate": "f64",
+        },
+        "evolution_s": fp32["seconds"],
+        "adaptation_s": fp32["adapt_seconds"],
+        "integrand_samples": samples,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "nlo_kernel<FP32> + per-parent reduce",
+            "launch_ms": launch_ms,
+            "flops_per_sample": FLOPS_PER_SAMPLE,
+        },
+        "cpu_baseline": cpu,
+        "nonfinite": fp32["nonfinite"],
+    }
+    result.update(extra)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

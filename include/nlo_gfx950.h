@@ -212,6 +212,10 @@ double* nlo_bk_evolver_state(nlo_bk_evolver* ev);
 /* Device int32 flag, non-zero once any step produced a non-finite N
  * (S/bk/evolution.py:152-153 raises FloatingPointError). */
 int32_t* nlo_bk_evolver_nonfinite_flag(nlo_bk_evolver* ev);
+/* Device uint64: number of (parent, right-hand side) evaluations with an
+ * active parent since creation; times n_samples it is the exact count of
+ * K2 + Kf integrand samples evaluated. */
+uint64_t* nlo_bk_evolver_active_evaluations(nlo_bk_evolver* ev);
 /* Last step's dN/dy stages: d_k1 (R) and d_k2kf (R) of the final stage. */
 const double* nlo_bk_evolver_k1(nlo_bk_evolver* ev);
 const double* nlo_bk_evolver_k2kf(nlo_bk_evolver* ev);

@@ -119,6 +119,7 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_bk_evolver_state": (_vp, [_vp]),
     "nlo_bk_evolver_nonfinite_flag": (_vp, [_vp]),
     "nlo_bk_evolver_k1": (_vp, [_vp]),
+    "nlo_bk_evolver_active_evaluations": (_vp, [_vp]),
     "nlo_bk_evolver_k2kf": (_vp, [_vp]),
     "nlo_bk_evolver_step": (ctypes.c_int, [_vp, _vp]),
     "nlo_bk_evolver_destroy": (ctypes.c_int, [_vp]),

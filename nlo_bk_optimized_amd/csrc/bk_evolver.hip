@@ -22,6 +22,7 @@ struct nlo_bk_evolver {
     void* nlo_ws;
     size_t nlo_ws_bytes;
     int32_t* flag;
+    unsigned long long* active;  // active-parent evaluations since creation
     hipStream_t capture;
     hipGraph_t graph;
     hipGraphExec_t exec;
@@ -31,7 +32,7 @@ namespace nlo {
 namespace {
 
 struct EvolverLayout {
-    size_t N, Nst, K, k1, k2kf, sums, t64, t32, flag, nlo_ws, nlo_ws_bytes, total;
+    size_t N, Nst, K, k1, k2kf, sums, t64, t32, flag, active, nlo_ws, nlo_ws_bytes, total;
 };
 
 EvolverLayout evolver_layout(const nlo_bk_evolver_desc& d) {
@@ -52,6 +53,7 @@ EvolverLayout evolver_layout(const nlo_bk_evolver_desc& d) {
     L.t64 = take((size_t)NLO_SPLINE_WORDS(d.R) * sizeof(double));
     L.t32 = take((size_t)NLO_SPLINE_WORDS(d.R) * sizeof(float));
     L.flag = take(sizeof(int32_t));
+    L.active = take(sizeof(unsigned long long));
     L.nlo_ws_bytes = d.nlo ? nlo_plan(d.R, d.bins, d.n_samples, false).workspace : 0;
     L.nlo_ws = take(L.nlo_ws_bytes > 0 ? L.nlo_ws_bytes : 1);
     L.total = o;
@@ -124,10 +126,11 @@ int enqueue_step(nlo_bk_evolver* ev, hipStream_t s) {
                             d.n_samples, d.precision, ev->c, ev->sums, nullptr, nullptr, ev->nlo_ws,
                             ev->nlo_ws_bytes, s);
             if (rc != NLO_OK) return rc;
-            rc = launch_rhs_finalize(R, in, ev->k1, ev->sums, d.n_samples, ev->K[st], ev->k2kf, s);
+            rc = launch_rhs_finalize(R, in, ev->k1, ev->sums, d.n_samples, ev->K[st], ev->k2kf,
+                                     ev->active, s);
         } else {
             // LO-only: K2 + Kf sums stay zero
-            rc = launch_rhs_finalize(R, in, ev->k1, ev->sums, 1, ev->K[st], ev->k2kf, s);
+            rc = launch_rhs_finalize(R, in, ev->k1, ev->sums, 1, ev->K[st], ev->k2kf, ev->active, s);
         }
         if (rc != NLO_OK) return rc;
     }
@@ -198,6 +201,7 @@ int nlo_bk_evolver_create(const nlo_bk_evolver_desc* desc, const double* d_r, co
     ev->t64 = reinterpret_cast<double*>(w + L.t64);
     ev->t32 = reinterpret_cast<float*>(w + L.t32);
     ev->flag = reinterpret_cast<int32_t*>(w + L.flag);
+    ev->active = reinterpret_cast<unsigned long long*>(w + L.active);
     ev->nlo_ws = w + L.nlo_ws;
     ev->nlo_ws_bytes = L.nlo_ws_bytes;
 
@@ -206,6 +210,7 @@ int nlo_bk_evolver_create(const nlo_bk_evolver_desc* desc, const double* d_r, co
     hipLaunchKernelGGL(nlo::zero_f64_kernel, dim3((4 * d.R + 255) / 256), dim3(256), 0, stream,
                        ev->sums, 4 * d.R);
     rc = nlo::launch_fill_i32(ev->flag, 0, 1, stream);
+    if (rc == NLO_OK) rc = nlo::launch_fill_u64(ev->active, 0ull, 1, stream);
     if (rc != NLO_OK) {
         delete ev;
         return rc;
@@ -253,6 +258,10 @@ double* nlo_bk_evolver_state(nlo_bk_evolver* ev) { return ev ? ev->N : nullptr; 
 int32_t* nlo_bk_evolver_nonfinite_flag(nlo_bk_evolver* ev) { return ev ? ev->flag : nullptr; }
 
 const double* nlo_bk_evolver_k1(nlo_bk_evolver* ev) { return ev ? ev->k1 : nullptr; }
+
+uint64_t* nlo_bk_evolver_active_evaluations(nlo_bk_evolver* ev) {
+    return ev ? reinterpret_cast<uint64_t*>(ev->active) : nullptr;
+}
 
 const double* nlo_bk_evolver_k2kf(nlo_bk_evolver* ev) { return ev ? ev->k2kf : nullptr; }
 

@@ -35,12 +35,13 @@ int launch_nlo(int R, const double* r, const double* N, const double* t64, const
                int precision, const BkConsts& c, double* sums, double* hist, double* count, void* ws,
                size_t ws_bytes, hipStream_t stream);
 int launch_rhs_finalize(int R, const double* N, const double* k1, const double* sums, int64_t n,
-                        double* dndy, double* k2kf, hipStream_t stream);
+                        double* dndy, double* k2kf, unsigned long long* counter, hipStream_t stream);
 int launch_rk_stage(int R, const double* N, const double* K0, const double* K1, const double* K2,
                     double a0, double a1, double a2, double h, double* out, hipStream_t stream);
 int launch_rk_final(int R, double* N, const double* K0, const double* K1, const double* K2,
                     const double* K3, double b0, double b1, double b2, double b3, double h,
                     int force_positive, int32_t* flag, hipStream_t stream);
 int launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t stream);
+int launch_fill_u64(unsigned long long* p, unsigned long long v, int n, hipStream_t stream);
 
 }  // namespace nlo

@@ -473,16 +473,29 @@ __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bin
 }
 
 // S/bk/evolution.py:340-351 plus the active set of :501.
+// counter (optional) accumulates the number of active parents: integer
+// adds, so the total is exact whatever the order.
 __global__ void rhs_finalize_kernel(int R, const double* __restrict__ N, const double* __restrict__ k1,
                                     const double* __restrict__ sums, double inv_n,
-                                    double* __restrict__ dndy, double* __restrict__ k2kf) {
+                                    double* __restrict__ dndy, double* __restrict__ k2kf,
+                                    unsigned long long* __restrict__ counter) {
+    __shared__ unsigned int block_active;
+    if (threadIdx.x == 0) block_active = 0u;
+    __syncthreads();
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= R) return;
-    const bool active = p >= 1 && N[p] <= 0.99999;
-    const double nlo = active ? sums[(size_t)p * 4] * inv_n : 0.0;
-    double v = active ? k1[p] + nlo : 0.0;
-    dndy[p] = isfinite(v) ? v : 0.0;
-    if (k2kf) k2kf[p] = nlo;
+    bool active = false;
+    if (p < R) {
+        active = p >= 1 && N[p] <= 0.99999;
+        const double nlo = active ? sums[(size_t)p * 4] * inv_n : 0.0;
+        double v = active ? k1[p] + nlo : 0.0;
+        dndy[p] = isfinite(v) ? v : 0.0;
+        if (k2kf) k2kf[p] = nlo;
+    }
+    if (counter) {
+        if (active) atomicAdd(&block_active, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0 && block_active) atomicAdd(counter, (unsigned long long)block_active);
+    }
 }
 
 // out = N + h (a0 K0 + a1 K1 + a2 K2)
@@ -516,6 +529,11 @@ __global__ void rk_final_kernel(int R, double* __restrict__ N, const double* __r
 }
 
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u64_kernel(unsigned long long* p, unsigned long long v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -657,9 +675,9 @@ int launch_nlo(int R, const double* r, const double* N, const double* t64, const
 }
 
 int launch_rhs_finalize(int R, const double* N, const double* k1, const double* sums, int64_t n,
-                        double* dndy, double* k2kf, hipStream_t stream) {
+                        double* dndy, double* k2kf, unsigned long long* counter, hipStream_t stream) {
     hipLaunchKernelGGL(rhs_finalize_kernel, dim3((R + 255) / 256), dim3(256), 0, stream, R, N, k1,
-                       sums, 1.0 / (double)n, dndy, k2kf);
+                       sums, 1.0 / (double)n, dndy, k2kf, counter);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }
@@ -683,6 +701,12 @@ int launch_rk_final(int R, double* N, const double* K0, const double* K1, const 
 
 int launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t stream) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_fill_u64(unsigned long long* p, unsigned long long v, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_u64_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }
@@ -755,7 +779,7 @@ int nlo_bk_rhs_finalize(int32_t R, const double* d_N, const double* d_k1, const 
                         int64_t n, double* d_dndy, double* d_k2kf, hipStream_t stream) {
     NLO_REQUIRE(R >= 1 && n >= 1, "nlo_bk_rhs_finalize: R=%d n=%lld invalid", R, (long long)n);
     NLO_REQUIRE(d_N && d_k1 && d_sums && d_dndy, "nlo_bk_rhs_finalize: null pointer");
-    const int rc = nlo::launch_rhs_finalize(R, d_N, d_k1, d_sums, n, d_dndy, d_k2kf, stream);
+    const int rc = nlo::launch_rhs_finalize(R, d_N, d_k1, d_sums, n, d_dndy, d_k2kf, nullptr, stream);
     if (rc == NLO_OK) nlo::clear_error();
     return rc;
 }

@@ -6,6 +6,10 @@
 
 #include "common.hpp"
 
+// The VEGAS map and refinement reproduce the reference's FP64 operation
+// sequence bit for bit, so no multiply-add in this file may be fused.
+#pragma clang fp contract(off)
+
 namespace nlo {
 
 namespace {
@@ -54,8 +58,9 @@ __global__ void __launch_bounds__(256) sobol_draw_kernel(nlo_sobol_table t, int6
 
 // ------------------------------------------------------------------ VEGAS
 
-// map_importance_grid (S/dis/fixed_sobol.py:100-113) in IEEE order, without
-// contraction, so bins and coordinates equal the FP64 reference bit for bit.
+// map_importance_grid (S/dis/fixed_sobol.py:100-113) in IEEE order (no
+// contraction in this file), so bins and coordinates equal the FP64
+// reference bit for bit.
 __global__ void __launch_bounds__(256) vegas_map_kernel(int dim, int bins, int64_t n,
                                                          const double* __restrict__ u,
                                                          const double* __restrict__ edges,
@@ -68,19 +73,19 @@ __global__ void __launch_bounds__(256) vegas_map_kernel(int dim, int bins, int64
          i += (int64_t)gridDim.x * blockDim.x) {
         double j = 1.0, vol = 1.0;
         for (int d = 0; d < dim; ++d) {
-            const double scaled = __dmul_rn(u[i * dim + d], (double)bins);
+            const double scaled = u[i * dim + d] * (double)bins;
             int b = (int)floor(scaled);
             b = b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
-            const double frac = __dsub_rn(scaled, (double)b);
+            const double frac = scaled - (double)b;
             const double left = edges[d * (bins + 1) + b];
             const double right = edges[d * (bins + 1) + b + 1];
-            const double w = __dsub_rn(right, left);
-            x[i * dim + d] = __dadd_rn(lo[d], __dmul_rn(width[d], __dadd_rn(left, __dmul_rn(w, frac))));
-            j = __dmul_rn(j, __dmul_rn((double)bins, w));
-            vol = __dmul_rn(vol, width[d]);
+            const double w = right - left;
+            x[i * dim + d] = lo[d] + width[d] * (left + w * frac);
+            j = j * ((double)bins * w);
+            vol = vol * width[d];
             if (bin) bin[i * dim + d] = (uint8_t)b;
         }
-        jac[i] = __dmul_rn(vol, j);
+        jac[i] = vol * j;
     }
 }
 
@@ -146,23 +151,21 @@ __global__ void vegas_hist_reduce_kernel(int values, int blocks, const double* _
 // Smoothed importance of bin i (S/numerics/integration.py:828-833): 3-point
 // mean inside, 2-point mean at the ends, plus eps.
 __device__ __forceinline__ double vegas_importance(const double* h, const double* c, int i) {
-    return __ddiv_rn(h[i], fmax(c[i], 1.0));
+    return h[i] / fmax(c[i], 1.0);
 }
 
 __device__ __forceinline__ double vegas_smoothed(const double* h, const double* c, int bins, int i,
                                                  double eps) {
     double v;
     if (i == 0) {
-        v = __ddiv_rn(__dadd_rn(vegas_importance(h, c, 0), vegas_importance(h, c, 1)), 2.0);
+        v = (vegas_importance(h, c, 0) + vegas_importance(h, c, 1)) / 2.0;
     } else if (i == bins - 1) {
-        v = __ddiv_rn(
-            __dadd_rn(vegas_importance(h, c, bins - 2), vegas_importance(h, c, bins - 1)), 2.0);
+        v = (vegas_importance(h, c, bins - 2) + vegas_importance(h, c, bins - 1)) / 2.0;
     } else {
-        v = __ddiv_rn(__dadd_rn(__dadd_rn(vegas_importance(h, c, i - 1), vegas_importance(h, c, i)),
-                                vegas_importance(h, c, i + 1)),
-                      3.0);
+        v = (vegas_importance(h, c, i - 1) + vegas_importance(h, c, i) + vegas_importance(h, c, i + 1)) /
+            3.0;
     }
-    return __dadd_rn(v, eps);
+    return v + eps;
 }
 
 // _adapt_vegas_edges (S/numerics/integration.py:835-866) for one (row, dim)
@@ -180,8 +183,8 @@ __global__ void vegas_adapt_kernel(int rows, int dim, int bins, const double* __
     const double* cc = count + (size_t)k * bins;
     double* o = out + (size_t)k * (bins + 1);
     double total = 0.0;
-    for (int i = 0; i < bins; ++i) total = __dadd_rn(total, vegas_smoothed(hh, cc, bins, i, eps));
-    const double step = __ddiv_rn(total, (double)bins);
+    for (int i = 0; i < bins; ++i) total = total + vegas_smoothed(hh, cc, bins, i, eps);
+    const double step = total / (double)bins;
     // j: largest index with cumulative[j] <= target; cum_j = cumulative[j];
     // sm_j = smoothed[j] (cumulative[j + 1] = cum_j + sm_j).
     int j = 0;
@@ -190,9 +193,9 @@ __global__ void vegas_adapt_kernel(int rows, int dim, int bins, const double* __
     double cum_prev = 0.0, sm_prev = sm_j;  // values at j - 1 (valid when j == bins)
     for (int t = 0; t <= bins; ++t) {
         const double target =
-            t < (bins + 1) / 2 ? __dmul_rn((double)t, step) : fma(-step, (double)(bins - t), total);
+            t < (bins + 1) / 2 ? (double)t * step : fma(-step, (double)(bins - t), total);
         while (j < bins) {
-            const double cum_next = __dadd_rn(cum_j, sm_j);
+            const double cum_next = cum_j + sm_j;
             if (!(target >= cum_next)) break;
             cum_prev = cum_j;
             sm_prev = sm_j;
@@ -203,8 +206,8 @@ __global__ void vegas_adapt_kernel(int rows, int dim, int bins, const double* __
         const int b = j < bins ? j : bins - 1;
         const double cb = j < bins ? cum_j : cum_prev;
         const double sb = j < bins ? sm_j : sm_prev;
-        const double frac = __ddiv_rn(__dsub_rn(target, cb), sb);
-        o[t] = __dadd_rn(e[b], __dmul_rn(frac, __dsub_rn(e[b + 1], e[b])));
+        const double frac = (target - cb) / sb;
+        o[t] = e[b] + frac * (e[b + 1] - e[b]);
     }
     o[0] = 0.0;
     o[bins] = 1.0;

@@ -299,6 +299,7 @@ class Evolver:
         self.k1 = view(lib().nlo_bk_evolver_k1(handle), R, torch.float64)
         self.k2kf = view(lib().nlo_bk_evolver_k2kf(handle), R, torch.float64)
         self.flag = view(lib().nlo_bk_evolver_nonfinite_flag(handle), 1, torch.int32)
+        self.active_evaluations = view(lib().nlo_bk_evolver_active_evaluations(handle), 1, torch.int64)
 
     def step(self) -> None:
         check(lib().nlo_bk_evolver_step(self.handle, stream_handle(self.r.device)), "nlo_bk_evolver_step")

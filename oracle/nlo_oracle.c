@@ -217,8 +217,8 @@ static double kernel_lo_impl(double r, double z, double theta, const orc_bk_para
     const double result = dlog * slog * lo - subtract + k1fin;
     if (magnitude) {
         const double lo_abs = p->nc / (2.0 * M_PI * M_PI) * a_r *
-                              (r * r / (sX * sX * sY * sY) + fabs(a_Y / a_X - 1.0) / (sY * sY) +
-                               fabs(a_X / a_Y - 1.0) / (sX * sX));
+                              (r * r / (sX * sX * sY * sY) + (a_Y / a_X + 1.0) / (sY * sY) +
+                               (a_X / a_Y + 1.0) / (sX * sX));
         *magnitude = (invalid || !resum_valid)
                          ? 0.0
                          : fabs(dlog * slog) * lo_abs + fabs(subtract) + fabs(k1fin) +
@@ -297,6 +297,30 @@ void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN, int
                     const double wt = gt_w[j] * M_PI * v;
                     total += (long double)(orc_k1_integrand(r, z, theta, sN, p) * (2.0 * z * z) *
                                            wr * wt);
+                }
+            }
+        }
+        out[pi] = (double)total;
+    }
+}
+
+/* Same rule over |integrand| magnitudes: the rounding scale of K1. */
+void orc_k1_fixed_magnitude(int parents, const double* parent_r, const orc_spline* sN, int nr,
+                            const double* gr_x, const double* gr_w, int nt, const double* gt_x,
+                            const double* gt_w, const orc_bk_params* p, double* out) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int pi = 0; pi < parents; ++pi) {
+        const double r = parent_r[pi];
+        long double total = 0.0L;
+        for (int panel = 0; panel < sN->n - 1; ++panel) {
+            const double lower = sN->x[panel], upper = sN->x[panel + 1];
+            const double center = (lower + upper) / 2.0, half = (upper - lower) / 2.0;
+            for (int i = 0; i < nr; ++i) {
+                const double z = exp(center + half * gr_x[i]);
+                for (int j = 0; j < nt; ++j) {
+                    const double v = (gt_x[j] + 1.0) / 2.0;
+                    total += (long double)(orc_k1_integrand_magnitude(r, z, M_PI * v * v, sN, p) *
+                                           (2.0 * z * z) * (half * gr_w[i]) * (gt_w[j] * M_PI * v));
                 }
             }
         }
@@ -463,30 +487,12 @@ static double nlo_impl(double r, double log_z, double log_z2, double theta_z,
     for (int i = 0; i < 5; ++i) mn = cand[i] < mn ? cand[i] : mn;
     const double as = orc_alpha_s(mn, p);
 
-    const double J = exp(2.0 * log_z + 2.0 * log_z2);
-    double k, ks, kf_st = 0.0, kfs_st = 0.0, cut_limit = 1e10;
-    if (p->stable) {
-        double kk[4];
-        orc_nlo_kernels_stable(r, z, theta_z, z2, theta_z2, p->nf, p->nc, kk);
-        k = kk[0];
-        ks = kk[1];
-        kf_st = kk[2];
-        kfs_st = kk[3];
-        cut_limit = 1e10 * J; /* |K2| > 1e10 <=> |K2 J| > 1e10 J */
-    } else {
-        k = orc_kernel_nlo(r, X, Y, X2, Y2, zz);
-        ks = orc_kernel_nlo(r, X2, Y2, X, Y, zz);
-    }
+    double k = orc_kernel_nlo(r, X, Y, X2, Y2, zz);
+    double ks = orc_kernel_nlo(r, X2, Y2, X, Y, zz);
     double dip = -(S_X * S_zz * S_Y2 - S_X * S_Y);
     double dips = -(S_X2 * S_zz * S_Y - S_X2 * S_Y2);
-    if (p->stable) {
-        /* dipole combinations in N = 1 - S form (no 1 - S cancellation) */
-        const double N_Y = 1.0 - S_Y, N_Y2 = 1.0 - S_Y2, N_zz = 1.0 - S_zz;
-        dip = S_X * (N_zz + N_Y2 - N_Y - N_zz * N_Y2);
-        dips = S_X2 * (N_zz + N_Y - N_Y2 - N_zz * N_Y);
-    }
-    if (fabs(k) > cut_limit && fabs(dip) < 1e-10) { k = 0.0; dip = 0.0; }
-    if (fabs(ks) > cut_limit && fabs(dips) < 1e-10) { ks = 0.0; dips = 0.0; }
+    if (fabs(k) > 1e10 && fabs(dip) < 1e-10) { k = 0.0; dip = 0.0; }
+    if (fabs(ks) > 1e10 && fabs(dips) < 1e-10) { ks = 0.0; dips = 0.0; }
     double k2r = p->symmetrize ? (k * dip + ks * dips) / 2.0 : k * dip;
     double kfr = 0.0;
     double mag = (kernel_nlo_scale(r, X, Y, X2, Y2, zz, 0) *
@@ -494,8 +500,8 @@ static double nlo_impl(double r, double log_z, double log_z2, double theta_z,
                   kernel_nlo_scale(r, X2, Y2, X, Y, zz, 0) *
                       (fabs(S_X2 * S_zz * S_Y) + fabs(S_X2 * S_Y2)));
     if (p->nf > 0.0) {
-        const double kf = p->stable ? kf_st : orc_kernel_nlo_fermion(r, X, Y, X2, Y2, zz, p->nf, p->nc);
-        const double kfs = p->stable ? kfs_st : orc_kernel_nlo_fermion(r, X2, Y2, X, Y, zz, p->nf, p->nc);
+        const double kf = orc_kernel_nlo_fermion(r, X, Y, X2, Y2, zz, p->nf, p->nc);
+        const double kfs = orc_kernel_nlo_fermion(r, X2, Y2, X, Y, zz, p->nf, p->nc);
         const double df = S_Y * (S_X2 - S_X);
         const double dfs = S_Y2 * (S_X - S_X2);
         kfr = -(p->symmetrize ? (kf * df + kfs * dfs) / 2.0 : kf * df);
@@ -503,7 +509,7 @@ static double nlo_impl(double r, double log_z, double log_z2, double theta_z,
                (kernel_nlo_scale(r, X, Y, X2, Y2, zz, 1) * fabs(S_Y) * (fabs(S_X2) + fabs(S_X)) +
                 kernel_nlo_scale(r, X2, Y2, X, Y, zz, 1) * fabs(S_Y2) * (fabs(S_X) + fabs(S_X2)));
     }
-    const double scale = sqr(as * p->nc) / (8.0 * pow(M_PI, 4.0)) * (p->stable ? 1.0 : J);
+    const double scale = sqr(as * p->nc) / (8.0 * pow(M_PI, 4.0)) * exp(2.0 * log_z + 2.0 * log_z2);
     k2r *= scale;
     kfr *= scale;
     if (!isfinite(k2r)) k2r = 0.0;
@@ -528,6 +534,78 @@ double orc_nlo_integrand_magnitude(double r, double log_z, double log_z2, double
     double m = 0.0;
     (void)nlo_impl(r, log_z, log_z2, theta_z, theta_z2, sS, p, NULL, NULL, &m);
     return m;
+}
+
+/* N from the log-log N spline at ln d: clamp and range tests in log space
+ * (below the grid 0, above it 1; S/bk/derivatives.py:761-772). */
+static double n_from_log(const orc_spline* sN, double log_d) {
+    const double lo = sN->x[0], hi = sN->x[sN->n - 1];
+    double lv = log_d < lo ? lo : log_d;
+    lv = lv > hi ? hi : lv;
+    double out = exp(orc_spline_eval(sN, lv));
+    if (!isfinite(out)) out = 0.0;
+    if (log_d < lo) out = 0.0;
+    if (log_d > hi) out = 1.0;
+    return out;
+}
+
+/* The K2 + Kf integrand of S/bk/derivatives.py:521-612 evaluated without
+ * catastrophic cancellation: distances and their logs from the sin^2 forms,
+ * kernels times J = z^2 z'^2 from orc_nlo_kernels_stable, N from the log-log
+ * N spline and the dipole combinations in N = 1 - S form
+ *   dip  = S_X (N_Z + N_Y' - N_Y - N_Z N_Y'),  dip' = S_X' (N_Z + N_Y - N_Y' - N_Z N_Y),
+ *   dipf = S_Y (N_X - N_X'),                   dipf' = S_Y' (N_X' - N_X),
+ * the coupling at the smallest distance from ln d.  Same result as the
+ * reference formula in exact arithmetic, up to the N- vs S-spline choice. */
+double orc_nlo_integrand_stable(double r, double log_z, double log_z2, double theta_z,
+                                double theta_z2, const orc_spline* sN, const orc_bk_params* p,
+                                double* k2_part, double* kf_part) {
+    if (k2_part) *k2_part = 0.0;
+    if (kf_part) *kf_part = 0.0;
+    const double z = exp(log_z), z2 = exp(log_z2);
+    const double s1 = sin(0.5 * theta_z), s2 = sin(0.5 * theta_z2);
+    const double s3 = sin(0.5 * (theta_z - theta_z2));
+    const double xsq = (r - z) * (r - z) + 4.0 * r * z * s1 * s1;
+    const double x2sq = (r - z2) * (r - z2) + 4.0 * r * z2 * s2 * s2;
+    const double zsq = (z - z2) * (z - z2) + 4.0 * z * z2 * s3 * s3;
+    if (!(xsq >= 1e-40) || !(x2sq >= 1e-40) || !(zsq >= 1e-40)) return 0.0;
+    const double lxy = orc_log_x_over_y(r, z, theta_z, xsq);
+    const double lx2y2 = orc_log_x_over_y(r, z2, theta_z2, x2sq);
+    const double log_x = log_z + lxy, log_x2 = log_z2 + lx2y2, log_zz = 0.5 * log(zsq);
+    const double n_x = n_from_log(sN, log_x), n_y = n_from_log(sN, log_z);
+    const double n_x2 = n_from_log(sN, log_x2), n_y2 = n_from_log(sN, log_z2);
+    const double n_z = n_from_log(sN, log_zz);
+    double dip = (1.0 - n_x) * (n_z + n_y2 - n_y - n_z * n_y2);
+    double dips = (1.0 - n_x2) * (n_z + n_y - n_y2 - n_z * n_y);
+    double kk[4];
+    orc_nlo_kernels_stable(r, z, theta_z, z2, theta_z2, p->nf, p->nc, kk);
+    double k = kk[0], ks = kk[1];
+    const double cut = 1e10 * exp(2.0 * (log_z + log_z2)); /* |K2| > 1e10 */
+    if (fabs(k) > cut && fabs(dip) < 1e-10) { k = 0.0; dip = 0.0; }
+    if (fabs(ks) > cut && fabs(dips) < 1e-10) { ks = 0.0; dips = 0.0; }
+    double k2r = p->symmetrize ? (k * dip + ks * dips) / 2.0 : k * dip;
+    double kfr = 0.0;
+    if (p->nf > 0.0) {
+        const double df = (1.0 - n_y) * (n_x - n_x2);
+        const double dfs = (1.0 - n_y2) * (n_x2 - n_x);
+        kfr = -(p->symmetrize ? (kk[2] * df + kk[3] * dfs) / 2.0 : kk[2] * df);
+    }
+    double lmin = log(r);
+    const double cand[5] = {log_x, log_z, log_x2, log_z2, log_zz};
+    for (int i = 0; i < 5; ++i) lmin = cand[i] < lmin ? cand[i] : lmin;
+    const double b0 = (11.0 * p->nc - 2.0 * p->nf) / (12.0 * M_PI);
+    const double left = 2.0 / 0.2 * log(2.5);
+    const double right = 5.0 * (log(4.0 * p->c2 / (p->lambda_qcd * p->lambda_qcd)) - 2.0 * lmin);
+    const double hi = left > right ? left : right;
+    const double as = 1.0 / (b0 * (0.2 * (hi + log1p(exp(-fabs(left - right))))));
+    const double scale = sqr(as * p->nc) / (8.0 * pow(M_PI, 4.0));
+    k2r *= scale;
+    kfr *= scale;
+    if (!isfinite(k2r)) k2r = 0.0;
+    if (!isfinite(kfr)) kfr = 0.0;
+    if (k2_part) *k2_part = k2r;
+    if (kf_part) *kf_part = kfr;
+    return k2r + kfr;
 }
 
 /* ------------------------------------------------------------------- Sobol */
@@ -631,12 +709,12 @@ void orc_bk_nlo_sums(int parents, const double* parent_r, const orc_spline* sS, 
                      const double* edges, const double* lo, const double* width,
                      const uint32_t* direction, const uint32_t* shift, const uint32_t* first_q,
                      int64_t first, int64_t n, const orc_bk_params* p, double* sums, double* hist,
-                     double* count) {
+                     double* count, double* magnitude) {
     const int dim = 4;
 #pragma omp parallel for schedule(dynamic, 1)
     for (int pi = 0; pi < parents; ++pi) {
         const double* e = edges + (size_t)pi * dim * (bins + 1);
-        long double s = 0, s2 = 0, sk2 = 0, skf = 0;
+        long double s = 0, s2 = 0, sk2 = 0, skf = 0, sm = 0;
         long double* lh = hist ? (long double*)calloc((size_t)dim * bins, sizeof(long double)) : NULL;
         double* lc = count ? count + (size_t)pi * dim * bins : NULL;
         if (lc) memset(lc, 0, sizeof(double) * (size_t)dim * bins);
@@ -652,9 +730,15 @@ void orc_bk_nlo_sums(int parents, const double* parent_r, const orc_spline* sS, 
             int32_t b[4];
             map_one(dim, bins, q, e, lo, width, x, &jac, b);
             double k2, kf;
-            const double f = orc_nlo_integrand_log(parent_r[pi], x[0], x[1], x[2], x[3], sS, p,
-                                                   &k2, &kf);
+            const double f =
+                p->stable
+                    ? orc_nlo_integrand_stable(parent_r[pi], x[0], x[1], x[2], x[3], sS, p, &k2, &kf)
+                    : orc_nlo_integrand_log(parent_r[pi], x[0], x[1], x[2], x[3], sS, p, &k2, &kf);
             const double w = f * jac;
+            if (magnitude && !p->stable)
+                sm += (long double)(orc_nlo_integrand_magnitude(parent_r[pi], x[0], x[1], x[2], x[3],
+                                                                sS, p) *
+                                    fabs(jac));
             s += w;
             s2 += (long double)w * w;
             sk2 += (long double)(k2 * jac);
@@ -668,6 +752,7 @@ void orc_bk_nlo_sums(int parents, const double* parent_r, const orc_spline* sS, 
         sums[4 * pi + 1] = (double)s2;
         sums[4 * pi + 2] = (double)sk2;
         sums[4 * pi + 3] = (double)skf;
+        if (magnitude) magnitude[pi] = (double)sm;
         if (lh) {
             for (int k = 0; k < dim * bins; ++k) hist[(size_t)pi * dim * bins + k] = (double)lh[k];
             free(lh);

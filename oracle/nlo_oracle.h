@@ -34,9 +34,8 @@ typedef struct orc_bk_params {
     int symmetrize;     /* SYMMETRIZE_Z_Z2_INTEGRATION */
     int geometry_cos;   /* 1: X^2 = r^2 + z^2 - 2 r z cos(theta) (torch code);
                            0: X^2 = (r - z)^2 + 4 r z sin^2(theta / 2) */
-    int stable;         /* 0: K2/Kf exactly as S/bk_kernels/k2.py, kf.py;
-                           1: the same kernels rewritten without the 1/Delta
-                              cancellation (see orc_nlo_kernels_stable) */
+    int stable;         /* orc_bk_nlo_sums: 0 = reference integrand on the S
+                           spline, 1 = orc_nlo_integrand_stable on the N spline */
 } orc_bk_params;
 
 /* Natural cubic spline of log(y) against log(r) (S/numerics/interpolation.py:95-121). */
@@ -87,12 +86,29 @@ double orc_nlo_integrand_log(double r, double log_z, double log_z2, double theta
                              double theta_z2, const orc_spline* sS, const orc_bk_params* p,
                              double* k2_part, double* kf_part);
 
+/* Cancellation-free form of the same integrand (N spline, N-form dipoles,
+ * kernels from orc_nlo_kernels_stable), as the FP32 GPU path evaluates it. */
+double orc_nlo_integrand_stable(double r, double log_z, double log_z2, double theta_z,
+                                double theta_z2, const orc_spline* sN, const orc_bk_params* p,
+                                double* k2_part, double* kf_part);
+/* K2 J, K2s J, Kf J, Kfs J (J = z^2 z'^2) without the 1/Delta cancellation. */
+void orc_nlo_kernels_stable(double r, double z, double theta_z, double z2, double theta_z2,
+                            double nf, double nc, double* out);
+double orc_coth_minus_one(double L);
+double orc_h_ratio(double L);
+double orc_log_x_over_y(double r, double z, double theta, double xsq);
+
 /* K1 on the fixed production rule: (n-1) log-r panels x GL(nr) x GL(nt) in
  * v with theta = pi v^2 (K/bk/k1/kernel.cu:2040-2092). */
 void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN,
                   int nr, const double* gr_x, const double* gr_w,
                   int nt, const double* gt_x, const double* gt_w,
                   const orc_bk_params* p, double* out);
+
+void orc_k1_fixed_magnitude(int parents, const double* parent_r, const orc_spline* sN,
+                            int nr, const double* gr_x, const double* gr_w,
+                            int nt, const double* gt_x, const double* gt_w,
+                            const orc_bk_params* p, double* out);
 
 /* Scrambled Sobol integer states (gray-code order, SobolEngine semantics). */
 void orc_sobol_u32(int dim, const uint32_t* direction /* dim x 30 */, const uint32_t* shift,
@@ -108,14 +124,17 @@ void orc_vegas_map(int dim, int bins, int64_t n, const uint32_t* q, const double
 void orc_vegas_adapt(int dim, int bins, const double* edges, const double* hist,
                      const double* count, double eps, double* out);
 
-/* Per-parent K2 + Kf VEGAS sums over Sobol points [first, first + n):
+/* Per-parent K2 + Kf VEGAS sums over Sobol points [first, first + n)
+ * (sS: the S spline for p->stable = 0, the N spline for p->stable = 1):
  * sums[4 p + k] = (sum w, sum w^2, sum w_K2, sum w_Kf).  hist / count
  * (parents x 4 x bins, may be NULL) receive sum |w| and hits per bin. */
 void orc_bk_nlo_sums(int parents, const double* parent_r, const orc_spline* sS, int bins,
                      const double* edges /* parents x 4 x (bins + 1) */,
                      const double* lo, const double* width, const uint32_t* direction,
                      const uint32_t* shift, const uint32_t* first_q, int64_t first, int64_t n,
-                     const orc_bk_params* p, double* sums, double* hist, double* count);
+                     const orc_bk_params* p, double* sums, double* hist, double* count,
+                     double* magnitude /* parents, may be NULL: sum |jac| x rounding scale of
+                                          the reference integrand (p->stable = 0 only) */);
 
 int orc_num_threads(void);
 

@@ -133,6 +133,14 @@ def lib() -> ctypes.CDLL:
             _dp,
             _dp,
         ]
+        L.orc_nlo_integrand_stable.restype = ctypes.c_double
+        L.orc_nlo_integrand_stable.argtypes = [ctypes.c_double] * 5 + [
+            ctypes.POINTER(_Spline),
+            ctypes.POINTER(BKParams),
+            _dp,
+            _dp,
+        ]
+        L.orc_nlo_kernels_stable.argtypes = [ctypes.c_double] * 7 + [_dp]
         L.orc_k1_fixed.argtypes = [
             ctypes.c_int,
             _dp,
@@ -146,6 +154,7 @@ def lib() -> ctypes.CDLL:
             ctypes.POINTER(BKParams),
             _dp,
         ]
+        L.orc_k1_fixed_magnitude.argtypes = L.orc_k1_fixed.argtypes
         L.orc_sobol_u32.argtypes = [
             ctypes.c_int,
             _u32p,
@@ -189,6 +198,7 @@ def lib() -> ctypes.CDLL:
             ctypes.c_int64,
             ctypes.c_int64,
             ctypes.POINTER(BKParams),
+            _dp,
             _dp,
             _dp,
             _dp,
@@ -384,7 +394,30 @@ def nlo_integrand_log(r, log_z, log_z2, theta_z, theta_z2, sS: Spline, p: BKPara
     return np.array(tot), np.array(a), np.array(b)
 
 
-def k1_fixed(parent_r: np.ndarray, sN: Spline, nr=8, nt=24, p: BKParams | None = None) -> np.ndarray:
+def nlo_integrand_stable(r, log_z, log_z2, theta_z, theta_z2, sN: Spline, p: BKParams | None = None):
+    """Cancellation-free K2 + Kf integrand on the N spline: (total, k2 part, kf part)."""
+
+    p = p or params()
+    st = sN.struct()
+    k2 = ctypes.c_double()
+    kf = ctypes.c_double()
+    tot, a, b = [], [], []
+    for v in zip(np.ravel(r), np.ravel(log_z), np.ravel(log_z2), np.ravel(theta_z), np.ravel(theta_z2)):
+        tot.append(
+            lib().orc_nlo_integrand_stable(
+                *map(float, v), ctypes.byref(st), ctypes.byref(p), ctypes.byref(k2), ctypes.byref(kf)
+            )
+        )
+        a.append(k2.value)
+        b.append(kf.value)
+    return np.array(tot), np.array(a), np.array(b)
+
+
+def k1_fixed(
+    parent_r: np.ndarray, sN: Spline, nr=8, nt=24, p: BKParams | None = None, magnitude: bool = False
+) -> np.ndarray:
+    """K1 on the fixed rule; ``magnitude`` integrates |integrand| (rounding scale)."""
+
     p = p or params()
     parent_r = np.ascontiguousarray(parent_r, dtype=np.float64)
     gx, gw = np.polynomial.legendre.leggauss(nr)
@@ -392,7 +425,8 @@ def k1_fixed(parent_r: np.ndarray, sN: Spline, nr=8, nt=24, p: BKParams | None =
     gx, gw, tx, tw = (np.ascontiguousarray(a) for a in (gx, gw, tx, tw))
     out = np.empty(parent_r.size)
     st = sN.struct()
-    lib().orc_k1_fixed(
+    fn = lib().orc_k1_fixed_magnitude if magnitude else lib().orc_k1_fixed
+    fn(
         parent_r.size, _d(parent_r), ctypes.byref(st), nr, _d(gx), _d(gw), nt, _d(tx), _d(tw),
         ctypes.byref(p), _d(out),
     )
@@ -480,7 +514,7 @@ def nlo_bounds(r_grid_values: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
 
 def bk_nlo_sums(
     parent_r: np.ndarray,
-    sS: Spline,
+    spline: Spline,
     edges: np.ndarray,
     lo: np.ndarray,
     width: np.ndarray,
@@ -489,8 +523,13 @@ def bk_nlo_sums(
     n: int,
     p: BKParams | None = None,
     want_hist: bool = False,
+    want_magnitude: bool = False,
 ):
-    """Per-parent (sum w, sum w^2, sum K2 w, sum Kf w) and optional histograms."""
+    """Per-parent (sum w, sum w^2, sum K2 w, sum Kf w) and optional histograms.
+
+    ``spline`` is the S spline for the reference integrand (p.stable = 0) and
+    the N spline for the cancellation-free one (p.stable = 1).
+    """
 
     p = p or params()
     parent_r = np.ascontiguousarray(parent_r, dtype=np.float64)
@@ -500,7 +539,8 @@ def bk_nlo_sums(
     sums = np.empty((P, 4))
     hist = np.empty((P, 4, bins)) if want_hist else None
     count = np.empty((P, 4, bins)) if want_hist else None
-    st = sS.struct()
+    mag = np.empty(P) if want_magnitude else None
+    st = spline.struct()
     lib().orc_bk_nlo_sums(
         P,
         _d(parent_r),
@@ -518,7 +558,10 @@ def bk_nlo_sums(
         _d(sums),
         _d(hist) if want_hist else None,
         _d(count) if want_hist else None,
+        _d(mag) if want_magnitude else None,
     )
+    if want_magnitude:
+        return sums, hist, count, mag
     return sums, hist, count
 
 

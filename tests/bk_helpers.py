@@ -56,7 +56,8 @@ def oracle_rhs(r, N, edges, table, n, p, *, first=0, nr=8, nt=24):
     if active.size:
         k1[active] = O.k1_fixed(r[active], sN, nr=nr, nt=nt, p=p)
         lo, width = O.nlo_bounds(r)
-        sums, _, _ = O.bk_nlo_sums(r[active], sS, edges[active], lo, width, table, first, n, p)
+        spline = sN if p.stable else sS
+        sums, _, _ = O.bk_nlo_sums(r[active], spline, edges[active], lo, width, table, first, n, p)
         k2kf[active] = sums[:, 0] / n
     return k1, k2kf
 

@@ -1,0 +1,108 @@
+"""The C ABI boundary without a GPU: the library loads, exports every entry
+point include/nlo_gfx950.h declares, its host-only functions are right, and
+argument errors come back as status codes with messages before any launch."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from nlo_bk_optimized_amd import _lib
+
+HEADER = Path(__file__).resolve().parent.parent / "include" / "nlo_gfx950.h"
+
+
+def declared_functions() -> list[str]:
+    text = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b(nlo_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    assert "nlo_bk_nlo" in names and "nlo_sobol_draw" in names and "nlo_bk_evolver_step" in names
+    assert set(names) == set(_lib.SIGNATURES), set(names) ^ set(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(_lib.LIBRARY))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert _lib.lib().nlo_api_version() == 1
+
+
+def _gauss_legendre_mp(n: int):
+    """Nodes and weights by Newton iteration at 40 digits (the exact rule)."""
+
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    xs, ws = [], []
+    for i in range(n):
+        z = mp.cos(mp.pi * (i + mp.mpf(3) / 4) / (n + mp.mpf(1) / 2))
+        for _ in range(60):
+            p0, p1 = mp.mpf(1), z
+            for k in range(2, n + 1):
+                p0, p1 = p1, ((2 * k - 1) * z * p1 - (k - 1) * p0) / k
+            dp = n * (z * p1 - p0) / (z * z - 1)
+            z -= p1 / dp
+        xs.append(z)
+        ws.append(2 / ((1 - z * z) * dp * dp))
+    order = sorted(range(n), key=lambda k: xs[k])
+    return np.array([float(xs[k]) for k in order]), np.array([float(ws[k]) for k in order])
+
+
+@pytest.mark.parametrize("n", [1, 2, 8, 24, 48, 64])
+def test_gauss_legendre_exact_to_double(n):
+    x = np.empty(n)
+    w = np.empty(n)
+    assert _lib.lib().nlo_gauss_legendre(n, x.ctypes.data, w.ctypes.data) == 0
+    if n == 1:
+        np.testing.assert_array_equal(x, [0.0])
+        np.testing.assert_array_equal(w, [2.0])
+        return
+    ex, ew = _gauss_legendre_mp(n)
+    np.testing.assert_allclose(x, ex, rtol=0, atol=2.3e-16)
+    np.testing.assert_allclose(w, ew, rtol=1e-15, atol=0)
+    # numpy's leggauss (the reference's table source, K/quadrature.py:18) agrees to its accuracy
+    rx, rw = np.polynomial.legendre.leggauss(n)
+    np.testing.assert_allclose(x, rx, rtol=0, atol=1e-14)
+
+
+def test_invalid_arguments_return_status_and_message():
+    L = _lib.lib()
+    t = _lib.SobolTable()
+    t.dim = 0
+    assert L.nlo_sobol_draw(ctypes.byref(t), 0, 10, 0, None, None) == 1
+    assert b"dim" in L.nlo_last_error()
+    assert L.nlo_vegas_map(4, 1, 10, None, None, None, None, None, None, None, None) == 1
+    assert b"bins" in L.nlo_last_error()
+    ph = _lib.Physics(3.0, 5.0, 0.241, 1.0, 0.65, 1e-6, 1, 0)
+    # NF = 5 needs the threshold coupling, which is outside the implemented set
+    rc = L.nlo_bk_k1(8, 1, 1, 1, ctypes.byref(ph), 8, 24, 1, None)
+    assert rc == 4 and b"NF" in L.nlo_last_error()
+    with pytest.raises(RuntimeError, match="status 1"):
+        _lib.check(L.nlo_vegas_adapt(0, 4, 50, None, None, None, 1e-16, None, None), "adapt")
+
+
+def test_workspace_queries_are_host_only_and_monotone():
+    L = _lib.lib()
+    a = L.nlo_bk_nlo_workspace(64, 50, 1 << 14, 0)
+    b = L.nlo_bk_nlo_workspace(64, 50, 1 << 16, 0)
+    c = L.nlo_bk_nlo_workspace(64, 50, 1 << 14, 1)
+    assert 0 < a <= b and c > a
+    assert L.nlo_bk_nlo_workspace(2, 50, 10, 0) == 0  # R < 3 is invalid
+    assert L.nlo_vegas_histogram_workspace(4, 50, 4096) == 4 * 50 * 2 * 8
+    d = _lib.EvolverDesc()
+    d.R, d.bins, d.n_samples, d.nlo = 64, 50, 1 << 14, 1
+    assert L.nlo_bk_evolver_workspace(ctypes.byref(d)) > a
+
+
+def test_struct_layouts_match_the_header():
+    # nlo_sobol_table: dim + 8 x 30 directions + 8 shifts + 8 first points
+    assert ctypes.sizeof(_lib.SobolTable) == 4 * (1 + 8 * 30 + 8 + 8)
+    assert ctypes.sizeof(_lib.Physics) == 6 * 8 + 2 * 4
