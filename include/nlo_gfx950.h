@@ -126,8 +126,12 @@ int nlo_vegas_adapt(int32_t rows, int32_t dim, int32_t bins, const double* d_edg
 #define NLO_SPLINE_WORDS(R) (2 + (R) + 8 * ((R) - 1))
 
 /* Natural cubic log-log splines of N (clamped to [0,1] / force_positive) and
- * S = 1 - N on log r, one FP64 Thomas solve per row, plus an FP32 copy of the
- * coefficients (d_table32 may be NULL).  Replaces
+ * S = 1 - N on log r, one FP64 Thomas solve per row (d_table64: records
+ * {a, b, c, d} of ln N = a + b dx + c dx^2 + d dx^3, dx = ln r - ln r_i).
+ * d_table32 (may be NULL) holds the same cubics for the FP32 path in the
+ * local coordinate t = (ln r - ln r_0) / h - i, h = (ln r_{R-1} - ln r_0)/(R-1),
+ * multiplied by log2 e, so that N = 2^(a' + t (b' + t (c' + t d'))); it
+ * assumes the log-uniform r grid of the evolution.  Replaces
  * evolution_loglog_splines_mixed (K/interpolation.cu:374-451) and
  * _evolution_interpolators (S/bk/evolution.py:536-560). */
 int nlo_bk_spline_build(int32_t R, const double* d_r, const double* d_N, int32_t force_positive,

@@ -367,4 +367,186 @@ __device__ __forceinline__ NloStable<T> nlo_integrand_stable(T r, T log_r, T lz,
     return out;
 }
 
+// ------------------------------------------- FP32 path: fast primitives
+
+// e^x from the hardware 2^x with the product x log2(e) split so that its
+// rounding error (up to 1e-6 for |x| ~ 15) is carried: 2^(t + e) = 2^t (1 + e ln 2).
+__device__ __forceinline__ float fexp(float x) {
+    const float t = x * 1.44269502f;
+    const float e = fmaf(x, 1.44269502f, -t) + x * 1.9259629e-8f;
+    return __builtin_amdgcn_exp2f(t) * fmaf(e, 0.693147182f, 1.0f);
+}
+
+// ln x from the hardware log2 (absolute error ~1e-7: used only where the
+// result is not itself a small difference).
+__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147182f; }
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+// sin y for y in [-pi, pi]: fold |y| > pi/2 with pi - |y| computed exactly
+// (Sterbenz) plus the low part of pi, then the Taylor polynomial to y^11
+// (relative error < 6e-8, accurate for small y).
+__device__ __forceinline__ float fsin_pi(float y) {
+    const float ay = fabsf(y);
+    float v = ay;
+    if (ay > 1.57079637f) v = (3.14159274f - ay) + -8.74227766e-8f;
+    const float v2 = v * v;
+    float p = fmaf(v2, -2.50521084e-8f, 2.75573192e-6f);
+    p = fmaf(v2, p, -1.98412698e-4f);
+    p = fmaf(v2, p, 8.33333333e-3f);
+    p = fmaf(v2, p, -1.66666667e-1f);
+    p = fmaf(v2 * v, p, v);
+    return y < 0.0f ? -p : p;
+}
+
+// ln(1 + u) for |u| < 0.5: 2 atanh(s), s = u / (2 + u), |s| < 0.2, series to s^9.
+__device__ __forceinline__ float flog1p_small(float u) {
+    const float s = u * frcp(2.0f + u);
+    const float s2 = s * s;
+    float p = fmaf(s2, 1.0f / 9.0f, 1.0f / 7.0f);
+    p = fmaf(s2, p, 0.2f);
+    p = fmaf(s2, p, 1.0f / 3.0f);
+    p = fmaf(s2, p, 1.0f);
+    return 2.0f * s * p;
+}
+
+// x / (e^x - 1) and x e^x / (e^x - 1) for x = 2L: Bernoulli series
+// 1 - x/2 + x^2/12 - x^4/720 + x^6/30240 for |x| < 0.25, else from e^x
+// (no cancellation there; the second form avoids 1 - e^x for x << 0).
+__device__ __forceinline__ void fh_pair(float x, float& h, float& hm) {
+    if (fabsf(x) < 0.25f) {
+        const float x2 = x * x;
+        float p = fmaf(x2, 1.0f / 30240.0f, -1.0f / 720.0f);
+        p = fmaf(x2, p, 1.0f / 12.0f);
+        h = fmaf(x2, p, fmaf(x, -0.5f, 1.0f));
+        hm = h + x;
+    } else {
+        const float E = fexp(fminf(x, 86.0f));
+        h = x * frcp(E - 1.0f);
+        hm = h * E;
+    }
+}
+
+// FP32 spline table: per interval the cubic in the local coordinate
+// t = (ln d - ln r_0) / h - k in [0, 1), coefficients pre-multiplied by
+// log2(e) so that N = 2^(a + t (b + t (c + t d))): one 16-byte LDS read.
+struct FastSpline {
+    const float4* rec;
+    float x0, inv_h, kmax;  // kmax = R - 2
+    float log_min, log_max;
+};
+
+__device__ __forceinline__ float fspline_log2n(const FastSpline& s, float log_d) {
+    const float u = fminf(fmaxf((log_d - s.x0) * s.inv_h, 0.0f), s.kmax + 1.0f);
+    const float k = fminf(floorf(u), s.kmax);
+    const float t = u - k;
+    const float4 c = s.rec[(int)k];
+    return fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
+}
+
+__device__ __forceinline__ float fspline_n(const FastSpline& s, float log_d) {
+    float out = __builtin_amdgcn_exp2f(fspline_log2n(s, log_d));
+    out = log_d < s.log_min ? 0.0f : out;
+    return log_d > s.log_max ? 1.0f : out;
+}
+
+// N and S = 1 - N.  Near N = 1 the subtraction would keep only the absolute
+// precision of N, so S = -expm1(p ln 2) comes from the exponent p = log2 N
+// by its series when |p ln 2| < 1/4 (relative error < 2e-8).
+__device__ __forceinline__ void fspline_ns(const FastSpline& s, float log_d, float& n, float& sv) {
+    const float p = fspline_log2n(s, log_d);
+    n = __builtin_amdgcn_exp2f(p);
+    const float x = p * 0.693147182f;
+    float e = fmaf(x, 1.0f / 720.0f, 1.0f / 120.0f);
+    e = fmaf(x, e, 1.0f / 24.0f);
+    e = fmaf(x, e, 1.0f / 6.0f);
+    e = fmaf(x, e, 0.5f);
+    e = fmaf(x * x, e, x);  // expm1(x)
+    sv = fabsf(x) < 0.25f ? -e : 1.0f - n;
+    if (log_d < s.log_min) { n = 0.0f; sv = 1.0f; }
+    if (log_d > s.log_max) { n = 1.0f; sv = 0.0f; }
+}
+
+// The FP32 K2 + Kf integrand: the cancellation-free formula of
+// nlo_integrand_stable, every transcendental on the fast primitives above,
+// cos(theta) = 1 - 2 sin^2(theta / 2), logs of distances from the log1p
+// forms, N from the local-coordinate spline.
+__device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
+    float r, float log_r, float lz, float lz2, float tz, float tz2, const FastSpline& sp, float b0,
+    float log_mu0, float log_scale0, float nf_over_nc, float nc, float norm, bool symmetrize) {
+    NloStable<float> out{0.0f, 0.0f, 0.0f};
+    const float z = fexp(lz), z2 = fexp(lz2);
+    const float s1 = fsin_pi(0.5f * tz), s2 = fsin_pi(0.5f * tz2);
+    const float s3 = fsin_pi(0.5f * (tz - tz2));
+    const float xsq = fmaf(4.0f * r * z, s1 * s1, (r - z) * (r - z));
+    const float x2sq = fmaf(4.0f * r * z2, s2 * s2, (r - z2) * (r - z2));
+    const float zsq = fmaf(4.0f * z * z2, s3 * s3, (z - z2) * (z - z2));
+    if (!(xsq >= 1e-37f) || !(x2sq >= 1e-37f) || !(zsq >= 1e-37f)) return out;
+    const float X = fsqrt(xsq), X2 = fsqrt(x2sq), Z = fsqrt(zsq);
+    // ln(X / Y): (X^2 - z^2) / z^2 = rho (rho - 2 cos t), cos t = 1 - 2 s^2
+    const float rho1 = r * frcp(z), rho2 = r * frcp(z2);
+    const float u1 = rho1 * (rho1 - 2.0f + 4.0f * s1 * s1);
+    const float u2 = rho2 * (rho2 - 2.0f + 4.0f * s2 * s2);
+    const float lxy = fabsf(u1) < 0.5f ? 0.5f * flog1p_small(u1) : 0.5f * flog(xsq) - lz;
+    const float lx2y2 = fabsf(u2) < 0.5f ? 0.5f * flog1p_small(u2) : 0.5f * flog(x2sq) - lz2;
+    const float log_x = lz + lxy, log_x2 = lz2 + lx2y2, log_zz = 0.5f * flog(zsq);
+    float n_x, s_x, n_y, s_y, n_x2, s_x2, n_y2, s_y2;
+    fspline_ns(sp, log_x, n_x, s_x);
+    fspline_ns(sp, lz, n_y, s_y);
+    fspline_ns(sp, log_x2, n_x2, s_x2);
+    fspline_ns(sp, lz2, n_y2, s_y2);
+    const float n_z = fspline_n(sp, log_zz);
+    float dip = s_x * (n_z + n_y2 - n_y - n_z * n_y2);
+    float dips = s_x2 * (n_z + n_y - n_y2 - n_z * n_y);
+
+    const float L = lxy - lx2y2;
+    float h, hm;
+    fh_pair(2.0f * L, h, hm);
+    const float aL = fabsf(L);
+    float g;
+    if (aL < 1.0f) {
+        const float y = L * L;
+        const float p = fmaf(y, fmaf(y, fmaf(y, fmaf(y, 1.0f / 518918400.0f, 1.0f / 3991680.0f),
+                                              1.0f / 45360.0f), 1.0f / 840.0f), 1.0f / 30.0f);
+        const float q = fmaf(y, fmaf(y, fmaf(y, fmaf(y, 1.0f / 39916800.0f, 1.0f / 362880.0f),
+                                              1.0f / 5040.0f), 1.0f / 120.0f), 1.0f / 6.0f);
+        g = y * fmaf(p, y, 1.0f / 3.0f) * frcp(fmaf(q, y, 1.0f));
+    } else {
+        g = aL + (L >= 0.0f ? h : hm) - 1.0f;  // |L| + h(|L|) - 1
+    }
+    const float iz = frcp(zsq);
+    const float A = z * z2 * iz;
+    const float rZ = r * frcp(Z);
+    const float B = rZ * z2 * frcp(X2), C = r * r * frcp(X * X2), D = rZ * z * frcp(X);
+    const float B2 = B * B, C2 = C * C, D2 = D * D;
+    const float A2g = 2.0f * A * A * g;
+    float k = A2g - 4.0f * h * B2 + h * C2 + 2.0f * L * D2;
+    float ks = A2g - 4.0f * hm * D2 + hm * C2 - 2.0f * L * B2;
+    const float cut = 1e10f * fexp(2.0f * (lz + lz2));  // |K2| > 1e10 (S/bk/derivatives.py:589)
+    if (fabsf(k) > cut && fabsf(dip) < 1e-10f) { k = 0.0f; dip = 0.0f; }
+    if (fabsf(ks) > cut && fabsf(dips) < 1e-10f) { ks = 0.0f; dips = 0.0f; }
+    float k2r = symmetrize ? (k * dip + ks * dips) * 0.5f : k * dip;
+    float kfr = 0.0f;
+    if (nf_over_nc > 0.0f) {
+        const float kf = (h * B2 - A2g) * nf_over_nc;
+        const float kfs = (hm * D2 - A2g) * nf_over_nc;
+        const float df = s_y * (n_x - n_x2);
+        const float dfs = s_y2 * (n_x2 - n_x);
+        kfr = -(symmetrize ? (kf * df + kfs * dfs) * 0.5f : kf * df);
+    }
+    const float log_min = fminf(fminf(fminf(log_r, log_x), lz), fminf(fminf(log_x2, lz2), log_zz));
+    const float right = 5.0f * (log_scale0 - 2.0f * log_min);
+    const float hi = fmaxf(log_mu0, right);
+    const float as = frcp(b0 * (0.2f * (hi + flog(1.0f + fexp(-fabsf(log_mu0 - right))))));
+    const float scale = (as * nc) * (as * nc) * norm;
+    k2r *= scale;
+    kfr *= scale;
+    out.k2 = isfinite(k2r) ? k2r : 0.0f;
+    out.kf = isfinite(kfr) ? kfr : 0.0f;
+    out.total = out.k2 + out.kf;
+    return out;
+}
+
 }  // namespace nlo

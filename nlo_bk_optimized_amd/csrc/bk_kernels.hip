@@ -123,10 +123,14 @@ __global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* _
         t64[o + 2] = c;
         t64[o + 3] = d;
         if (t32) {
-            t32[o] = (float)a;
-            t32[o + 1] = (float)b;
-            t32[o + 2] = (float)c;
-            t32[o + 3] = (float)d;
+            // local coordinate t = (ln d - ln r_0) / h_avg - i and base-2
+            // exponent: N = 2^(a' + t (b' + t (c' + t d')))
+            const double hav = (x[R - 1] - x[0]) / (double)(R - 1);
+            const double l2e = 1.4426950408889634;
+            t32[o] = (float)(a * l2e);
+            t32[o + 1] = (float)(b * hav * l2e);
+            t32[o + 2] = (float)(c * hav * hav * l2e);
+            t32[o + 3] = (float)(d * hav * hav * hav * l2e);
         }
     }
     for (int i = threadIdx.x; i < R; i += blockDim.x) {
@@ -170,6 +174,7 @@ __global__ void __launch_bounds__(256) k1_kernel(int R, const double* __restrict
                                                  GaussRule g, double* __restrict__ k1) {
     extern __shared__ double sh[];
     __shared__ double red[4];
+    __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], gxt[kMaxGaussT], gwt[kMaxGaussT];
     const int p = blockIdx.x;
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
         if (threadIdx.x == 0) k1[p] = 0.0;
@@ -177,6 +182,14 @@ __global__ void __launch_bounds__(256) k1_kernel(int R, const double* __restrict
     }
     LdsSpline<double> s;
     stage_spline<double>(t64, R, 0, sh, sh + R, s);
+    for (int i = threadIdx.x; i < g.nr; i += blockDim.x) {
+        gxr[i] = g.xr[i];
+        gwr[i] = g.wr[i];
+    }
+    for (int i = threadIdx.x; i < g.nt; i += blockDim.x) {
+        gxt[i] = g.xt[i];
+        gwt[i] = g.wt[i];
+    }
     __syncthreads();
     const double r_min = t64[0], r_max = t64[1];
     const double rp = r[p];
@@ -191,11 +204,11 @@ __global__ void __launch_bounds__(256) k1_kernel(int R, const double* __restrict
         const int i = loc / g.nt, j = loc - i * g.nt;
         const double lower = s.x[panel], upper = s.x[panel + 1];
         const double center = (lower + upper) / 2.0, half = (upper - lower) / 2.0;
-        const double log_z = center + half * g.xr[i];
+        const double log_z = center + half * gxr[i];
         const double z = exp(log_z);
-        const double v = (g.xt[j] + 1.0) / 2.0;
+        const double v = (gxt[j] + 1.0) / 2.0;
         const double theta = kPi * v * v;
-        const double w = (half * g.wr[i]) * (g.wt[j] * kPi * v);
+        const double w = (half * gwr[i]) * (gwt[j] * kPi * v);
         total += k1_integrand64(rp, z, theta, n_r, a_r, s, r_min, r_max, c) * (2.0 * z * z) * w;
     }
     total = block_sum_256(total, red);
@@ -213,7 +226,7 @@ __host__ __device__ inline NloLayout nlo_layout(int R, int bins, int spt, bool h
     NloLayout L{};
     size_t o = 0;
     L.x = o;
-    o = align_up(o + (size_t)R * tsize, 16);
+    if (tsize == sizeof(double)) o = align_up(o + (size_t)R * tsize, 16);  // FP32 needs no nodes
     L.rec = o;
     o = align_up(o + (size_t)4 * (R - 1) * tsize, 16);
     L.edges = o;
@@ -274,15 +287,37 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
             }
         return;
     }
-    T* sx = reinterpret_cast<T*>(smem + lay.x);
-    T* srec = reinterpret_cast<T*>(smem + lay.rec);
+    unsigned char* sx = smem + lay.x;
+    unsigned char* srec = smem + lay.rec;
     T* E = reinterpret_cast<T*>(smem + lay.edges);
     uint32_t* dir = reinterpret_cast<uint32_t*>(smem + lay.dir);
-    LdsSpline<T> spl;
-    // FP64 reference formula reads the S spline (row 1); the stable forms read N (row 0).
-    const int row = MODE == NLO_PREC_FP64 ? 1 : 0;
-    if constexpr (MODE == NLO_PREC_FP32) stage_spline<float>(a.t32, a.R, row, sx, srec, spl);
-    else stage_spline<double>(a.t64, a.R, row, sx, srec, spl);
+    LdsSpline<double> spl{};
+    FastSpline fsp{};
+    if constexpr (MODE == NLO_PREC_FP32) {
+        // N records (row 0) of the local-coordinate FP32 table, 16 B each
+        float4* rec4 = reinterpret_cast<float4*>(srec);
+        const float4* src = reinterpret_cast<const float4*>(a.t32 + 2 + a.R);
+        const bool aligned = ((reinterpret_cast<uintptr_t>(a.t32) + (2 + a.R) * 4) & 15) == 0;
+        for (int i = threadIdx.x; i < a.R - 1; i += blockDim.x) {
+            if (aligned) {
+                rec4[i] = src[i];
+            } else {
+                const float* f = a.t32 + 2 + a.R + 4 * i;
+                rec4[i] = make_float4(f[0], f[1], f[2], f[3]);
+            }
+        }
+        const double x0 = a.t64[2], xl = a.t64[2 + a.R - 1];
+        fsp.rec = rec4;
+        fsp.x0 = (float)x0;
+        fsp.inv_h = (float)((double)(a.R - 1) / (xl - x0));
+        fsp.kmax = (float)(a.R - 2);
+        fsp.log_min = (float)x0;
+        fsp.log_max = (float)xl;
+    } else {
+        // FP64 reference formula reads the S spline (row 1); the stable form reads N (row 0).
+        stage_spline<double>(a.t64, a.R, MODE == NLO_PREC_FP64 ? 1 : 0, reinterpret_cast<double*>(sx),
+                             reinterpret_cast<double*>(srec), spl);
+    }
     const int ne = 4 * (a.bins + 1);
     const double* ep = a.edges + (size_t)p * ne;
     for (int i = threadIdx.x; i < ne; i += blockDim.x) E[i] = (T)ep[i];
@@ -330,19 +365,27 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     }
 
     double s1 = 0.0, s2 = 0.0, sk2 = 0.0, skf = 0.0;
-    const int64_t hi0 = (a.first >> 6) + (int64_t)blk * a.spt * 4 + wave;
+    // Each wave walks spt consecutive 64-point groups hi; the wave-uniform
+    // high part of the gray code (direction bits 6..29) is built once and
+    // then updated by one XOR per group: gray(h) ^ gray(h - 1) = lowest set
+    // bit of h.
+    const int64_t hi0 = (a.first >> 6) + ((int64_t)blk * 4 + wave) * a.spt;
+    uint32_t vhi[4] = {0u, 0u, 0u, 0u};
+    {
+        uint64_t g = (uint64_t)hi0 ^ ((uint64_t)hi0 >> 1);
+        for (int b = 6; g && b < 30; ++b, g >>= 1)
+            if (g & 1u)
+                for (int d = 0; d < 4; ++d) vhi[d] ^= dir[d * 30 + b];
+    }
     for (int j = 0; j < a.spt; ++j) {
-        const int64_t hi = hi0 + (int64_t)j * 4;
+        const int64_t hi = hi0 + j;
+        if (j > 0) {
+            const int b = 6 + __builtin_ctzll((unsigned long long)hi);
+            if (b < 30)
+                for (int d = 0; d < 4; ++d) vhi[d] ^= dir[d * 30 + b];
+        }
         const int64_t sample = (hi << 6) + lane - a.first;
         const bool valid = sample < a.n;
-        // wave-uniform high part of the gray code (bits 6..29)
-        uint32_t vhi[4] = {0u, 0u, 0u, 0u};
-        {
-            uint64_t g = (uint64_t)hi ^ ((uint64_t)hi >> 1);
-            for (int b = 6; g && b < 30; ++b, g >>= 1)
-                if (g & 1u)
-                    for (int d = 0; d < 4; ++d) vhi[d] ^= dir[d * 30 + b];
-        }
         const int par = (int)(hi & 1);
         T x[4];
         T jac = T(1);
@@ -354,7 +397,7 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
             const uint64_t scaled = (uint64_t)q * (uint64_t)a.bins;
             uint32_t b = (uint32_t)(scaled >> 30);
             b = b > (uint32_t)(a.bins - 1) ? (uint32_t)(a.bins - 1) : b;
-            const T frac = (T)(double)(scaled - ((uint64_t)b << 30)) * (T)0x1p-30;
+            const T frac = (T)(uint32_t)(scaled - ((uint64_t)b << 30)) * (T)0x1p-30;
             const T left = E[d * (a.bins + 1) + b], right = E[d * (a.bins + 1) + b + 1];
             const T w = right - left;
             x[d] = lo[d] + wd[d] * (left + w * frac);
@@ -370,15 +413,21 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
                 wv = f.total * (double)jac;
                 k2 = f.k2 * (double)jac;
                 kf = f.kf * (double)jac;
-            } else {
-                const NloStable<T> f = nlo_integrand_stable<T>(
-                    r_t, log_r_t, x[0], x[1], x[2], x[3], spl, (T)a.c.b0, (T)a.c.log_mu0,
-                    (T)a.c.log_scale0, (T)(a.c.nf / a.c.nc), (T)a.c.nc, (T)a.c.norm,
+            } else if constexpr (MODE == NLO_PREC_FP32) {
+                const NloStable<float> f = nlo_integrand_fp32(
+                    r_t, log_r_t, x[0], x[1], x[2], x[3], fsp, (float)a.c.b0, (float)a.c.log_mu0,
+                    (float)a.c.log_scale0, (float)(a.c.nf / a.c.nc), (float)a.c.nc, (float)a.c.norm,
                     a.c.symmetrize != 0);
-                const T wt = f.total * jac;
-                wv = (double)wt;
+                wv = (double)(f.total * jac);
                 k2 = (double)(f.k2 * jac);
                 kf = (double)(f.kf * jac);
+            } else {
+                const NloStable<double> f = nlo_integrand_stable<double>(
+                    rp, log(rp), x[0], x[1], x[2], x[3], spl, a.c.b0, a.c.log_mu0, a.c.log_scale0,
+                    a.c.nf / a.c.nc, a.c.nc, a.c.norm, a.c.symmetrize != 0);
+                wv = f.total * jac;
+                k2 = f.k2 * jac;
+                kf = f.kf * jac;
             }
             s1 += wv;
             s2 += wv * wv;
@@ -435,7 +484,9 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     }
 }
 
-// Fixed-order sum of the per-block partials of each parent.
+// Fixed-order sum of the per-block partials of each parent: each thread adds
+// its strided partials in order, then a fixed butterfly over lanes and a
+// fixed sum over waves.
 __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bins, int hist,
                                                          const double* __restrict__ part,
                                                          const double* __restrict__ ph,
@@ -443,6 +494,7 @@ __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bin
                                                          double* __restrict__ sums,
                                                          double* __restrict__ hsum,
                                                          double* __restrict__ csum) {
+    __shared__ double red[4][4];
     const int p = blockIdx.x;  // 0 .. R-1
     if (p == 0) {
         if (threadIdx.x < 4) sums[threadIdx.x] = 0.0;
@@ -454,10 +506,24 @@ __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bin
         return;
     }
     const size_t base = (size_t)(p - 1) * bpp;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < bpp; b += blockDim.x) {
+        const double4 v = *reinterpret_cast<const double4*>(part + (base + b) * 4);
+        t[0] += v.x;
+        t[1] += v.y;
+        t[2] += v.z;
+        t[3] += v.w;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < 4; ++c) {
+        const double v = wave_sum(t[c]);
+        if (lane == 0) red[wave][c] = v;
+    }
+    __syncthreads();
     if (threadIdx.x < 4) {
-        double t = 0.0;
-        for (int b = 0; b < bpp; ++b) t += part[(base + b) * 4 + threadIdx.x];
-        sums[(size_t)p * 4 + threadIdx.x] = t;
+        double v = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
+        sums[(size_t)p * 4 + threadIdx.x] = v;
     }
     if (hist) {
         for (int k = threadIdx.x; k < 4 * bins; k += blockDim.x) {
@@ -472,7 +538,6 @@ __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bin
     }
 }
 
-// S/bk/evolution.py:340-351 plus the active set of :501.
 // counter (optional) accumulates the number of active parents: integer
 // adds, so the total is exact whatever the order.
 __global__ void rhs_finalize_kernel(int R, const double* __restrict__ N, const double* __restrict__ k1,
