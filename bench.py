@@ -32,7 +32,7 @@ METRIC = "NLO-BK evolution wall time + integrand samples/s, FP32 vs FP64 at fixe
 # Algorithmic flops per K2 + Kf sample of the FP32 path (VEGAS map, the
 # cancellation-free integrand and the accumulation), counted on the source
 # with add/sub/mul/div/sqrt/exp/log/log1p/expm1/sin/cos = 1, fma = 2,
-# compares and selects = 0; itemised in DESIGN.md.
+# compares and selects = 0.
 FLOPS_PER_SAMPLE = 263
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 FP64_PEAK_TFLOPS = 78.6
@@ -56,8 +56,8 @@ def main() -> int:
     if args.gpus != 1:
         print(
             f"bench.py runs on one GPU only (got --gpus {args.gpus}): the BK right-hand side "
-            "shards over parents with one all-gather of N per RK stage, which this build "
-            "does not launch; run with --gpus 1",
+            "would shard over parents with one all-gather of N per RK stage, which is not "
+            "implemented; run with --gpus 1",
             file=sys.stderr,
         )
         return 2

@@ -229,8 +229,8 @@ __host__ __device__ inline NloLayout nlo_layout(int R, int bins, int spt, bool h
     if (tsize == sizeof(double)) o = align_up(o + (size_t)R * tsize, 16);  // FP32 needs no nodes
     L.rec = o;
     o = align_up(o + (size_t)4 * (R - 1) * tsize, 16);
-    L.edges = o;
-    o = align_up(o + (size_t)4 * (bins + 1) * tsize, 16);
+    L.edges = o;  // FP64: 4 x (bins + 1) edges; FP32: 4 x bins (left, width) pairs
+    o = align_up(o + (size_t)4 * (bins + 1) * sizeof(double), 16);
     L.dir = o;
     o = align_up(o + (size_t)4 * 30 * sizeof(uint32_t), 16);
     if (hist) {
@@ -318,9 +318,28 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
         stage_spline<double>(a.t64, a.R, MODE == NLO_PREC_FP64 ? 1 : 0, reinterpret_cast<double*>(sx),
                              reinterpret_cast<double*>(srec), spl);
     }
+    const double r0 = a.r[0], rmax = a.r[a.R - 1];
+    // nlo_vegas_bounds (S/bk/derivatives.py:424-431)
+    const double lo_r = log(0.5 * r0);
+    const double w_r = log(2.0 * rmax) - lo_r;
     const int ne = 4 * (a.bins + 1);
     const double* ep = a.edges + (size_t)p * ne;
-    for (int i = threadIdx.x; i < ne; i += blockDim.x) E[i] = (T)ep[i];
+    if constexpr (MODE == NLO_PREC_FP32) {
+        // Per bin, the absolute left end lo + w left and width w (right - left)
+        // of the mapped interval, formed in FP64 and rounded once: the sample
+        // is then x = X_b + W_b frac with one more rounding, where forming
+        // lo + w (left + width frac) in FP32 cancels (lo = ln(r_0 / 2) against
+        // the scaled edge) and leaves several ulp in ln z.
+        float2* E2 = reinterpret_cast<float2*>(smem + lay.edges);
+        for (int i = threadIdx.x; i < 4 * a.bins; i += blockDim.x) {
+            const int d = i / a.bins, b = i - d * a.bins;
+            const double lo_d = d < 2 ? lo_r : 0.0, w_d = d < 2 ? w_r : 2.0 * kPi;
+            const double left = ep[d * (a.bins + 1) + b], right = ep[d * (a.bins + 1) + b + 1];
+            E2[i] = make_float2((float)(lo_d + w_d * left), (float)(w_d * (right - left)));
+        }
+    } else {
+        for (int i = threadIdx.x; i < ne; i += blockDim.x) E[i] = ep[i];
+    }
     for (int i = threadIdx.x; i < 120; i += blockDim.x) dir[i] = a.dir[i / 30][i % 30];
     unsigned long long* hh = nullptr;
     unsigned int* hc = nullptr;
@@ -338,15 +357,16 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     }
     __syncthreads();
 
-    const double r0 = a.r[0], rmax = a.r[a.R - 1];
     const double rp = a.r[p];
-    // nlo_vegas_bounds (S/bk/derivatives.py:424-431)
-    const double lo_r = log(0.5 * r0);
-    const double w_r = log(2.0 * rmax) - lo_r;
     const T lo[4] = {(T)lo_r, (T)lo_r, T(0), T(0)};
     const T wd[4] = {(T)w_r, (T)w_r, (T)(2.0 * kPi), (T)(2.0 * kPi)};
     T vol = T(1);
-    for (int d = 0; d < 4; ++d) vol = vol * wd[d];
+    if constexpr (MODE == NLO_PREC_FP32) {
+        const T bb = (T)a.bins;  // the widths W_b already carry the volume
+        vol = (bb * bb) * (bb * bb);
+    } else {
+        for (int d = 0; d < 4; ++d) vol = vol * wd[d];
+    }
     const T r_t = (T)rp;
     const T log_r_t = (T)log(rp);
     const double r_min64 = a.t64[0], r_max64 = a.t64[1];
@@ -393,15 +413,24 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
         for (int d = 0; d < 4; ++d) {
             uint32_t q = vhi[d] ^ vlo[d][par];
             if (hi == 0 && lane == 0) q = a.first_q[d];
-            // bin = min(floor(u B), B - 1), frac = u B - bin with u = q 2^-30
-            const uint64_t scaled = (uint64_t)q * (uint64_t)a.bins;
-            uint32_t b = (uint32_t)(scaled >> 30);
-            b = b > (uint32_t)(a.bins - 1) ? (uint32_t)(a.bins - 1) : b;
-            const T frac = (T)(uint32_t)(scaled - ((uint64_t)b << 30)) * (T)0x1p-30;
-            const T left = E[d * (a.bins + 1) + b], right = E[d * (a.bins + 1) + b + 1];
-            const T w = right - left;
-            x[d] = lo[d] + wd[d] * (left + w * frac);
-            jac = jac * ((T)a.bins * w);
+            // bin = min(floor(u B), B - 1), frac = u B - bin with u = q 2^-30:
+            // for q < 2^30, 4 q B = bin 2^32 + frac 2^32 (high and low words of
+            // one 32 x 32 product); q = 2^30 (a rounded first point) is the
+            // clamped case bin = B - 1, frac = 1.
+            const uint32_t q4 = q << 2;
+            const bool top = q >= (1u << 30);
+            const uint32_t b = top ? (uint32_t)(a.bins - 1) : __umulhi(q4, (uint32_t)a.bins);
+            const T frac = top ? T(1) : (T)(q4 * (uint32_t)a.bins) * (T)0x1p-32;
+            if constexpr (MODE == NLO_PREC_FP32) {
+                const float2 e = reinterpret_cast<const float2*>(E)[d * a.bins + b];
+                x[d] = fmaf(e.y, frac, e.x);
+                jac = jac * e.y;
+            } else {
+                const T left = E[d * (a.bins + 1) + b], right = E[d * (a.bins + 1) + b + 1];
+                const T w = right - left;
+                x[d] = lo[d] + wd[d] * (left + w * frac);
+                jac = jac * ((T)a.bins * w);
+            }
             packed |= b << (8 * d);
         }
         jac = vol * jac;

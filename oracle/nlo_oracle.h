@@ -1,8 +1,8 @@
 /*
  * FP64 CPU oracle for the NLO BK right-hand side and rapidity driver.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path:
- * only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * TEST INFRASTRUCTURE ONLY.  This library is the reference the HIP path is
+ * tested against: only tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
  * may load it.  The product library (nlo_bk_optimized_amd/) never links,
  * loads or calls it.
  *

@@ -1,6 +1,6 @@
 """ctypes binding of the FP64 CPU oracle (``libnlo_oracle.so``).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only ``tests/``,
+TEST INFRASTRUCTURE ONLY: the reference the HIP path is tested against.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
 import this module.  The product package never imports it.
 
