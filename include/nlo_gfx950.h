@@ -226,6 +226,113 @@ const double* nlo_bk_evolver_k2kf(nlo_bk_evolver* ev);
 int nlo_bk_evolver_step(nlo_bk_evolver* ev, hipStream_t stream);
 int nlo_bk_evolver_destroy(nlo_bk_evolver* ev);
 
+/* ================================================================== DIS */
+
+/* Special functions on device arrays (n doubles in, n doubles out), in FP64
+ * (NLO_PREC_FP64) or computed in FP32 (NLO_PREC_FP32).  Replace the
+ * reference's bessel_K0/K1/K2 and Li2 (S/numerics/special.py:18-97). */
+#define NLO_FN_K0 0
+#define NLO_FN_K1 1
+#define NLO_FN_K2 2
+#define NLO_FN_LI2 3
+int nlo_dis_special(int32_t fn, int64_t n, const double* d_x, int32_t precision, double* d_out,
+                    hipStream_t stream);
+
+#define NLO_DIPOLE_GBW 0      /* S/dipole/amplitude.py:22-46 */
+#define NLO_DIPOLE_BK_TABLE 1 /* S/dipole/amplitude.py:49-270 */
+
+/* A dipole amplitude N(r, Y).  For a BK table the device arrays hold r[R]
+ * (logarithmic grid), Y[T] = table rapidities + max(0, ln(1/x0)) and N[T x R];
+ * `initialized` selects the spline set up by nlo_dis_bk_initialize at y_init
+ * (with its coefficients d_init_coef, 4 (R - 1) doubles, and
+ * maxr_interpolate) wherever BKDipole.interpolator_initialized would. */
+typedef struct nlo_dis_dipole {
+    int32_t kind;
+    int32_t initialized;
+    double qs0sqr, lambda, gamma, x0; /* GBW */
+    int32_t R, T;
+    const double* d_r;
+    const double* d_Y;
+    const double* d_N;
+    double y_init, maxr_interpolate;
+    const double* d_init_coef;
+} nlo_dis_dipole;
+
+/* N(r_i, Y_i) for n device points (BKDipole.dipole_amplitude /
+ * GBW.dipole_amplitude, S/dipole/amplitude.py:38-43, 133-168). */
+int nlo_dis_dipole_amplitude(const nlo_dis_dipole* dipole, int64_t n, const double* d_r,
+                             const double* d_Y, double* d_out, hipStream_t stream);
+
+/* BKDipole.initialize_interpolation (S/dipole/amplitude.py:111-131, 242-260)
+ * at rapidity Y: the natural spline in r through the amplitude at the table
+ * nodes (first and last moved inwards by 1e-4) into d_coef (4 (R - 1)
+ * doubles) and maxr_interpolate (or -1) into d_maxr (one double). */
+int nlo_dis_bk_initialize(const nlo_dis_dipole* dipole, double Y, double* d_coef, double* d_maxr,
+                          hipStream_t stream);
+
+/* Integrand kinds (contributions) on the unit cube; dimension in brackets.
+ * LO sums every quark of the descriptor; the others use quark 0. */
+#define NLO_DIS_LO 0             /* [2] S/dis/observables.py:345-352, S/dis/lo.py:12-51 */
+#define NLO_DIS_DIP_T0 1         /* [2] integrand_dip_massive "T0" (S/dis/observables.py:107-172) */
+#define NLO_DIS_DIP_T1 2         /* [3] "T1" */
+#define NLO_DIS_DIP_T2 3         /* [4] "T2" */
+#define NLO_DIS_DIP_L_CONST 4    /* [2] "Omega_L_const" */
+#define NLO_DIS_DIP_L_AB 5       /* [3] "ab" */
+#define NLO_DIS_DIP_L_CD 6       /* [4] "cd" */
+#define NLO_DIS_QG_I1 7          /* [5] integrand_qgunsub_massive "I1" (:175-250) */
+#define NLO_DIS_QG_I2 8          /* [6] "I2" */
+#define NLO_DIS_QG_I3 9          /* [7] "I3" */
+#define NLO_DIS_QG_I2_NESTED 10  /* [5] integrand_qg_nested_massive "I2" (:253-331) */
+#define NLO_DIS_QG_I3_NESTED 11  /* [5] "I3" */
+#define NLO_DIS_KINDS 12
+#define NLO_DIS_MAX_QUARKS 8
+
+typedef struct nlo_dis_desc {
+    double Q2, xbj;
+    int32_t transverse;  /* 1: T, 0: L */
+    int32_t n_quarks;
+    double mass[NLO_DIS_MAX_QUARKS];
+    double charge[NLO_DIS_MAX_QUARKS];
+    double maxr;                 /* DISConfig.maxr */
+    int32_t finite_nc;           /* NcScheme.FiniteNC */
+    int32_t parent_coupling;     /* RunningCouplingScheme.PARENT */
+    int32_t smooth_coupling;     /* RunningCouplingIRScheme.SMOOTH */
+    int32_t mass_dependent_x;    /* HeavyQuarkX.MassDependentX */
+    double c2_alpha, max_alpha;  /* C2_alpha, max_alpha_s_freeze */
+    int32_t nf_alphas;           /* active flavours of the coupling */
+    int32_t nested_points;       /* inner Gauss-Legendre points of the nested I2/I3 (8..128) */
+    nlo_dis_dipole dipole;
+} nlo_dis_desc;
+
+/* Unit-cube dimension of a contribution, or -1. */
+int nlo_dis_dimensions(int32_t contribution);
+
+/* The integrand at n device points x (n x dim doubles, row-major) into d_out
+ * (n doubles).  Replaces the reference's integrand callables (dis_lo_integrand,
+ * dis_dipole_integrand, the I1 / nested I2-I3 kernels in K/dis/). */
+int nlo_dis_integrand(const nlo_dis_desc* desc, int32_t contribution, int64_t n, const double* d_x,
+                      int32_t precision, double* d_out, hipStream_t stream);
+
+/* One VEGAS pass: n scrambled-Sobol points (indices first_index ..
+ * first_index + n - 1 of `sobol`, dimension = the contribution's) mapped
+ * through d_edges (dim x (bins + 1), on [0, 1]), the integrand evaluated and
+ * w = f x Jacobian summed in FP64 with a fixed reduction order: d_sums[0] =
+ * sum w, d_sums[1] = sum w^2.  d_hist / d_count (dim x bins, may both be NULL)
+ * receive sum |w| and the sample count per bin.  first_index % 64 == 0.
+ * Replaces the sampling, integrand and histogram steps of vegas()
+ * (S/numerics/integration.py:237-499) for the DIS integrands. */
+size_t nlo_dis_vegas_workspace(int32_t contribution, int32_t bins, int64_t n);
+int nlo_dis_vegas(const nlo_dis_desc* desc, int32_t contribution, int32_t bins, const double* d_edges,
+                  const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision,
+                  double* d_sums, double* d_hist, double* d_count, void* d_workspace,
+                  size_t workspace_bytes, hipStream_t stream);
+
+/* Tensor Gauss-Legendre on the unit square for a two-dimensional
+ * contribution: d_out[0] with `points` nodes per axis, d_out[1] with
+ * max(2, points / 2) (tensor_gauss_legendre, S/numerics/integration.py:95-117). */
+int nlo_dis_gauss_legendre(const nlo_dis_desc* desc, int32_t contribution, int32_t points,
+                           int32_t precision, double* d_out, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

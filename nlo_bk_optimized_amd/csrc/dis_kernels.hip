@@ -1,0 +1,825 @@
+// NLO DIS kernels and their C ABI: special functions, dipole amplitudes, the
+// BKDipole initialisation, per-point integrands, one VEGAS pass with in-kernel
+// scrambled Sobol points and deterministic FP64 sums, and the 2-D tensor
+// Gauss-Legendre rule.  Reference prefixes: S/ = torch_nlo_clean/src/nlo_torch/,
+// K/ = S/custom_kernels/.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "dis_device.hpp"
+#include "nlo_gfx950.h"
+
+namespace nlo {
+namespace dis {
+namespace {
+
+constexpr double kZmin = 1e-6;       // ZMIN (S/dis/observables.py:46)
+constexpr double kCF = 4.0 / 3.0;    // (Nc^2 - 1) / (2 Nc), Nc = 3
+constexpr double kNc = 3.0;
+constexpr double kQ0sqr = 1.0;       // Q0sqr (S/dis/config.py:13)
+constexpr int kMaxNested = 128;
+
+constexpr int kDims[NLO_DIS_KINDS] = {2, 2, 3, 4, 2, 3, 4, 5, 6, 7, 5, 5};
+
+struct NestedRule {
+    int n;
+    double y[kMaxNested];
+    double w[kMaxNested];
+};
+
+// Everything an integrand reads, by value in the kernel arguments.
+struct DisArgs {
+    double Q2, xbj;
+    int transverse, nq;
+    double mass[NLO_DIS_MAX_QUARKS], charge[NLO_DIS_MAX_QUARKS];
+    double maxr;
+    int finite_nc, parent_coupling;
+    DisCoupling coup;
+    double y_lo;    // LO rapidity max(ln(X0 / xbj), 0)
+    double y_dip;   // dipole-sector rapidity of quark 0 (S/dis/observables.py:55-68)
+    double z2min;   // Q0^2 / W^2 (S/dis/observables.py:71-73)
+    double min_r, max_r;  // the dipole's table range (inside_table, :145-147)
+    DisDipole dip;
+    NestedRule rule;
+};
+
+// ------------------------------------------------------------- integrands
+
+template <typename T>
+__device__ __forceinline__ T amp(const DisArgs& a, T r, T Y) {
+    return dipole_amplitude<T>(r, Y, a.dip);
+}
+
+// S/dis/observables.py:86-104
+template <typename T>
+__device__ __forceinline__ T tripole_amp(const DisArgs& a, T x01, T x02, T x21, T Y) {
+    const T S01 = T(1) - amp(a, x01, Y), S02 = T(1) - amp(a, x02, Y), S12 = T(1) - amp(a, x21, Y);
+    if (a.finite_nc) return T(1) - T(kNc / (2.0 * kCF)) * (S02 * S12 - S01 / T(kNc * kNc));
+    return T(1) - S02 * S12;
+}
+
+template <typename T, int K>
+__device__ T dis_value(const DisArgs& a, const double* u) {
+    const T Q2 = (T)a.Q2;
+    if constexpr (K == NLO_DIS_LO) {
+        // S/dis/observables.py:345-352 and S/dis/lo.py:12-51
+        const T z = (T)u[0];
+        const T r = (T)(a.maxr * u[1]);
+        T tot = T(0);
+        for (int q = 0; q < a.nq; ++q)
+            tot += lo_wavefunction<T>(r, z, Q2, (T)a.mass[q], (T)a.charge[q], a.transverse != 0);
+        return T(2.0 * kPiD) * r * (T)a.maxr * tot * amp(a, r, (T)a.y_lo);
+    } else if constexpr (K <= NLO_DIS_DIP_L_CD) {
+        // integrand_dip_massive (S/dis/observables.py:107-172)
+        const T z1 = (T)(kZmin + u[0] * (1.0 - 2.0 * kZmin));
+        const double x01d = a.maxr * u[1];
+        const bool inside = x01d >= a.min_r && x01d <= a.max_r;
+        const T x01 = (T)x01d;
+        const T x01e = (T)fmin(fmax(x01d, a.min_r), a.max_r);
+        const T mf = (T)a.mass[0];
+        const T N01 = amp(a, x01e, (T)a.y_dip);
+        T res;
+        if constexpr (K == NLO_DIS_DIP_T0) res = it_dip_0<T>(Q2, z1, x01e, mf);
+        else if constexpr (K == NLO_DIS_DIP_T1) res = it_dip_1<T>(Q2, z1, x01e, mf, (T)u[2]);
+        else if constexpr (K == NLO_DIS_DIP_T2) res = it_dip_2<T>(Q2, z1, x01e, mf, (T)u[2], (T)u[3]);
+        else if constexpr (K == NLO_DIS_DIP_L_CONST) res = il_dip_const<T>(Q2, z1, x01e, mf);
+        else if constexpr (K == NLO_DIS_DIP_L_AB) res = il_dip_ab<T>(Q2, z1, x01e, mf, (T)u[2]);
+        else res = il_dip_cd<T>(Q2, z1, x01e, mf, (T)u[2], (T)u[3]);
+        res = N01 * res * (x01 * (T)a.maxr * T(1.0 - 2.0 * kZmin)) * (dis_alpha<T>(x01e, a.coup) * T(kCF / kPiD));
+        return (inside && isfinite(res)) ? res : T(0);
+    } else {
+        // integrand_qgunsub_massive (S/dis/observables.py:175-250) and the nested
+        // form (:253-331)
+        if (a.z2min > 1.0) return T(0);
+        const T z2min = (T)a.z2min;
+        const T z1 = (T(1) - z2min) * (T)u[0];
+        const T z2 = (T(1) - z1 - z2min) * (T)u[1] + z2min;
+        const T x01 = (T)(a.maxr * u[2]), x02 = (T)(a.maxr * u[3]);
+        const T phi = (T)(2.0 * kPiD * u[4]);
+        const T x01sq = x01 * x01, x02sq = x02 * x02;
+        const T x21sq = fmax(x01sq + x02sq - T(2) * x01 * x02 * cos(phi), T(0));
+        const T x21 = sqrt(x21sq);
+        const T maxr = (T)a.maxr;
+        const T jac = (T(1) - z2min) * (T(1) - z1 - z2min) * x01 * x02 * maxr * maxr * T(2.0 * kPiD);
+        const T Y = log((Q2 / (T)a.xbj) * z2 / T(kQ0sqr));
+        const T Ye = fmax(Y, T(0));
+        const T mf = (T)a.mass[0], m2 = mf * mf;
+        const T N012 = tripole_amp<T>(a, x01, x02, x21, Ye);
+        const T ra = a.parent_coupling ? x01 : fmin(x01, fmin(x02, x21));
+        const T af = dis_alpha<T>(ra, a.coup) * T(kCF / kPiD);
+        const Tripole<T> v = tripole_vars<T>(Q2, mf, z1, z2, x01sq, x02sq, x21sq);
+        T res;
+        if constexpr (K == NLO_DIS_QG_I1) {
+            const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
+            const T N01 = amp(a, x01, Ye);
+            if (a.transverse)
+                res = N01 * it_qg_uvsub<T>(v, m2, z1, z2, x01sq) + N012 * it_qg_i1<T>(v, f, m2, z1, z2);
+            else
+                res = N01 * il_qg_uvsub<T>(v, Q2, z1, z2, x01sq) + N012 * il_qg_i1<T>(v, f, Q2, z1, z2);
+        } else if constexpr (K == NLO_DIS_QG_I2 || K == NLO_DIS_QG_I3) {
+            const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
+            const TMoments<T> m1 = t_moments<T>(v, m2, (T)u[5]);
+            if constexpr (K == NLO_DIS_QG_I2) {
+                res = N012 * (a.transverse ? it_qg_i2<T>(v, f, m2, z1, z2, m1) : il_qg_i2<T>(v, f, Q2, z1, z2, m1));
+            } else {
+                const TMoments<T> mm2 = t_moments<T>(v, m2, (T)u[6]);
+                res = N012 * (a.transverse ? it_qg_i3<T>(v, f, m2, z1, z2, m1, mm2)
+                                           : il_qg_i3<T>(v, Q2, m2, z1, z2, m1, mm2));
+            }
+        } else {
+            // y_t integrals on the mapped inner rule: I2 is linear and I3
+            // bilinear in the G values (K/dis/i2_i3/kernel.cu:768-912)
+            const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
+            TMoments<T> acc{T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
+            for (int i = 0; i < a.rule.n; ++i) t_moments_axpy(acc, (T)a.rule.w[i], t_moments<T>(v, m2, (T)a.rule.y[i]));
+            if constexpr (K == NLO_DIS_QG_I2_NESTED)
+                res = N012 * (a.transverse ? it_qg_i2<T>(v, f, m2, z1, z2, acc) : il_qg_i2<T>(v, f, Q2, z1, z2, acc));
+            else
+                res = N012 * (a.transverse ? it_qg_i3<T>(v, f, m2, z1, z2, acc, acc)
+                                           : il_qg_i3<T>(v, Q2, m2, z1, z2, acc, acc));
+        }
+        res = res * jac * af / z2;
+        return (Y >= T(0) && isfinite(res)) ? res : T(0);
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+
+template <typename T>
+__global__ void __launch_bounds__(256) special_kernel(int fn, int64_t n, const double* __restrict__ x,
+                                                      double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T v = (T)x[i];
+    T r;
+    if (fn == NLO_FN_K0) r = bessel_k0<T>(v);
+    else if (fn == NLO_FN_K1) r = bessel_k1<T>(v);
+    else if (fn == NLO_FN_K2) r = bessel_k2<T>(v);
+    else r = dilog<T>(v);
+    out[i] = (double)r;
+}
+
+__global__ void __launch_bounds__(256) dipole_kernel(DisDipole d, int64_t n, const double* __restrict__ r,
+                                                     const double* __restrict__ Y, double* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = dipole_amplitude<double>(r[i], Y[i], d);
+}
+
+// One block (64 lanes): N at the (moved) table nodes through the local-spline
+// path, a natural spline in r through them, then the maxr_interpolate scan of
+// BKDipole._find_maxr_interpolate on that spline (lane 0).
+__global__ void __launch_bounds__(64) bk_init_kernel(DisDipole d, double Y, double* __restrict__ coef,
+                                                     double* __restrict__ maxr_out) {
+    extern __shared__ double sh[];
+    const int R = d.R;
+    double* yv = sh;          // R
+    double* m = sh + R;       // R second derivatives
+    double* cp = sh + 2 * R;  // R
+    d.initialized = 0;
+    for (int i = threadIdx.x; i < R; i += blockDim.x) {
+        double rr = d.r[i];
+        if (i == 0) rr *= 1.0001;
+        if (i == R - 1) rr *= 0.9999;
+        yv[i] = bk_amplitude(rr, Y, d);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // natural spline through (r_i, yv_i): Thomas on the interior second derivatives
+        const int ni = R - 2;
+        double dprev = 0.0;
+        for (int i = 0; i < ni; ++i) {
+            const double h0 = d.r[i + 1] - d.r[i], h1 = d.r[i + 2] - d.r[i + 1];
+            const double rhs = 6.0 * ((yv[i + 2] - yv[i + 1]) / h1 - (yv[i + 1] - yv[i]) / h0);
+            const double lower = i > 0 ? h0 : 0.0;
+            const double den = 2.0 * (h0 + h1) - lower * (i > 0 ? cp[i - 1] : 0.0);
+            cp[i] = h1 / den;
+            dprev = (rhs - lower * dprev) / den;
+            m[i + 1] = dprev;  // forward value, back-substituted below
+        }
+        m[0] = 0.0;
+        m[R - 1] = 0.0;
+        for (int i = ni - 2; i >= 0; --i) m[i + 1] = m[i + 1] - cp[i] * m[i + 2];
+        for (int k = 0; k < R - 1; ++k) {
+            const double h = d.r[k + 1] - d.r[k];
+            coef[4 * k + 0] = yv[k];
+            coef[4 * k + 1] = (yv[k + 1] - yv[k]) / h - h * (2.0 * m[k] + m[k + 1]) / 6.0;
+            coef[4 * k + 2] = 0.5 * m[k];
+            coef[4 * k + 3] = (m[k + 1] - m[k]) / (6.0 * h);
+        }
+        __threadfence_block();
+        // _find_maxr_interpolate (S/dipole/amplitude.py:242-260)
+        double step = 2.0, prev = 0.1, rr = 0.01, maxr = -1.0;
+        const double rmin = d.r[0], rmax = d.r[R - 1];
+        for (int it = 0; it < 41; ++it) {
+            if (rr > rmax) break;
+            double re = rr < rmin ? rmin * 1.000001 : (rr > rmax ? rmax * 0.999999 : rr);
+            int k = search_right(d.r, R, re);
+            k = k < 0 ? 0 : (k > R - 2 ? R - 2 : k);
+            const double t = re - d.r[k];
+            double v = coef[4 * k] + t * (coef[4 * k + 1] + t * (coef[4 * k + 2] + t * coef[4 * k + 3]));
+            v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+            if (v >= 0.99999) {
+                if (step < 1e-2) {
+                    maxr = rr;
+                    break;
+                }
+                step /= 1.5;
+                rr = prev;
+            }
+            prev = rr;
+            rr += step;
+        }
+        *maxr_out = maxr;
+    }
+}
+
+template <typename T, int K>
+__global__ void __launch_bounds__(256) integrand_kernel(DisArgs a, int64_t n, const double* __restrict__ x,
+                                                        double* __restrict__ out) {
+    constexpr int D = kDims[K];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = x[i * D + d];
+    out[i] = (double)dis_value<T, K>(a, u);
+}
+
+struct VegasArgs {
+    int bins, spt, blocks;
+    int64_t n, first;
+    const double* edges;  // D x (bins + 1)
+    double* part;         // blocks x 2
+    double* part_hist;    // blocks x D x bins
+    double* part_cnt;
+    uint32_t dir[NLO_SOBOL_MAXDIM][NLO_SOBOL_BITS];
+    uint32_t shift[NLO_SOBOL_MAXDIM];
+    uint32_t first_q[NLO_SOBOL_MAXDIM];
+};
+
+// LDS of the VEGAS kernel: edges, directions, and (with the histogram) the
+// staged |w| and packed bins of the block's samples plus the fixed-point sums.
+struct VegasLayout {
+    size_t edges, dir, absw, pbin, hist, cnt, total;
+};
+
+__host__ __device__ inline VegasLayout vegas_layout(int D, int bins, int spt, bool hist) {
+    VegasLayout L{};
+    size_t o = 0;
+    L.edges = o;
+    o = align_up(o + (size_t)D * (bins + 1) * sizeof(double), 16);
+    L.dir = o;
+    o = align_up(o + (size_t)D * NLO_SOBOL_BITS * sizeof(uint32_t), 16);
+    if (hist) {
+        L.absw = o;
+        o = align_up(o + (size_t)256 * spt * sizeof(double), 16);
+        L.pbin = o;
+        o = align_up(o + (size_t)256 * spt * sizeof(uint64_t), 16);
+        L.hist = o;
+        o = align_up(o + (size_t)D * bins * sizeof(unsigned long long), 16);
+        L.cnt = o;
+        o = align_up(o + (size_t)D * bins * sizeof(unsigned int), 16);
+    }
+    L.total = o;
+    return L;
+}
+
+// One VEGAS pass (see nlo_dis_vegas).  Sobol point index = first + s with
+// s = ((blk * 4 + wave) * spt + j) * 64 + lane: each wave walks spt
+// consecutive 64-point groups; the high part of the gray code (direction
+// bits 6..29) is wave-uniform and updated by one XOR per group.
+template <typename T, int K, bool HIST>
+__global__ void __launch_bounds__(256) vegas_kernel(DisArgs a, VegasArgs v) {
+    constexpr int D = kDims[K];
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[4][2];
+    __shared__ double wmax[4];
+    const VegasLayout lay = vegas_layout(D, v.bins, v.spt, HIST);
+    double* E = reinterpret_cast<double*>(smem + lay.edges);
+    uint32_t* dir = reinterpret_cast<uint32_t*>(smem + lay.dir);
+    for (int i = threadIdx.x; i < D * (v.bins + 1); i += blockDim.x) E[i] = v.edges[i];
+    for (int i = threadIdx.x; i < D * NLO_SOBOL_BITS; i += blockDim.x) dir[i] = v.dir[i / NLO_SOBOL_BITS][i % NLO_SOBOL_BITS];
+    unsigned long long* hh = nullptr;
+    unsigned int* hc = nullptr;
+    double* st_w = nullptr;
+    uint64_t* st_b = nullptr;
+    if (HIST) {
+        hh = reinterpret_cast<unsigned long long*>(smem + lay.hist);
+        hc = reinterpret_cast<unsigned int*>(smem + lay.cnt);
+        st_w = reinterpret_cast<double*>(smem + lay.absw);
+        st_b = reinterpret_cast<uint64_t*>(smem + lay.pbin);
+        for (int k = threadIdx.x; k < D * v.bins; k += blockDim.x) {
+            hh[k] = 0ull;
+            hc[k] = 0u;
+        }
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = uniform(threadIdx.x >> 6);
+    const uint32_t glo0 = (uint32_t)(lane ^ (lane >> 1));
+    uint32_t vlo[D][2];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        uint32_t v0 = 0;
+        for (int b = 0; b < 6; ++b)
+            if ((glo0 >> b) & 1u) v0 ^= dir[d * NLO_SOBOL_BITS + b];
+        vlo[d][0] = v0 ^ v.shift[d];
+        vlo[d][1] = v0 ^ dir[d * NLO_SOBOL_BITS + 5] ^ v.shift[d];
+    }
+    const int64_t hi0 = (v.first >> 6) + ((int64_t)blockIdx.x * 4 + wave) * v.spt;
+    uint32_t vhi[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) vhi[d] = 0u;
+    {
+        uint64_t g = (uint64_t)hi0 ^ ((uint64_t)hi0 >> 1);
+        for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
+            if (g & 1u)
+#pragma unroll
+                for (int d = 0; d < D; ++d) vhi[d] ^= dir[d * NLO_SOBOL_BITS + b];
+    }
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < v.spt; ++j) {
+        const int64_t hi = hi0 + j;
+        if (j > 0) {
+            const int b = 6 + __builtin_ctzll((unsigned long long)hi);
+            if (b < NLO_SOBOL_BITS)
+#pragma unroll
+                for (int d = 0; d < D; ++d) vhi[d] ^= dir[d * NLO_SOBOL_BITS + b];
+        }
+        const int64_t sample = (hi << 6) + lane - v.first;
+        const bool valid = sample < v.n;
+        const int par = (int)(hi & 1);
+        double u[D];
+        double jac = 1.0;
+        uint64_t packed = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            uint32_t q = vhi[d] ^ vlo[d][par];
+            if (hi == 0 && lane == 0) q = v.first_q[d];
+            // bin = min(floor(u B), B - 1), frac = u B - bin, u = q 2^-30
+            // (map_importance_grid, S/dis/fixed_sobol.py:100-113)
+            const uint32_t q4 = q << 2;
+            const bool top = q >= (1u << 30);
+            const uint32_t b = top ? (uint32_t)(v.bins - 1) : __umulhi(q4, (uint32_t)v.bins);
+            const double frac = top ? 1.0 : (double)(q4 * (uint32_t)v.bins) * 0x1p-32;
+            const double left = E[d * (v.bins + 1) + b], right = E[d * (v.bins + 1) + b + 1];
+            const double w = right - left;
+            u[d] = left + w * frac;
+            jac = jac * ((double)v.bins * w);
+            packed |= (uint64_t)b << (8 * d);
+        }
+        double wv = 0.0;
+        if (valid) {
+            wv = (double)dis_value<T, K>(a, u) * jac;
+            s1 += wv;
+            s2 += wv * wv;
+        }
+        if (HIST) {
+            st_w[threadIdx.x + j * 256] = valid ? fabs(wv) : 0.0;
+            st_b[threadIdx.x + j * 256] = valid ? packed : ~0ull;
+        }
+    }
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+        red[wave][0] = s1;
+        red[wave][1] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
+        v.part[(size_t)blockIdx.x * 2 + threadIdx.x] = t;
+    }
+    if (HIST) {
+        // block max of |w| sets a fixed-point scale; the integer LDS sums are
+        // exact and independent of the order of the atomics
+        double mx = 0.0;
+        for (int j = 0; j < v.spt; ++j) mx = fmax(mx, st_w[threadIdx.x + j * 256]);
+        mx = wave_max(mx);
+        if (lane == 0) wmax[wave] = mx;
+        __syncthreads();
+        mx = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+        const int e = mx > 0.0 ? ilogb(mx) : 0;
+        const int sc = 50 - e;  // each term < 2^51; <= 2^12 terms per bin stay below 2^63
+        for (int j = 0; j < v.spt; ++j) {
+            const uint64_t pk = st_b[threadIdx.x + j * 256];
+            if (pk == ~0ull) continue;
+            const unsigned long long q = (unsigned long long)rint(ldexp(st_w[threadIdx.x + j * 256], sc));
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const int b = (int)((pk >> (8 * d)) & 0xff);
+                atomicAdd(&hh[d * v.bins + b], q);
+                atomicAdd(&hc[d * v.bins + b], 1u);
+            }
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < D * v.bins; k += blockDim.x) {
+            v.part_hist[(size_t)blockIdx.x * D * v.bins + k] = ldexp((double)hh[k], -sc);
+            v.part_cnt[(size_t)blockIdx.x * D * v.bins + k] = (double)hc[k];
+        }
+    }
+}
+
+// Fixed-order sums over the per-block partials: sums (2) by strided thread
+// sums, a lane butterfly and a wave sum; each histogram cell by one thread in
+// block order.
+__global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells, const double* __restrict__ part,
+                                                           const double* __restrict__ ph,
+                                                           const double* __restrict__ pc, double* __restrict__ sums,
+                                                           double* __restrict__ hist, double* __restrict__ cnt) {
+    __shared__ double red[4][2];
+    if (blockIdx.x == 0) {
+        double t0 = 0.0, t1 = 0.0;
+        for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
+            t0 += part[2 * (size_t)b];
+            t1 += part[2 * (size_t)b + 1];
+        }
+        t0 = wave_sum(t0);
+        t1 = wave_sum(t1);
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        if (lane == 0) {
+            red[wave][0] = t0;
+            red[wave][1] = t1;
+        }
+        __syncthreads();
+        if (threadIdx.x < 2) {
+            double t = 0.0;
+            for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
+            sums[threadIdx.x] = t;
+        }
+        return;
+    }
+    if (hist == nullptr) return;
+    const int k = (blockIdx.x - 1) * blockDim.x + threadIdx.x;
+    if (k >= cells) return;
+    double h = 0.0, c = 0.0;
+    for (int b = 0; b < blocks; ++b) {
+        h += ph[(size_t)b * cells + k];
+        c += pc[(size_t)b * cells + k];
+    }
+    hist[k] = h;
+    cnt[k] = c;
+}
+
+struct GLRule {
+    int n;
+    double x[128];
+    double w[128];
+};
+
+// Tensor Gauss-Legendre over the unit square, fine rule in block 0 and coarse
+// rule in block 1; FP64 sums in a fixed order.
+template <typename T, int K>
+__global__ void __launch_bounds__(256) gl_kernel(DisArgs a, GLRule fine, GLRule coarse, double* __restrict__ out) {
+    __shared__ double red[4];
+    const GLRule& g = blockIdx.x == 0 ? fine : coarse;
+    double s = 0.0;
+    for (int idx = threadIdx.x; idx < g.n * g.n; idx += blockDim.x) {
+        const int i = idx / g.n, j = idx - i * g.n;
+        const double u[2] = {g.x[i], g.x[j]};
+        s += g.w[i] * g.w[j] * (double)dis_value<T, K>(a, u);
+    }
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// -------------------------------------------------------------------- host
+
+DisDipole make_dipole(const nlo_dis_dipole& d) {
+    DisDipole o{};
+    o.kind = d.kind;
+    o.qs0sqr = d.qs0sqr;
+    o.lambda = d.lambda;
+    o.gamma = d.gamma;
+    o.y0_gbw = std::log(1.0 / d.x0);
+    o.R = d.R;
+    o.T = d.T;
+    o.r = d.d_r;
+    o.Y = d.d_Y;
+    o.N = d.d_N;
+    o.initialized = d.initialized;
+    o.y_init = d.y_init;
+    o.maxr_interpolate = d.maxr_interpolate;
+    o.init_coef = d.d_init_coef;
+    return o;
+}
+
+int check_dipole(const nlo_dis_dipole* d) {
+    NLO_REQUIRE(d != nullptr, "dipole is NULL");
+    if (d->kind == NLO_DIPOLE_GBW) {
+        NLO_REQUIRE(d->x0 > 0.0 && d->qs0sqr > 0.0, "GBW needs x0 > 0 and Qs0sqr > 0");
+        return NLO_OK;
+    }
+    NLO_REQUIRE(d->kind == NLO_DIPOLE_BK_TABLE, "unknown dipole kind %d", d->kind);
+    NLO_REQUIRE(d->R >= 9 && d->T >= 1, "BK table needs R >= 9 r points and T >= 1 rapidities (R=%d, T=%d)",
+                d->R, d->T);
+    NLO_REQUIRE(d->d_r && d->d_Y && d->d_N, "BK table arrays are NULL");
+    NLO_REQUIRE(!d->initialized || d->d_init_coef, "initialised BK dipole without coefficients");
+    return NLO_OK;
+}
+
+int make_args(const nlo_dis_desc* desc, DisArgs& a) {
+    NLO_REQUIRE(desc != nullptr, "desc is NULL");
+    int rc = check_dipole(&desc->dipole);
+    if (rc != NLO_OK) return rc;
+    NLO_REQUIRE(desc->Q2 > 0.0 && desc->xbj > 0.0 && desc->xbj <= 1.0, "Q2 must be positive and xbj in (0, 1]");
+    NLO_REQUIRE(desc->n_quarks >= 1 && desc->n_quarks <= NLO_DIS_MAX_QUARKS, "n_quarks must be 1..%d",
+                NLO_DIS_MAX_QUARKS);
+    NLO_REQUIRE(desc->maxr > 0.0 && desc->c2_alpha > 0.0 && desc->max_alpha > 0.0,
+                "maxr, C2_alpha and max_alpha_s_freeze must be positive");
+    a = DisArgs{};
+    a.Q2 = desc->Q2;
+    a.xbj = desc->xbj;
+    a.transverse = desc->transverse != 0;
+    a.nq = desc->n_quarks;
+    for (int q = 0; q < a.nq; ++q) {
+        a.mass[q] = desc->mass[q];
+        a.charge[q] = desc->charge[q];
+    }
+    a.maxr = desc->maxr;
+    a.finite_nc = desc->finite_nc != 0;
+    a.parent_coupling = desc->parent_coupling != 0;
+    a.coup.b0 = (11.0 * kNc - 2.0 * desc->nf_alphas) / (12.0 * kPiD);
+    a.coup.scale_num = 4.0 * desc->c2_alpha / (0.241 * 0.241);  // LambdaQCD (S/dis/config.py:9)
+    a.coup.max_alpha = desc->max_alpha;
+    a.coup.smooth = desc->smooth_coupling != 0;
+    a.dip = make_dipole(desc->dipole);
+    const double x0 = desc->dipole.kind == NLO_DIPOLE_GBW ? desc->dipole.x0 : 1.0;  // dipole.X0()
+    a.y_lo = std::fmax(std::log(x0 / desc->xbj), 0.0);
+    {
+        // evolution_rapidity_dipole (S/dis/observables.py:55-68)
+        const double W2 = (1.0 - desc->xbj) * desc->Q2 / desc->xbj;
+        const double mf = desc->mass[0];
+        const double X = desc->mass_dependent_x ? (desc->Q2 + 4.0 * mf * mf) / (desc->Q2 + W2) : desc->Q2 / (desc->Q2 + W2);
+        a.y_dip = std::log(1.0 / X);
+    }
+    a.z2min = kQ0sqr / (desc->Q2 / desc->xbj);
+    if (desc->dipole.kind == NLO_DIPOLE_GBW) {
+        a.min_r = 1e-30;
+        a.max_r = 1e3;
+    } else {
+        // r_min / r_max are device values: the table's first and last r
+        double ends[2];
+        NLO_HIP_CHECK(hipMemcpy(&ends[0], desc->dipole.d_r, sizeof(double), hipMemcpyDeviceToHost));
+        NLO_HIP_CHECK(hipMemcpy(&ends[1], desc->dipole.d_r + desc->dipole.R - 1, sizeof(double), hipMemcpyDeviceToHost));
+        a.min_r = ends[0];
+        a.max_r = ends[1];
+    }
+    const int np = desc->nested_points;
+    NLO_REQUIRE(np >= 8 && np <= kMaxNested, "nested_points must be 8..%d", kMaxNested);
+    std::vector<double> x(np), w(np);
+    int grc = nlo_gauss_legendre(np, x.data(), w.data());
+    if (grc != NLO_OK) return grc;
+    a.rule.n = np;
+    for (int i = 0; i < np; ++i) {
+        // y = t^3, t = (x + 1)/2, weight w/2 * 3 t^2 (K/dis/i2_i3/extension.py:21-36)
+        const double t = 0.5 * (x[i] + 1.0);
+        a.rule.y[i] = t * t * t;
+        a.rule.w[i] = 0.5 * w[i] * 3.0 * t * t;
+    }
+    return NLO_OK;
+}
+
+#define DIS_DISPATCH(MACRO)           \
+    switch (contribution) {           \
+        MACRO(NLO_DIS_LO)             \
+        MACRO(NLO_DIS_DIP_T0)         \
+        MACRO(NLO_DIS_DIP_T1)         \
+        MACRO(NLO_DIS_DIP_T2)         \
+        MACRO(NLO_DIS_DIP_L_CONST)    \
+        MACRO(NLO_DIS_DIP_L_AB)       \
+        MACRO(NLO_DIS_DIP_L_CD)       \
+        MACRO(NLO_DIS_QG_I1)          \
+        MACRO(NLO_DIS_QG_I2)          \
+        MACRO(NLO_DIS_QG_I3)          \
+        MACRO(NLO_DIS_QG_I2_NESTED)   \
+        MACRO(NLO_DIS_QG_I3_NESTED)   \
+        default:                      \
+            break;                    \
+    }
+
+struct VegasPlan {
+    int spt, blocks;
+    size_t part, hist, workspace, lds;
+};
+
+VegasPlan vegas_plan(int contribution, int bins, int64_t n) {
+    const int D = kDims[contribution];
+    VegasPlan p{};
+    const int64_t groups = (n + 63) / 64;
+    int spt = 8;
+    while (spt > 1 && (groups + 4 * spt - 1) / (4 * spt) < 2048) spt >>= 1;
+    p.spt = spt;
+    p.blocks = (int)((groups + 4 * spt - 1) / (4 * spt));
+    p.part = align_up((size_t)p.blocks * 2 * sizeof(double), 256);
+    p.hist = align_up((size_t)p.blocks * D * bins * sizeof(double), 256);
+    p.workspace = p.part + 2 * p.hist;
+    p.lds = vegas_layout(D, bins, spt, true).total;
+    return p;
+}
+
+}  // namespace
+}  // namespace dis
+}  // namespace nlo
+
+using namespace nlo;
+using namespace nlo::dis;
+
+extern "C" {
+
+int nlo_dis_dimensions(int32_t contribution) {
+    return (contribution >= 0 && contribution < NLO_DIS_KINDS) ? kDims[contribution] : -1;
+}
+
+int nlo_dis_special(int32_t fn, int64_t n, const double* d_x, int32_t precision, double* d_out,
+                    hipStream_t stream) {
+    clear_error();
+    NLO_REQUIRE(fn >= NLO_FN_K0 && fn <= NLO_FN_LI2, "unknown special function %d", fn);
+    NLO_REQUIRE(n >= 0, "n must be >= 0");
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    if (n == 0) return NLO_OK;
+    NLO_REQUIRE(d_x && d_out, "NULL array");
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (precision == NLO_PREC_FP64)
+        hipLaunchKernelGGL(special_kernel<double>, grid, dim3(256), 0, stream, fn, n, d_x, d_out);
+    else
+        hipLaunchKernelGGL(special_kernel<float>, grid, dim3(256), 0, stream, fn, n, d_x, d_out);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int nlo_dis_dipole_amplitude(const nlo_dis_dipole* dipole, int64_t n, const double* d_r, const double* d_Y,
+                             double* d_out, hipStream_t stream) {
+    clear_error();
+    int rc = check_dipole(dipole);
+    if (rc != NLO_OK) return rc;
+    NLO_REQUIRE(n >= 0, "n must be >= 0");
+    if (n == 0) return NLO_OK;
+    NLO_REQUIRE(d_r && d_Y && d_out, "NULL array");
+    hipLaunchKernelGGL(dipole_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, make_dipole(*dipole),
+                       n, d_r, d_Y, d_out);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int nlo_dis_bk_initialize(const nlo_dis_dipole* dipole, double Y, double* d_coef, double* d_maxr,
+                          hipStream_t stream) {
+    clear_error();
+    int rc = check_dipole(dipole);
+    if (rc != NLO_OK) return rc;
+    NLO_REQUIRE(dipole->kind == NLO_DIPOLE_BK_TABLE, "nlo_dis_bk_initialize needs a BK table dipole");
+    NLO_REQUIRE(Y >= 0.0, "cannot initialise the interpolation at negative Y=%g", Y);
+    NLO_REQUIRE(d_coef && d_maxr, "NULL output");
+    const size_t lds = (size_t)3 * dipole->R * sizeof(double);
+    hipLaunchKernelGGL(bk_init_kernel, dim3(1), dim3(64), lds, stream, make_dipole(*dipole), Y, d_coef, d_maxr);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int nlo_dis_integrand(const nlo_dis_desc* desc, int32_t contribution, int64_t n, const double* d_x,
+                      int32_t precision, double* d_out, hipStream_t stream) {
+    clear_error();
+    NLO_REQUIRE(contribution >= 0 && contribution < NLO_DIS_KINDS, "unknown contribution %d", contribution);
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    NLO_REQUIRE(n >= 0, "n must be >= 0");
+    DisArgs a;
+    int rc = make_args(desc, a);
+    if (rc != NLO_OK) return rc;
+    if (n == 0) return NLO_OK;
+    NLO_REQUIRE(d_x && d_out, "NULL array");
+    const dim3 grid((unsigned)((n + 255) / 256));
+#define DIS_POINT(K)                                                                                     \
+    case K:                                                                                              \
+        if (precision == NLO_PREC_FP64)                                                                  \
+            hipLaunchKernelGGL((integrand_kernel<double, K>), grid, dim3(256), 0, stream, a, n, d_x, d_out); \
+        else                                                                                             \
+            hipLaunchKernelGGL((integrand_kernel<float, K>), grid, dim3(256), 0, stream, a, n, d_x, d_out);  \
+        break;
+    DIS_DISPATCH(DIS_POINT)
+#undef DIS_POINT
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+size_t nlo_dis_vegas_workspace(int32_t contribution, int32_t bins, int64_t n) {
+    if (contribution < 0 || contribution >= NLO_DIS_KINDS || bins < 1 || n < 1) return 0;
+    return vegas_plan(contribution, bins, n).workspace;
+}
+
+int nlo_dis_vegas(const nlo_dis_desc* desc, int32_t contribution, int32_t bins, const double* d_edges,
+                  const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision, double* d_sums,
+                  double* d_hist, double* d_count, void* d_workspace, size_t workspace_bytes, hipStream_t stream) {
+    clear_error();
+    NLO_REQUIRE(contribution >= 0 && contribution < NLO_DIS_KINDS, "unknown contribution %d", contribution);
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    const int D = kDims[contribution];
+    NLO_REQUIRE(bins >= 2 && bins <= 255, "bins must be 2..255");
+    NLO_REQUIRE(n >= 1, "n must be >= 1");
+    NLO_REQUIRE(sobol && sobol->dim == D, "the Sobol table must have the contribution's dimension %d", D);
+    NLO_REQUIRE(first_index >= 0 && first_index % 64 == 0, "first_index must be a non-negative multiple of 64");
+    NLO_REQUIRE(first_index + n <= (int64_t)1 << NLO_SOBOL_BITS, "Sobol indices beyond 2^30");
+    NLO_REQUIRE(d_edges && d_sums, "NULL array");
+    NLO_REQUIRE((d_hist == nullptr) == (d_count == nullptr), "d_hist and d_count must both be given or both NULL");
+    DisArgs a;
+    int rc = make_args(desc, a);
+    if (rc != NLO_OK) return rc;
+    const VegasPlan pl = vegas_plan(contribution, bins, n);
+    if (workspace_bytes < pl.workspace) {
+        set_error("nlo_dis_vegas: workspace %zu bytes < %zu", workspace_bytes, pl.workspace);
+        return NLO_ERR_WORKSPACE;
+    }
+    NLO_REQUIRE(d_workspace != nullptr, "NULL workspace");
+    VegasArgs v{};
+    v.bins = bins;
+    v.spt = pl.spt;
+    v.blocks = pl.blocks;
+    v.n = n;
+    v.first = first_index;
+    v.edges = d_edges;
+    unsigned char* ws = static_cast<unsigned char*>(d_workspace);
+    v.part = reinterpret_cast<double*>(ws);
+    v.part_hist = reinterpret_cast<double*>(ws + pl.part);
+    v.part_cnt = reinterpret_cast<double*>(ws + pl.part + pl.hist);
+    for (int d = 0; d < D; ++d) {
+        for (int b = 0; b < NLO_SOBOL_BITS; ++b) v.dir[d][b] = sobol->direction[d][b];
+        v.shift[d] = sobol->shift[d];
+        v.first_q[d] = sobol->first[d];
+    }
+    const bool hist = d_hist != nullptr;
+    const size_t lds = vegas_layout(D, bins, pl.spt, hist).total;
+    NLO_REQUIRE(lds <= 64 * 1024, "LDS %zu bytes exceeds 64 KiB", lds);
+#define DIS_VEGAS(K)                                                                                            \
+    case K:                                                                                                     \
+        if (precision == NLO_PREC_FP64) {                                                                       \
+            if (hist) hipLaunchKernelGGL((vegas_kernel<double, K, true>), dim3(pl.blocks), dim3(256), lds, stream, a, v); \
+            else hipLaunchKernelGGL((vegas_kernel<double, K, false>), dim3(pl.blocks), dim3(256), lds, stream, a, v);     \
+        } else {                                                                                                \
+            if (hist) hipLaunchKernelGGL((vegas_kernel<float, K, true>), dim3(pl.blocks), dim3(256), lds, stream, a, v);  \
+            else hipLaunchKernelGGL((vegas_kernel<float, K, false>), dim3(pl.blocks), dim3(256), lds, stream, a, v);      \
+        }                                                                                                       \
+        break;
+    DIS_DISPATCH(DIS_VEGAS)
+#undef DIS_VEGAS
+    NLO_LAUNCH_CHECK();
+    const int cells = D * bins;
+    const int rblocks = 1 + (hist ? (cells + 255) / 256 : 0);
+    hipLaunchKernelGGL(vegas_reduce_kernel, dim3(rblocks), dim3(256), 0, stream, pl.blocks, cells, v.part,
+                       v.part_hist, v.part_cnt, d_sums, d_hist, d_count);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int nlo_dis_gauss_legendre(const nlo_dis_desc* desc, int32_t contribution, int32_t points, int32_t precision,
+                           double* d_out, hipStream_t stream) {
+    clear_error();
+    NLO_REQUIRE(contribution >= 0 && contribution < NLO_DIS_KINDS && kDims[contribution] == 2,
+                "Gauss-Legendre needs a two-dimensional contribution (got %d)", contribution);
+    NLO_REQUIRE(points >= 4 && points <= 128, "points must be 4..128");
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    NLO_REQUIRE(d_out != nullptr, "NULL output");
+    DisArgs a;
+    int rc = make_args(desc, a);
+    if (rc != NLO_OK) return rc;
+    GLRule fine{}, coarse{};
+    auto fill = [](GLRule& g, int n) -> int {
+        std::vector<double> x(n), w(n);
+        int r = nlo_gauss_legendre(n, x.data(), w.data());
+        if (r != NLO_OK) return r;
+        g.n = n;
+        for (int i = 0; i < n; ++i) {
+            g.x[i] = 0.5 * (x[i] + 1.0);
+            g.w[i] = 0.5 * w[i];
+        }
+        return NLO_OK;
+    };
+    rc = fill(fine, points);
+    if (rc == NLO_OK) rc = fill(coarse, points / 2 > 2 ? points / 2 : 2);
+    if (rc != NLO_OK) return rc;
+#define DIS_GL(K)                                                                                          \
+    case K:                                                                                                \
+        if (precision == NLO_PREC_FP64)                                                                    \
+            hipLaunchKernelGGL((gl_kernel<double, K>), dim3(2), dim3(256), 0, stream, a, fine, coarse, d_out); \
+        else                                                                                               \
+            hipLaunchKernelGGL((gl_kernel<float, K>), dim3(2), dim3(256), 0, stream, a, fine, coarse, d_out);  \
+        break;
+    switch (contribution) {
+        DIS_GL(NLO_DIS_LO)
+        DIS_GL(NLO_DIS_DIP_T0)
+        DIS_GL(NLO_DIS_DIP_L_CONST)
+        default:
+            break;
+    }
+#undef DIS_GL
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+}  // extern "C"
