@@ -1,5 +1,6 @@
 """MI355X (gfx950) implementation of the NLO BK right-hand side and rapidity
-driver, drop-in for the reference's BK entry points.
+driver and of the NLO DIS structure functions, drop-in for the reference's BK
+and DIS entry points.
 
 The numerics run in ``libnlo_gfx950.so`` (hand-written HIP kernels behind a C
 ABI, ``include/nlo_gfx950.h``); this package is the Python surface of the
@@ -19,11 +20,23 @@ from nlo_bk_optimized_amd.bk import (
     compute_dndy,
     solve_bk,
 )
-from nlo_bk_optimized_amd.dipole import DipoleTable, load_bk_table, save_bk_table
+from nlo_bk_optimized_amd.dipole import GBW, BKDipole, DipoleTable, load_bk_table, save_bk_table
+from nlo_bk_optimized_amd.dis import F2, FL, FT, DISConfig, DISOrder, Polarization, Quark, QuarkType, sigma_r
 
 __all__ = [
     "BKConfig",
+    "BKDipole",
     "BKOrder",
+    "DISConfig",
+    "DISOrder",
+    "F2",
+    "FL",
+    "FT",
+    "GBW",
+    "Polarization",
+    "Quark",
+    "QuarkType",
+    "sigma_r",
     "DNDYResult",
     "DipoleTable",
     "ICDataFile",

@@ -20,6 +20,15 @@ NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
 NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4 = 0, 1, 2, 3
 SOBOL_MAXDIM = 8
 SOBOL_BITS = 30
+NLO_FN_K0, NLO_FN_K1, NLO_FN_K2, NLO_FN_LI2 = 0, 1, 2, 3
+NLO_DIPOLE_GBW, NLO_DIPOLE_BK_TABLE = 0, 1
+DIS_KINDS = {
+    "LO": 0, "T0": 1, "T1": 2, "T2": 3, "Omega_L_const": 4, "ab": 5, "cd": 6,
+    "I1": 7, "I2": 8, "I3": 9, "I2_nested": 10, "I3_nested": 11,
+}
+DIS_DIMENSIONS = {"LO": 2, "T0": 2, "T1": 3, "T2": 4, "Omega_L_const": 2, "ab": 3, "cd": 4,
+                  "I1": 5, "I2": 6, "I3": 7, "I2_nested": 5, "I3_nested": 5}
+DIS_MAX_QUARKS = 8
 
 
 def spline_words(R: int) -> int:
@@ -65,6 +74,50 @@ class EvolverDesc(ctypes.Structure):
         ("nlo", ctypes.c_int32),
         ("physics", Physics),
         ("sobol", SobolTable),
+    ]
+
+
+class DisDipole(ctypes.Structure):
+    """nlo_dis_dipole."""
+
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("initialized", ctypes.c_int32),
+        ("qs0sqr", ctypes.c_double),
+        ("lambda_", ctypes.c_double),
+        ("gamma", ctypes.c_double),
+        ("x0", ctypes.c_double),
+        ("R", ctypes.c_int32),
+        ("T", ctypes.c_int32),
+        ("d_r", ctypes.c_void_p),
+        ("d_Y", ctypes.c_void_p),
+        ("d_N", ctypes.c_void_p),
+        ("y_init", ctypes.c_double),
+        ("maxr_interpolate", ctypes.c_double),
+        ("d_init_coef", ctypes.c_void_p),
+    ]
+
+
+class DisDesc(ctypes.Structure):
+    """nlo_dis_desc."""
+
+    _fields_ = [
+        ("Q2", ctypes.c_double),
+        ("xbj", ctypes.c_double),
+        ("transverse", ctypes.c_int32),
+        ("n_quarks", ctypes.c_int32),
+        ("mass", ctypes.c_double * DIS_MAX_QUARKS),
+        ("charge", ctypes.c_double * DIS_MAX_QUARKS),
+        ("maxr", ctypes.c_double),
+        ("finite_nc", ctypes.c_int32),
+        ("parent_coupling", ctypes.c_int32),
+        ("smooth_coupling", ctypes.c_int32),
+        ("mass_dependent_x", ctypes.c_int32),
+        ("c2_alpha", ctypes.c_double),
+        ("max_alpha", ctypes.c_double),
+        ("nf_alphas", ctypes.c_int32),
+        ("nested_points", ctypes.c_int32),
+        ("dipole", DisDipole),
     ]
 
 
@@ -123,6 +176,18 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_bk_evolver_k2kf": (_vp, [_vp]),
     "nlo_bk_evolver_step": (ctypes.c_int, [_vp, _vp]),
     "nlo_bk_evolver_destroy": (ctypes.c_int, [_vp]),
+    "nlo_dis_special": (ctypes.c_int, [_i32, _i64, _vp, _i32, _vp, _vp]),
+    "nlo_dis_dipole_amplitude": (ctypes.c_int, [ctypes.POINTER(DisDipole), _i64, _vp, _vp, _vp, _vp]),
+    "nlo_dis_bk_initialize": (ctypes.c_int, [ctypes.POINTER(DisDipole), _f64, _vp, _vp, _vp]),
+    "nlo_dis_dimensions": (ctypes.c_int, [_i32]),
+    "nlo_dis_integrand": (ctypes.c_int, [ctypes.POINTER(DisDesc), _i32, _i64, _vp, _i32, _vp, _vp]),
+    "nlo_dis_vegas_workspace": (_size, [_i32, _i32, _i64]),
+    "nlo_dis_vegas": (
+        ctypes.c_int,
+        [ctypes.POINTER(DisDesc), _i32, _i32, _vp, ctypes.POINTER(SobolTable), _i64, _i64, _i32, _vp, _vp,
+         _vp, _vp, _size, _vp],
+    ),
+    "nlo_dis_gauss_legendre": (ctypes.c_int, [ctypes.POINTER(DisDesc), _i32, _i32, _i32, _vp, _vp]),
 }
 
 _lib: ctypes.CDLL | None = None

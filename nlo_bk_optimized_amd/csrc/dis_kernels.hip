@@ -24,11 +24,55 @@ constexpr int kMaxNested = 128;
 
 constexpr int kDims[NLO_DIS_KINDS] = {2, 2, 3, 4, 2, 3, 4, 5, 6, 7, 5, 5};
 
-struct NestedRule {
+// Gauss-Legendre node i (ascending) and weight of the n-point rule on [-1, 1]:
+// Newton on the three-term recurrence from the Chebyshev-like initial guess.
+__device__ inline void gl_node(int n, int i, double& x, double& w) {
+    double z = -cos(kPiD * (i + 0.75) / (n + 0.5));
+    double p0 = 1.0, p1 = z, dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+        p0 = 1.0;
+        p1 = z;
+        for (int k = 2; k <= n; ++k) {
+            const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+            p0 = p1;
+            p1 = p2;
+        }
+        dp = n * (z * p1 - p0) / (z * z - 1.0);
+        const double dz = p1 / dp;
+        z -= dz;
+        if (fabs(dz) <= 1e-16) break;
+    }
+    p0 = 1.0;
+    p1 = z;
+    for (int k = 2; k <= n; ++k) {
+        const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+    }
+    dp = n * (z * p1 - p0) / (z * z - 1.0);
+    x = z;
+    w = 2.0 / ((1.0 - z * z) * dp * dp);
+}
+
+// The nested I2/I3 inner rule in LDS: y = t^3, t = (x + 1)/2, weight
+// w/2 * 3 t^2 (K/dis/i2_i3/extension.py:21-36).
+struct RuleView {
     int n;
-    double y[kMaxNested];
-    double w[kMaxNested];
+    const double* y;
+    const double* w;
 };
+
+__device__ inline RuleView stage_nested_rule(int n, double* y, double* w) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double x, wt;
+        gl_node(n, i, x, wt);
+        const double t = 0.5 * (x + 1.0);
+        y[i] = t * t * t;
+        w[i] = 0.5 * wt * 3.0 * t * t;
+    }
+    __syncthreads();
+    return {n, y, w};
+}
 
 // Everything an integrand reads, by value in the kernel arguments.
 struct DisArgs {
@@ -43,7 +87,7 @@ struct DisArgs {
     double z2min;   // Q0^2 / W^2 (S/dis/observables.py:71-73)
     double min_r, max_r;  // the dipole's table range (inside_table, :145-147)
     DisDipole dip;
-    NestedRule rule;
+    int nested_points;
 };
 
 // ------------------------------------------------------------- integrands
@@ -62,7 +106,7 @@ __device__ __forceinline__ T tripole_amp(const DisArgs& a, T x01, T x02, T x21, 
 }
 
 template <typename T, int K>
-__device__ T dis_value(const DisArgs& a, const double* u) {
+__device__ T dis_value(const DisArgs& a, const double* u, const RuleView& rule) {
     const T Q2 = (T)a.Q2;
     if constexpr (K == NLO_DIS_LO) {
         // S/dis/observables.py:345-352 and S/dis/lo.py:12-51
@@ -134,7 +178,7 @@ __device__ T dis_value(const DisArgs& a, const double* u) {
             // bilinear in the G values (K/dis/i2_i3/kernel.cu:768-912)
             const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
             TMoments<T> acc{T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-            for (int i = 0; i < a.rule.n; ++i) t_moments_axpy(acc, (T)a.rule.w[i], t_moments<T>(v, m2, (T)a.rule.y[i]));
+            for (int i = 0; i < rule.n; ++i) t_moments_axpy(acc, (T)rule.w[i], t_moments<T>(v, m2, (T)rule.y[i]));
             if constexpr (K == NLO_DIS_QG_I2_NESTED)
                 res = N012 * (a.transverse ? it_qg_i2<T>(v, f, m2, z1, z2, acc) : il_qg_i2<T>(v, f, Q2, z1, z2, acc));
             else
@@ -241,12 +285,15 @@ template <typename T, int K>
 __global__ void __launch_bounds__(256) integrand_kernel(DisArgs a, int64_t n, const double* __restrict__ x,
                                                         double* __restrict__ out) {
     constexpr int D = kDims[K];
+    __shared__ double ry[kMaxNested], rw[kMaxNested];
+    RuleView rule{0, ry, rw};
+    if constexpr (K == NLO_DIS_QG_I2_NESTED || K == NLO_DIS_QG_I3_NESTED) rule = stage_nested_rule(a.nested_points, ry, rw);
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double u[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) u[d] = x[i * D + d];
-    out[i] = (double)dis_value<T, K>(a, u);
+    out[i] = (double)dis_value<T, K>(a, u, rule);
 }
 
 struct VegasArgs {
@@ -298,6 +345,9 @@ __global__ void __launch_bounds__(256) vegas_kernel(DisArgs a, VegasArgs v) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red[4][2];
     __shared__ double wmax[4];
+    __shared__ double ry[kMaxNested], rw[kMaxNested];
+    RuleView rule{0, ry, rw};
+    if constexpr (K == NLO_DIS_QG_I2_NESTED || K == NLO_DIS_QG_I3_NESTED) rule = stage_nested_rule(a.nested_points, ry, rw);
     const VegasLayout lay = vegas_layout(D, v.bins, v.spt, HIST);
     double* E = reinterpret_cast<double*>(smem + lay.edges);
     uint32_t* dir = reinterpret_cast<uint32_t*>(smem + lay.dir);
@@ -375,7 +425,7 @@ __global__ void __launch_bounds__(256) vegas_kernel(DisArgs a, VegasArgs v) {
         }
         double wv = 0.0;
         if (valid) {
-            wv = (double)dis_value<T, K>(a, u) * jac;
+            wv = (double)dis_value<T, K>(a, u, rule) * jac;
             s1 += wv;
             s2 += wv * wv;
         }
@@ -467,23 +517,27 @@ __global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells
     cnt[k] = c;
 }
 
-struct GLRule {
-    int n;
-    double x[128];
-    double w[128];
-};
-
-// Tensor Gauss-Legendre over the unit square, fine rule in block 0 and coarse
-// rule in block 1; FP64 sums in a fixed order.
+// Tensor Gauss-Legendre over the unit square, `points` nodes per axis in
+// block 0 and max(2, points / 2) in block 1, nodes computed into LDS; FP64
+// sums in a fixed order.
 template <typename T, int K>
-__global__ void __launch_bounds__(256) gl_kernel(DisArgs a, GLRule fine, GLRule coarse, double* __restrict__ out) {
+__global__ void __launch_bounds__(256) gl_kernel(DisArgs a, int points, double* __restrict__ out) {
     __shared__ double red[4];
-    const GLRule& g = blockIdx.x == 0 ? fine : coarse;
+    __shared__ double gx[128], gw[128];
+    const int n = blockIdx.x == 0 ? points : (points / 2 > 2 ? points / 2 : 2);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        double x, w;
+        gl_node(n, i, x, w);
+        gx[i] = 0.5 * (x + 1.0);
+        gw[i] = 0.5 * w;
+    }
+    __syncthreads();
+    const RuleView none{0, gx, gw};
     double s = 0.0;
-    for (int idx = threadIdx.x; idx < g.n * g.n; idx += blockDim.x) {
-        const int i = idx / g.n, j = idx - i * g.n;
-        const double u[2] = {g.x[i], g.x[j]};
-        s += g.w[i] * g.w[j] * (double)dis_value<T, K>(a, u);
+    for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+        const int i = idx / n, j = idx - i * n;
+        const double u[2] = {gx[i], gx[j]};
+        s += gw[i] * gw[j] * (double)dis_value<T, K>(a, u, none);
     }
     s = wave_sum(s);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -573,18 +627,9 @@ int make_args(const nlo_dis_desc* desc, DisArgs& a) {
         a.min_r = ends[0];
         a.max_r = ends[1];
     }
-    const int np = desc->nested_points;
-    NLO_REQUIRE(np >= 8 && np <= kMaxNested, "nested_points must be 8..%d", kMaxNested);
-    std::vector<double> x(np), w(np);
-    int grc = nlo_gauss_legendre(np, x.data(), w.data());
-    if (grc != NLO_OK) return grc;
-    a.rule.n = np;
-    for (int i = 0; i < np; ++i) {
-        // y = t^3, t = (x + 1)/2, weight w/2 * 3 t^2 (K/dis/i2_i3/extension.py:21-36)
-        const double t = 0.5 * (x[i] + 1.0);
-        a.rule.y[i] = t * t * t;
-        a.rule.w[i] = 0.5 * w[i] * 3.0 * t * t;
-    }
+    NLO_REQUIRE(desc->nested_points >= 8 && desc->nested_points <= kMaxNested, "nested_points must be 8..%d",
+                kMaxNested);
+    a.nested_points = desc->nested_points;
     return NLO_OK;
 }
 
@@ -788,27 +833,12 @@ int nlo_dis_gauss_legendre(const nlo_dis_desc* desc, int32_t contribution, int32
     DisArgs a;
     int rc = make_args(desc, a);
     if (rc != NLO_OK) return rc;
-    GLRule fine{}, coarse{};
-    auto fill = [](GLRule& g, int n) -> int {
-        std::vector<double> x(n), w(n);
-        int r = nlo_gauss_legendre(n, x.data(), w.data());
-        if (r != NLO_OK) return r;
-        g.n = n;
-        for (int i = 0; i < n; ++i) {
-            g.x[i] = 0.5 * (x[i] + 1.0);
-            g.w[i] = 0.5 * w[i];
-        }
-        return NLO_OK;
-    };
-    rc = fill(fine, points);
-    if (rc == NLO_OK) rc = fill(coarse, points / 2 > 2 ? points / 2 : 2);
-    if (rc != NLO_OK) return rc;
 #define DIS_GL(K)                                                                                          \
     case K:                                                                                                \
         if (precision == NLO_PREC_FP64)                                                                    \
-            hipLaunchKernelGGL((gl_kernel<double, K>), dim3(2), dim3(256), 0, stream, a, fine, coarse, d_out); \
+            hipLaunchKernelGGL((gl_kernel<double, K>), dim3(2), dim3(256), 0, stream, a, points, d_out); \
         else                                                                                               \
-            hipLaunchKernelGGL((gl_kernel<float, K>), dim3(2), dim3(256), 0, stream, a, fine, coarse, d_out);  \
+            hipLaunchKernelGGL((gl_kernel<float, K>), dim3(2), dim3(256), 0, stream, a, points, d_out);  \
         break;
     switch (contribution) {
         DIS_GL(NLO_DIS_LO)

@@ -317,12 +317,158 @@ class Evolver:
             pass
 
 
+# ------------------------------------------------------------------- DIS
+
+
+def dis_special(fn: str, x: torch.Tensor, *, precision: str = "fp64") -> torch.Tensor:
+    """K0, K1, K2 or Li2 of a float64 device tensor (S/numerics/special.py:18-97)."""
+
+    codes = {"K0": _lib.NLO_FN_K0, "K1": _lib.NLO_FN_K1, "K2": _lib.NLO_FN_K2, "Li2": _lib.NLO_FN_LI2}
+    require_cuda(x, torch.float64, "x")
+    out = torch.empty_like(x)
+    check(lib().nlo_dis_special(codes[fn], x.numel(), ptr(x), PRECISION_CODES[precision], ptr(out),
+                                stream_handle(x.device)), "nlo_dis_special")
+    return out
+
+
+def dis_dipole_struct(dipole) -> _lib.DisDipole:
+    """The C view of a GBW or BKDipole (nlo_bk_optimized_amd.dipole.amplitude)."""
+
+    d = _lib.DisDipole()
+    if dipole.kind == "gbw":
+        d.kind = _lib.NLO_DIPOLE_GBW
+        d.qs0sqr, d.lambda_, d.gamma, d.x0 = dipole.Qs0sqr, dipole.lambda_, dipole.gamma, dipole.x0
+        return d
+    d.kind = _lib.NLO_DIPOLE_BK_TABLE
+    d.R, d.T = dipole.device_r.numel(), dipole.device_Y.numel()
+    d.d_r, d.d_Y, d.d_N = ptr(dipole.device_r), ptr(dipole.device_Y), ptr(dipole.device_N)
+    d.x0 = 1.0
+    if dipole.device_coef is not None:
+        d.initialized = 1
+        d.y_init = dipole.interpolator_Y
+        d.maxr_interpolate = dipole.maxr_interpolate
+        d.d_init_coef = ptr(dipole.device_coef)
+    return d
+
+
+def dis_dipole_amplitude(dipole, r: torch.Tensor, Y: torch.Tensor) -> torch.Tensor:
+    """N(r, Y) on the device for float64 tensors of one shape."""
+
+    require_cuda(r, torch.float64, "r")
+    require_cuda(Y, torch.float64, "Y")
+    if r.shape != Y.shape:
+        raise ValueError("r and Y must have the same shape")
+    out = torch.empty_like(r)
+    d = dis_dipole_struct(dipole)
+    check(lib().nlo_dis_dipole_amplitude(ctypes.byref(d), r.numel(), ptr(r), ptr(Y), ptr(out),
+                                         stream_handle(r.device)), "nlo_dis_dipole_amplitude")
+    return out
+
+
+def dis_bk_initialize(dipole, Y: float) -> tuple[torch.Tensor, float]:
+    """Coefficients (4 (R - 1)) and maxr_interpolate of BKDipole.initialize_interpolation at Y."""
+
+    d = dis_dipole_struct(dipole)
+    d.initialized = 0
+    coef = torch.empty(4 * (dipole.device_r.numel() - 1), dtype=torch.float64, device=dipole.device_r.device)
+    maxr = torch.empty(1, dtype=torch.float64, device=dipole.device_r.device)
+    check(lib().nlo_dis_bk_initialize(ctypes.byref(d), float(Y), ptr(coef), ptr(maxr),
+                                      stream_handle(coef.device)), "nlo_dis_bk_initialize")
+    return coef, float(maxr.item())
+
+
+def dis_desc(Q2: float, xbj: float, transverse: bool, quarks, dipole, config) -> _lib.DisDesc:
+    """nlo_dis_desc from DISConfig fields (S/dis/config.py:101-145)."""
+
+    from nlo_bk_optimized_amd.dis.config import (
+        HeavyQuarkX,
+        NcScheme,
+        RunningCouplingIRScheme,
+        RunningCouplingScheme,
+    )
+
+    if not 1 <= len(quarks) <= _lib.DIS_MAX_QUARKS:
+        raise ValueError(f"between 1 and {_lib.DIS_MAX_QUARKS} quarks are supported")
+    d = _lib.DisDesc()
+    d.Q2, d.xbj = float(Q2), float(xbj)
+    d.transverse = 1 if transverse else 0
+    d.n_quarks = len(quarks)
+    for i, q in enumerate(quarks):
+        d.mass[i] = q.mass
+        d.charge[i] = q.charge
+    d.maxr = config.maxr
+    d.finite_nc = 1 if config.nc_scheme is NcScheme.FiniteNC else 0
+    d.parent_coupling = 1 if config.rc_scheme is RunningCouplingScheme.PARENT else 0
+    d.smooth_coupling = 1 if config.rc_ir_scheme is RunningCouplingIRScheme.SMOOTH else 0
+    d.mass_dependent_x = 1 if config.heavy_quark_x_scheme is HeavyQuarkX.MassDependentX else 0
+    d.c2_alpha, d.max_alpha = config.C2_alpha, config.max_alpha_s_freeze
+    d.nf_alphas = config.active_flavors
+    d.nested_points = config.cuda_nested_points
+    d.dipole = dis_dipole_struct(dipole)
+    return d
+
+
+def dis_integrand(desc: _lib.DisDesc, kind: str, x: torch.Tensor, *, precision: str = "fp64") -> torch.Tensor:
+    """The contribution's integrand at unit-cube points x[n, dim] (float64 device)."""
+
+    require_cuda(x, torch.float64, "x")
+    dim = _lib.DIS_DIMENSIONS[kind]
+    if x.ndim != 2 or x.shape[1] != dim:
+        raise ValueError(f"{kind} needs points of shape (n, {dim})")
+    out = torch.empty(x.shape[0], dtype=torch.float64, device=x.device)
+    check(lib().nlo_dis_integrand(ctypes.byref(desc), _lib.DIS_KINDS[kind], x.shape[0], ptr(x),
+                                  PRECISION_CODES[precision], ptr(out), stream_handle(x.device)), "nlo_dis_integrand")
+    return out
+
+
+def dis_vegas(desc: _lib.DisDesc, kind: str, edges: torch.Tensor, table: _lib.SobolTable, n: int, *,
+              first_index: int = 0, precision: str = "fp64", histogram: bool = True):
+    """One VEGAS pass: (sums[2] = (sum w, sum w^2), hist[dim, B], count[dim, B])."""
+
+    require_cuda(edges, torch.float64, "edges")
+    dim = _lib.DIS_DIMENSIONS[kind]
+    bins = edges.shape[-1] - 1
+    if edges.shape != (dim, bins + 1):
+        raise ValueError(f"edges must be ({dim}, bins + 1)")
+    dev = edges.device
+    sums = torch.empty(2, dtype=torch.float64, device=dev)
+    hist = torch.empty((dim, bins), dtype=torch.float64, device=dev) if histogram else None
+    count = torch.empty((dim, bins), dtype=torch.float64, device=dev) if histogram else None
+    ws_bytes = lib().nlo_dis_vegas_workspace(_lib.DIS_KINDS[kind], bins, n)
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    check(
+        lib().nlo_dis_vegas(ctypes.byref(desc), _lib.DIS_KINDS[kind], bins, ptr(edges), ctypes.byref(table),
+                            first_index, n, PRECISION_CODES[precision], ptr(sums), ptr(hist), ptr(count), ptr(ws),
+                            ws_bytes, stream_handle(dev)),
+        "nlo_dis_vegas",
+    )
+    return sums, hist, count
+
+
+def dis_gauss_legendre(desc: _lib.DisDesc, kind: str, points: int, *, precision: str = "fp64",
+                       device: torch.device | str = "cuda") -> torch.Tensor:
+    """[fine, coarse] tensor Gauss-Legendre sums of a two-dimensional contribution."""
+
+    out = torch.empty(2, dtype=torch.float64, device=device)
+    check(lib().nlo_dis_gauss_legendre(ctypes.byref(desc), _lib.DIS_KINDS[kind], points, PRECISION_CODES[precision],
+                                       ptr(out), stream_handle(out.device)), "nlo_dis_gauss_legendre")
+    return out
+
+
 def dtype_epsilon(dtype: torch.dtype) -> float:
     return float(torch.finfo(dtype).eps)
 
 
 __all__ = [
     "Evolver",
+    "dis_bk_initialize",
+    "dis_desc",
+    "dis_dipole_amplitude",
+    "dis_dipole_struct",
+    "dis_gauss_legendre",
+    "dis_integrand",
+    "dis_special",
+    "dis_vegas",
     "PRECISION_CODES",
     "RK_CODES",
     "bk_k1",

@@ -106,3 +106,31 @@ def test_struct_layouts_match_the_header():
     # nlo_sobol_table: dim + 8 x 30 directions + 8 shifts + 8 first points
     assert ctypes.sizeof(_lib.SobolTable) == 4 * (1 + 8 * 30 + 8 + 8)
     assert ctypes.sizeof(_lib.Physics) == 6 * 8 + 2 * 4
+    # nlo_dis_dipole: 2 int32, 4 doubles, 2 int32, 3 pointers, 2 doubles, 1 pointer
+    assert ctypes.sizeof(_lib.DisDipole) == 8 + 32 + 8 + 24 + 16 + 8
+    # nlo_dis_desc: Q2, xbj, 2 int32, 8 + 8 doubles, maxr, 4 int32, 2 doubles, 2 int32, dipole
+    assert ctypes.sizeof(_lib.DisDesc) == 16 + 8 + 128 + 8 + 16 + 16 + 8 + ctypes.sizeof(_lib.DisDipole)
+
+
+def test_dis_dimensions_and_argument_errors():
+    L = _lib.lib()
+    assert [L.nlo_dis_dimensions(k) for k in range(12)] == [2, 2, 3, 4, 2, 3, 4, 5, 6, 7, 5, 5]
+    assert L.nlo_dis_dimensions(12) == -1
+    for name, k in _lib.DIS_KINDS.items():
+        assert _lib.DIS_DIMENSIONS[name] == L.nlo_dis_dimensions(k)
+    d = _lib.DisDesc()
+    d.Q2, d.xbj, d.n_quarks, d.maxr, d.c2_alpha, d.max_alpha, d.nested_points = 9.0, 1e-3, 1, 30.0, 1.0, 0.7, 48
+    d.dipole.kind, d.dipole.x0, d.dipole.qs0sqr = 0, 1.0, 1.0
+    # invalid contribution, precision and Sobol dimension fail before any launch
+    assert L.nlo_dis_integrand(ctypes.byref(d), 12, 4, None, 1, None, None) == 1
+    assert L.nlo_dis_integrand(ctypes.byref(d), 0, 4, None, 2, None, None) == 1
+    t = _lib.SobolTable()
+    t.dim = 4
+    assert L.nlo_dis_vegas(ctypes.byref(d), 0, 32, None, ctypes.byref(t), 0, 64, 0, None, None, None, None, 0,
+                           None) == 1
+    assert b"dimension" in L.nlo_last_error()
+    d.nested_points = 4
+    t.dim = 2
+    assert L.nlo_dis_gauss_legendre(ctypes.byref(d), 0, 24, 0, ctypes.c_void_p(8), None) == 1
+    assert b"nested_points" in L.nlo_last_error()
+    assert L.nlo_dis_vegas_workspace(9, 32, 1 << 20) > L.nlo_dis_vegas_workspace(9, 32, 1 << 10) > 0
