@@ -48,6 +48,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--kernel-launches", type=int, default=5)
+    p.add_argument("--cpu-samples-log2", type=int, default=20,
+                   help="Sobol points per parent in the CPU baseline sample")
+    p.add_argument("--kernel-probe", action="store_true",
+                   help="only run the FP32 K2+Kf launch and a 256 MiB device copy (for counter runs)")
     return p.parse_args()
 
 
@@ -110,6 +114,23 @@ def main() -> int:
         return dict(setup=setup, seconds=elapsed, adapt_seconds=t_adapt, evaluations=evaluations,
                     N=N, nonfinite=flag, config=config)
 
+    if args.kernel_probe:
+        # Counter runs (rocprofv3 --pmc) need only the dominant kernel and a copy
+        # of known size whose FETCH_SIZE / WRITE_SIZE calibrate the counters.
+        config = BKConfig(PRECISION="fp32", **base)
+        setup = evo.prepare(ic, config, device=device, seed=0)
+        t64, t32 = dev.spline_build(setup.r, setup.N0)
+        for _ in range(args.kernel_launches):
+            dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32")
+        src = torch.ones(1 << 25, dtype=torch.float64, device=device)  # 256 MiB
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"probe": "nlo_kernel<FP32>", "launches": args.kernel_launches,
+                          "copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}), flush=True)
+        return 0
+
     log(f"fp32 evolution: {args.steps} steps, 2^{args.samples_log2} samples/parent/RHS")
     fp32 = evolve("fp32", args.steps, args.warmup)
     samples = fp32["evaluations"] * n
@@ -155,9 +176,10 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         # The FP64 oracle (C restatement of the reference CPU algorithm) on a
-        # bounded sample of the same right-hand side: 64 parents x 2^17 Sobol
-        # points of K2 + Kf (reference formula) plus their K1, at the initial
-        # condition with the adapted grids, on every host core OpenMP uses.
+        # bounded sample of the same right-hand side: every active parent x
+        # 2^cpu_samples_log2 Sobol points of K2 + Kf (reference formula) plus
+        # their K1, at the initial condition with the adapted grids, on every
+        # host core OpenMP uses (about 10 s at the default size).
         sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
         import numpy as np
 
@@ -168,9 +190,8 @@ def main() -> int:
         edges = setup.edges.cpu().numpy()
         sN, sS = O.evolution_splines(r, N0)
         lo, width = O.nlo_bounds(r)
-        active = np.nonzero((np.arange(r.size) >= 1) & (N0 <= 0.99999))[0]
-        pick = active[np.linspace(0, active.size - 1, min(64, active.size)).astype(int)]
-        m = 1 << 17
+        pick = np.nonzero((np.arange(r.size) >= 1) & (N0 <= 0.99999))[0]
+        m = 1 << args.cpu_samples_log2
         table = O.sobol_table(4, 0)
         t0 = time.perf_counter()
         O.bk_nlo_sums(r[pick], sS, edges[pick], lo, width, table, 0, m, O.params(minr=1e-6))
@@ -181,8 +202,8 @@ def main() -> int:
             "unit": "samples/s",
             "cores": O.num_threads(),
             "kind": "port",
-            "sample": f"{pick.size} parents x 2^17 Sobol points of K2+Kf (FP64 reference formula) "
-                      f"plus their K1 (8x24 GL per panel) at Y=0, {cpu_s:.2f} s wall",
+            "sample": f"{pick.size} active parents x 2^{args.cpu_samples_log2} Sobol points of K2+Kf (FP64 "
+                      f"reference formula) plus their K1 (8x24 GL per panel) at Y=0, {cpu_s:.2f} s wall",
         }
         log(f"cpu baseline: {cpu['value']:.4e} samples/s on {cpu['cores']} threads")
 

@@ -1,0 +1,58 @@
+// J1 and I1 in FP64 for the double-log resummation of K1
+// (S/bk_kernels/k1.py:108-132 uses torch.special.bessel_j1 / modified_bessel_i1).
+// Both return f(x) / (x / 2) so that the caller's J1(2a)/a and I1(2a)/a are
+// direct; written for x in [0, 200] (the resummation arguments stay below ~80).
+//   I1: the ascending series sum (x^2/4)^k / (k! (k+1)!), all terms positive.
+//   J1: x < 2, the alternating series (at most 2x smaller than its first
+//       term); otherwise Miller's backward recurrence
+//       J_{k-1} = (2k / x) J_k - J_{k+1} from k = N ~ x + 32, normalised by
+//       J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.27 and 9.1.46).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace nlo {
+
+__host__ __device__ inline double i1_over_half_x(double x) {
+    const double t = 0.25 * x * x;
+    double term = 1.0, sum = 1.0;
+    for (int k = 1; k < 400; ++k) {
+        term *= t / ((double)k * (double)(k + 1));
+        sum += term;
+        if (term < 1e-17 * sum) break;
+    }
+    return sum;
+}
+
+__host__ __device__ inline double j1_over_half_x(double x) {
+    if (x < 2.0) {
+        const double t = -0.25 * x * x;
+        double term = 1.0, sum = 1.0;
+        for (int k = 1; k < 30; ++k) {
+            term *= t / ((double)k * (double)(k + 1));
+            sum += term;
+        }
+        return sum;
+    }
+    const int N = 2 * ((int)(0.625 * x) + 16);  // even, > 1.25 x + 30
+    const double two_over_x = 2.0 / x;
+    double jp = 0.0, j = 1e-30;  // J_{k+1}, J_k (unnormalised)
+    double norm = 0.0, j1 = 0.0;
+    for (int k = N; k >= 1; --k) {
+        const double jm = (double)k * two_over_x * j - jp;  // J_{k-1}
+        jp = j;
+        j = jm;
+        if (k == 2) j1 = j;                       // J_1, produced by the step from k = 2
+        if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * jm;  // even order >= 2
+        if (fabs(j) > 1e250) {  // rescale to stay in range
+            j *= 1e-250;
+            jp *= 1e-250;
+            norm *= 1e-250;
+            j1 *= 1e-250;
+        }
+    }
+    norm += j;  // J_0
+    return (j1 / norm) / (0.5 * x);
+}
+
+}  // namespace nlo

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "bessel_j1_i1.hpp"
 #include "common.hpp"
 
 namespace nlo {
@@ -101,50 +102,60 @@ __device__ __forceinline__ T loglog_from_log(const LdsSpline<T>& s, T log_d, T u
 
 // Kernel_lo (S/bk_kernels/k1.py:20-155) for BALITSKY_LO, RESUM_RC_SMALLEST,
 // NLO_RESUM_DLOG_SLOG, times the dipole of _lo_integrand
-// (S/bk/derivatives.py:445-468).  X^2 = (r - z)^2 + 4 r z sin^2(theta / 2)
-// (the form K/bk/k1/kernel.cu:478-479 uses; it keeps X accurate near X -> 0).
-__device__ double k1_integrand64(double r, double z, double theta, double n_r, double alpha_r,
-                                 const LdsSpline<double>& sN, double r_min, double r_max,
-                                 const BkConsts& c) {
-    const double sh = sin(theta / 2.0);
-    const double xsq = (r - z) * (r - z) + 4.0 * r * z * sh * sh;
-    const bool invalid = xsq < 1e-40 || z < 1e-20 || r < 1e-20;
-    if (invalid) return 0.0;
+// (S/bk/derivatives.py:445-468), split so that everything that depends only
+// on the daughter distance Y = z is computed once per z node (K1Z) and the
+// theta loop evaluates the rest (k1_theta_term).  X^2 = (r - z)^2 +
+// 4 r z sin^2(theta / 2) (the form K/bk/k1/kernel.cu:478-479 uses; it keeps
+// X accurate near X -> 0).  Logarithms of distances are formed once and
+// shared by the spline, the coupling and the resummation logs.
+
+// alpha(d) from ln d (S/coupling.py:81-97): ln(4 C2 / (d^2 Lambda^2)) = log_scale0 - 2 ln d.
+__device__ __forceinline__ double alpha_s64_log(double log_d, const BkConsts& c) {
+    const double right = (c.log_scale0 - 2.0 * log_d) / 0.2;
+    const double hi = fmax(c.log_mu0, right);
+    return 1.0 / (c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right))))));
+}
+
+struct K1Parent {
+    double r, log_r, n_r, a_r;
+    double lo_coef;     // Nc / (2 pi^2) alpha(r)
+    double fin_coef;    // alpha(r) Nc / (4 pi) (67/9 - pi^2/3 - 10/9 Nf/Nc)
+    double log_ksub;
+    double resum_on;    // 1 where r > 1.01 MINR (double-log resummation active)
+};
+
+struct K1Z {
+    double z, log_z, n_y, a_y, ly;
+};
+
+__device__ __forceinline__ double k1_theta_term(const K1Parent& P, const K1Z& Z, double s2,
+                                                const LdsSpline<double>& sN, const BkConsts& c) {
+    const double r = P.r, z = Z.z;
+    const double xsq = (r - z) * (r - z) + 4.0 * r * z * s2;
+    if (xsq < 1e-40 || z < 1e-20 || r < 1e-20) return 0.0;
     const double X = sqrt(xsq);
+    const double lnX = log(X);
+    const double n_x = fmin(loglog_from_log(sN, lnX, 0.0, 1.0), 1.0);
+    const double dipole = n_x + Z.n_y - P.n_r - n_x * Z.n_y;
+    const double a_x = alpha_s64_log(lnX, c);
     const double Y = z;
-    const double n_x = fmin(loglog64(sN, r_min, r_max, X, 0.0, 1.0), 1.0);
-    const double n_y = fmin(loglog64(sN, r_min, r_max, Y, 0.0, 1.0), 1.0);
-    const double dipole = n_x + n_y - n_r - n_x * n_y;
-
-    const double a_x = alpha_s64(X, c);
-    const double a_y = alpha_s64(Y, c);
-    // alpha(min(X, Y, r)) without a fourth evaluation
-    const double a_min = fmin(X, Y) <= r ? (X <= Y ? a_x : a_y) : alpha_r;
-    double lo = c.nc / (2.0 * kPi * kPi) * alpha_r *
-                (r * r / (X * X * Y * Y) + (a_y / a_x - 1.0) / (Y * Y) + (a_x / a_y - 1.0) / (X * X));
+    const double a_min = fmin(X, Y) <= r ? (X <= Y ? a_x : Z.a_y) : P.a_r;
+    double lo = P.lo_coef * (r * r / (xsq * Y * Y) + (Z.a_y / a_x - 1.0) / (Y * Y) + (a_x / Z.a_y - 1.0) / xsq);
     lo = isfinite(lo) ? lo : 0.0;
-
-    const double lx = log(X / r), ly = log(Y / r);
-    const double xl = 4.0 * lx * ly;
-    const double as_x = sqrt(a_min * c.nc / kPi * fabs(xl));
-    double dlog;
-    if (as_x == 0.0) {
-        dlog = 1.0;
-    } else {
-        dlog = xl >= 0.0 ? j1(2.0 * as_x) / as_x : cyl_bessel_i1(2.0 * as_x) / as_x;
-    }
-    dlog = r > 1.01 * c.minr ? dlog : 1.0;
-    if (!isfinite(dlog)) return 0.0;
-
-    const double minxy = fmin(X, Y);
+    const double xl = 4.0 * (lnX - P.log_r) * Z.ly;
     const double abar = a_min * c.nc / kPi;
-    const double ratio = r / minxy;
-    const double slog = exp(-abar * (11.0 / 12.0) * fabs(log(c.ksub * ratio * ratio)));
-    const double slog_exp = -abar * (11.0 / 12.0) * fabs(2.0 * log(sqrt(c.ksub) * r / minxy));
-    const double rxy = r / (X * Y);
-    const double lo_single = alpha_r * c.nc / (2.0 * kPi * kPi) * rxy * rxy;
-    const double k1fin = lo * alpha_r * c.nc / (4.0 * kPi) *
-                         (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
+    const double as_x = sqrt(abar * fabs(xl));
+    double dlog = 1.0;
+    // J1(2a)/a or I1(2a)/a (k1.py:108-132)
+    if (P.resum_on != 0.0 && as_x != 0.0) dlog = xl >= 0.0 ? j1_over_half_x(2.0 * as_x) : i1_over_half_x(2.0 * as_x);
+    if (!isfinite(dlog)) return 0.0;
+    // ln(KSUB r^2 / min(X, Y)^2): the single-log exponent of both slog terms
+    const double lmin = X <= Y ? lnX : Z.log_z;
+    const double slog_exp = -abar * (11.0 / 12.0) * fabs(P.log_ksub + 2.0 * (P.log_r - lmin));
+    const double slog = exp(slog_exp);
+    const double rxy2 = r * r / (xsq * Y * Y);
+    const double lo_single = P.lo_coef * rxy2;
+    const double k1fin = lo * P.fin_coef;
     return (dlog * slog * lo - lo_single * slog_exp + k1fin) * dipole;
 }
 
@@ -433,23 +444,24 @@ __device__ __forceinline__ void fh_pair(float x, float& h, float& hm) {
 // t = (ln d - ln r_0) / h - k in [0, 1), coefficients pre-multiplied by
 // log2(e) so that N = 2^(a + t (b + t (c + t d))): one 16-byte LDS read.
 struct FastSpline {
+    // rec[0] and rec[R] are constant records for ln d below ln r_0 (N = 2^-126,
+    // i.e. 0 to FP32 precision, S = 1) and above ln r_{R-1} (N = 1, S = 0);
+    // rec[k], k = 1..R-1, is interval k - 1.  u = ln d * inv_h + off is the
+    // position in records, off = 1 - ln r_0 * inv_h.
     const float4* rec;
-    float x0, inv_h, kmax;  // kmax = R - 2
-    float log_min, log_max;
+    float inv_h, off, umax;  // umax = R
 };
 
 __device__ __forceinline__ float fspline_log2n(const FastSpline& s, float log_d) {
-    const float u = fminf(fmaxf((log_d - s.x0) * s.inv_h, 0.0f), s.kmax + 1.0f);
-    const float k = fminf(floorf(u), s.kmax);
+    const float u = fminf(fmaxf(fmaf(log_d, s.inv_h, s.off), 0.0f), s.umax);
+    const float k = floorf(u);
     const float t = u - k;
     const float4 c = s.rec[(int)k];
     return fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
 }
 
 __device__ __forceinline__ float fspline_n(const FastSpline& s, float log_d) {
-    float out = __builtin_amdgcn_exp2f(fspline_log2n(s, log_d));
-    out = log_d < s.log_min ? 0.0f : out;
-    return log_d > s.log_max ? 1.0f : out;
+    return __builtin_amdgcn_exp2f(fspline_log2n(s, log_d));
 }
 
 // N and S = 1 - N.  Near N = 1 the subtraction would keep only the absolute
@@ -465,8 +477,6 @@ __device__ __forceinline__ void fspline_ns(const FastSpline& s, float log_d, flo
     e = fmaf(x, e, 0.5f);
     e = fmaf(x * x, e, x);  // expm1(x)
     sv = fabsf(x) < 0.25f ? -e : 1.0f - n;
-    if (log_d < s.log_min) { n = 0.0f; sv = 1.0f; }
-    if (log_d > s.log_max) { n = 1.0f; sv = 0.0f; }
 }
 
 // The FP32 K2 + Kf integrand: the cancellation-free formula of

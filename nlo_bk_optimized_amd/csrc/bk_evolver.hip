@@ -124,7 +124,7 @@ int enqueue_step(nlo_bk_evolver* ev, hipStream_t s) {
         if (d.nlo) {
             rc = launch_nlo(R, ev->r, in, ev->t64, ev->t32, ev->edges, d.bins, d.sobol, d.first_index,
                             d.n_samples, d.precision, ev->c, ev->sums, nullptr, nullptr, ev->nlo_ws,
-                            ev->nlo_ws_bytes, s);
+                            ev->nlo_ws_bytes, s, /*components=*/false);
             if (rc != NLO_OK) return rc;
             rc = launch_rhs_finalize(R, in, ev->k1, ev->sums, d.n_samples, ev->K[st], ev->k2kf,
                                      ev->active, s);

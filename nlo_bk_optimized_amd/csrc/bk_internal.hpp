@@ -33,7 +33,7 @@ int launch_k1(int R, const double* r, const double* N, const double* t64, const 
 int launch_nlo(int R, const double* r, const double* N, const double* t64, const float* t32,
                const double* edges, int bins, const nlo_sobol_table& sobol, int64_t first, int64_t n,
                int precision, const BkConsts& c, double* sums, double* hist, double* count, void* ws,
-               size_t ws_bytes, hipStream_t stream);
+               size_t ws_bytes, hipStream_t stream, bool components = true);
 int launch_rhs_finalize(int R, const double* N, const double* k1, const double* sums, int64_t n,
                         double* dndy, double* k2kf, unsigned long long* counter, hipStream_t stream);
 int launch_rk_stage(int R, const double* N, const double* K0, const double* K1, const double* K2,

@@ -167,14 +167,18 @@ __device__ __forceinline__ double block_sum_256(double v, double* red) {
 
 // K/bk/k1/kernel.cu:2040-2092 (production rule) for parent p = blockIdx.x:
 // panels [x_i, x_{i+1}] of the log-r grid, GL(nr) in ln z, GL(nt) in v with
-// theta = pi v^2, weight half_width w_r * w_t pi v, integrand * 2 z^2.
-__global__ void __launch_bounds__(256) k1_kernel(int R, const double* __restrict__ r,
-                                                 const double* __restrict__ N,
-                                                 const double* __restrict__ t64, BkConsts c,
-                                                 GaussRule g, double* __restrict__ k1) {
+// theta = pi v^2, weight half_width w_r * w_t pi v, integrand * 2 z^2.  Each
+// thread owns whole z nodes (panel, i) and loops over the nt theta nodes,
+// whose sin^2(theta/2) and weights sit in LDS.
+constexpr int kK1Threads = 1024;
+
+__global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __restrict__ r,
+                                                        const double* __restrict__ N,
+                                                        const double* __restrict__ t64, BkConsts c,
+                                                        GaussRule g, double* __restrict__ k1) {
     extern __shared__ double sh[];
-    __shared__ double red[4];
-    __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], gxt[kMaxGaussT], gwt[kMaxGaussT];
+    __shared__ double red[kK1Threads / 64];
+    __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], s2t[kMaxGaussT], wtt[kMaxGaussT];
     const int p = blockIdx.x;
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
         if (threadIdx.x == 0) k1[p] = 0.0;
@@ -186,33 +190,48 @@ __global__ void __launch_bounds__(256) k1_kernel(int R, const double* __restrict
         gxr[i] = g.xr[i];
         gwr[i] = g.wr[i];
     }
-    for (int i = threadIdx.x; i < g.nt; i += blockDim.x) {
-        gxt[i] = g.xt[i];
-        gwt[i] = g.wt[i];
+    for (int j = threadIdx.x; j < g.nt; j += blockDim.x) {
+        const double v = (g.xt[j] + 1.0) / 2.0;
+        const double sn = sin(kPi * v * v / 2.0);
+        s2t[j] = sn * sn;
+        wtt[j] = g.wt[j] * kPi * v;
     }
     __syncthreads();
     const double r_min = t64[0], r_max = t64[1];
-    const double rp = r[p];
-    const double n_r = fmin(loglog64(s, r_min, r_max, rp, 0.0, 1.0), 1.0);
-    const double a_r = alpha_s64(rp, c);
-    const int per_panel = g.nr * g.nt;
-    const int nodes = (R - 1) * per_panel;
+    K1Parent P;
+    P.r = r[p];
+    P.log_r = log(P.r);
+    P.n_r = fmin(loglog64(s, r_min, r_max, P.r, 0.0, 1.0), 1.0);
+    P.a_r = alpha_s64(P.r, c);
+    P.lo_coef = c.nc / (2.0 * kPi * kPi) * P.a_r;
+    P.fin_coef = P.a_r * c.nc / (4.0 * kPi) * (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
+    P.log_ksub = log(c.ksub);
+    P.resum_on = P.r > 1.01 * c.minr ? 1.0 : 0.0;
+    const int znodes = (R - 1) * g.nr;
     double total = 0.0;
-    for (int k = threadIdx.x; k < nodes; k += blockDim.x) {
-        const int panel = k / per_panel;
-        const int loc = k - panel * per_panel;
-        const int i = loc / g.nt, j = loc - i * g.nt;
+    for (int zi = threadIdx.x; zi < znodes; zi += blockDim.x) {
+        const int panel = zi / g.nr;
+        const int i = zi - panel * g.nr;
         const double lower = s.x[panel], upper = s.x[panel + 1];
         const double center = (lower + upper) / 2.0, half = (upper - lower) / 2.0;
-        const double log_z = center + half * gxr[i];
-        const double z = exp(log_z);
-        const double v = (gxt[j] + 1.0) / 2.0;
-        const double theta = kPi * v * v;
-        const double w = (half * gwr[i]) * (gwt[j] * kPi * v);
-        total += k1_integrand64(rp, z, theta, n_r, a_r, s, r_min, r_max, c) * (2.0 * z * z) * w;
+        K1Z Z;
+        Z.log_z = center + half * gxr[i];
+        Z.z = exp(Z.log_z);
+        Z.n_y = fmin(loglog_from_log(s, Z.log_z, 0.0, 1.0), 1.0);
+        Z.a_y = alpha_s64_log(Z.log_z, c);
+        Z.ly = Z.log_z - P.log_r;
+        double acc = 0.0;
+        for (int j = 0; j < g.nt; ++j) acc += k1_theta_term(P, Z, s2t[j], s, c) * wtt[j];
+        total += acc * ((2.0 * Z.z * Z.z) * (half * gwr[i]));
     }
-    total = block_sum_256(total, red);
-    if (threadIdx.x == 0) k1[p] = total;
+    total = wave_sum(total);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kK1Threads / 64; ++w) t += red[w];
+        k1[p] = t;
+    }
 }
 
 // ------------------------------------------------------------- K2 + Kf
@@ -227,8 +246,8 @@ __host__ __device__ inline NloLayout nlo_layout(int R, int bins, int spt, bool h
     size_t o = 0;
     L.x = o;
     if (tsize == sizeof(double)) o = align_up(o + (size_t)R * tsize, 16);  // FP32 needs no nodes
-    L.rec = o;
-    o = align_up(o + (size_t)4 * (R - 1) * tsize, 16);
+    L.rec = o;  // FP64: R - 1 records; FP32: R + 1 (with the two range records)
+    o = align_up(o + (size_t)4 * (tsize == sizeof(double) ? R - 1 : R + 1) * tsize, 16);
     L.edges = o;  // FP64: 4 x (bins + 1) edges; FP32: 4 x bins (left, width) pairs
     o = align_up(o + (size_t)4 * (bins + 1) * sizeof(double), 16);
     L.dir = o;
@@ -268,7 +287,9 @@ struct NloArgs {
 // s = (blk * spt * 4 + j * 4 + wave) * 64 + lane, Sobol index first + s, so
 // lanes of a wave hold 64 consecutive indices and the gray-code state splits
 // into a wave-uniform high part and a per-lane low part.
-template <int MODE, bool HIST>
+// FULL = false (the evolver): only sum w is accumulated; the sum of w^2 and
+// the K2 / Kf split read as 0.
+template <int MODE, bool HIST, bool FULL>
 __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     using T = typename std::conditional<MODE == NLO_PREC_FP32, float, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -294,25 +315,30 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     LdsSpline<double> spl{};
     FastSpline fsp{};
     if constexpr (MODE == NLO_PREC_FP32) {
-        // N records (row 0) of the local-coordinate FP32 table, 16 B each
+        // N records (row 0) of the local-coordinate FP32 table, 16 B each,
+        // between a constant record below the grid (N = 2^-126) and one above
+        // it (N = 1), so that out-of-range lookups need no selects
         float4* rec4 = reinterpret_cast<float4*>(srec);
         const float4* src = reinterpret_cast<const float4*>(a.t32 + 2 + a.R);
         const bool aligned = ((reinterpret_cast<uintptr_t>(a.t32) + (2 + a.R) * 4) & 15) == 0;
         for (int i = threadIdx.x; i < a.R - 1; i += blockDim.x) {
             if (aligned) {
-                rec4[i] = src[i];
+                rec4[i + 1] = src[i];
             } else {
                 const float* f = a.t32 + 2 + a.R + 4 * i;
-                rec4[i] = make_float4(f[0], f[1], f[2], f[3]);
+                rec4[i + 1] = make_float4(f[0], f[1], f[2], f[3]);
             }
         }
+        if (threadIdx.x == 0) {
+            rec4[0] = make_float4(-126.0f, 0.0f, 0.0f, 0.0f);
+            rec4[a.R] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
         const double x0 = a.t64[2], xl = a.t64[2 + a.R - 1];
+        const double inv_h = (double)(a.R - 1) / (xl - x0);
         fsp.rec = rec4;
-        fsp.x0 = (float)x0;
-        fsp.inv_h = (float)((double)(a.R - 1) / (xl - x0));
-        fsp.kmax = (float)(a.R - 2);
-        fsp.log_min = (float)x0;
-        fsp.log_max = (float)xl;
+        fsp.inv_h = (float)inv_h;
+        fsp.off = (float)(1.0 - x0 * inv_h);
+        fsp.umax = (float)a.R;
     } else {
         // FP64 reference formula reads the S spline (row 1); the stable form reads N (row 0).
         stage_spline<double>(a.t64, a.R, MODE == NLO_PREC_FP64 ? 1 : 0, reinterpret_cast<double*>(sx),
@@ -390,37 +416,46 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     // then updated by one XOR per group: gray(h) ^ gray(h - 1) = lowest set
     // bit of h.
     const int64_t hi0 = (a.first >> 6) + ((int64_t)blk * 4 + wave) * a.spt;
+    // vh[d]: the wave-uniform part of the state, direction bits 6..29 of
+    // gray(hi) plus bit 5 of the parity of hi (the one lane-independent bit of
+    // the low part), read from the kernel arguments so it stays scalar
     uint32_t vhi[4] = {0u, 0u, 0u, 0u};
     {
         uint64_t g = (uint64_t)hi0 ^ ((uint64_t)hi0 >> 1);
         for (int b = 6; g && b < 30; ++b, g >>= 1)
             if (g & 1u)
-                for (int d = 0; d < 4; ++d) vhi[d] ^= dir[d * 30 + b];
+                for (int d = 0; d < 4; ++d) vhi[d] ^= a.dir[d][b];
     }
     for (int j = 0; j < a.spt; ++j) {
         const int64_t hi = hi0 + j;
         if (j > 0) {
             const int b = 6 + __builtin_ctzll((unsigned long long)hi);
             if (b < 30)
-                for (int d = 0; d < 4; ++d) vhi[d] ^= dir[d * 30 + b];
+                for (int d = 0; d < 4; ++d) vhi[d] ^= a.dir[d][b];
         }
         const int64_t sample = (hi << 6) + lane - a.first;
         const bool valid = sample < a.n;
-        const int par = (int)(hi & 1);
+        const bool odd = (hi & 1) != 0;
+        uint32_t q[4];
+        for (int d = 0; d < 4; ++d) q[d] = vlo[d][0] ^ (odd ? (vhi[d] ^ a.dir[d][5]) : vhi[d]);
+        const bool first_group = hi == 0;  // wave-uniform: the wave holding Sobol index 0
+        if (first_group && lane == 0)
+            for (int d = 0; d < 4; ++d) q[d] = a.first_q[d];
         T x[4];
         T jac = T(1);
         uint32_t packed = 0u;
         for (int d = 0; d < 4; ++d) {
-            uint32_t q = vhi[d] ^ vlo[d][par];
-            if (hi == 0 && lane == 0) q = a.first_q[d];
             // bin = min(floor(u B), B - 1), frac = u B - bin with u = q 2^-30:
             // for q < 2^30, 4 q B = bin 2^32 + frac 2^32 (high and low words of
-            // one 32 x 32 product); q = 2^30 (a rounded first point) is the
-            // clamped case bin = B - 1, frac = 1.
-            const uint32_t q4 = q << 2;
-            const bool top = q >= (1u << 30);
-            const uint32_t b = top ? (uint32_t)(a.bins - 1) : __umulhi(q4, (uint32_t)a.bins);
-            const T frac = top ? T(1) : (T)(q4 * (uint32_t)a.bins) * (T)0x1p-32;
+            // one 32 x 32 product); q = 2^30 (only a rounded first point) is
+            // the clamped case bin = B - 1, frac = 1.
+            const uint32_t q4 = q[d] << 2;
+            uint32_t b = __umulhi(q4, (uint32_t)a.bins);
+            T frac = (T)(q4 * (uint32_t)a.bins) * (T)0x1p-32;
+            if (first_group && q[d] >= (1u << 30)) {
+                b = (uint32_t)(a.bins - 1);
+                frac = T(1);
+            }
             if constexpr (MODE == NLO_PREC_FP32) {
                 const float2 e = reinterpret_cast<const float2*>(E)[d * a.bins + b];
                 x[d] = fmaf(e.y, frac, e.x);
@@ -459,9 +494,11 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
                 kf = f.kf * jac;
             }
             s1 += wv;
-            s2 += wv * wv;
-            sk2 += k2;
-            skf += kf;
+            if constexpr (FULL) {
+                s2 += wv * wv;
+                sk2 += k2;
+                skf += kf;
+            }
         }
         if (HIST) {
             st_w[threadIdx.x + j * 256] = valid ? fabs(wv) : 0.0;
@@ -652,12 +689,15 @@ int init_kernel_attributes() {
         const void* fns[] = {
             (const void*)spline_build_kernel,
             (const void*)k1_kernel,
-            (const void*)nlo_kernel<NLO_PREC_FP64, false>,
-            (const void*)nlo_kernel<NLO_PREC_FP64, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP32, false>,
-            (const void*)nlo_kernel<NLO_PREC_FP32, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false>,
-            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP64, false, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP64, false, false>,
+            (const void*)nlo_kernel<NLO_PREC_FP64, true, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP32, false, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP32, false, false>,
+            (const void*)nlo_kernel<NLO_PREC_FP32, true, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false, true>,
+            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false, false>,
+            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, true, true>,
         };
         for (const void* f : fns) {
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess)
@@ -688,7 +728,7 @@ int launch_k1(int R, const double* r, const double* N, const double* t64, const 
     const size_t smem = ((size_t)R + 4 * (size_t)(R - 1)) * sizeof(double);
     int rc = init_kernel_attributes();
     if (rc != NLO_OK) return rc;
-    hipLaunchKernelGGL(k1_kernel, dim3(R), dim3(256), smem, stream, R, r, N, t64, c, g, k1);
+    hipLaunchKernelGGL(k1_kernel, dim3(R), dim3(kK1Threads), smem, stream, R, r, N, t64, c, g, k1);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }
@@ -712,7 +752,7 @@ NloPlan nlo_plan(int R, int bins, int64_t n, bool hist) {
 int launch_nlo(int R, const double* r, const double* N, const double* t64, const float* t32,
                const double* edges, int bins, const nlo_sobol_table& sobol, int64_t first, int64_t n,
                int precision, const BkConsts& c, double* sums, double* hist, double* count,
-               void* ws, size_t ws_bytes, hipStream_t stream) {
+               void* ws, size_t ws_bytes, hipStream_t stream, bool components) {
     const bool want_hist = hist != nullptr;
     const NloPlan pl = nlo_plan(R, bins, n, want_hist);
     if (ws_bytes < pl.workspace || (!ws && pl.workspace)) {
@@ -748,17 +788,19 @@ int launch_nlo(int R, const double* r, const double* N, const double* t64, const
     const dim3 grid(pl.bpp, R - 1);
     int rc = init_kernel_attributes();
     if (rc != NLO_OK) return rc;
-#define NLO_PICK(MODE, H) \
-    hipLaunchKernelGGL((nlo_kernel<MODE, H>), grid, dim3(256), smem, stream, a)
+#define NLO_PICK(MODE)                                                                        \
+    if (want_hist)                                                                            \
+        hipLaunchKernelGGL((nlo_kernel<MODE, true, true>), grid, dim3(256), smem, stream, a);  \
+    else if (components)                                                                      \
+        hipLaunchKernelGGL((nlo_kernel<MODE, false, true>), grid, dim3(256), smem, stream, a); \
+    else                                                                                      \
+        hipLaunchKernelGGL((nlo_kernel<MODE, false, false>), grid, dim3(256), smem, stream, a)
     if (precision == NLO_PREC_FP64) {
-        if (want_hist) NLO_PICK(NLO_PREC_FP64, true);
-        else NLO_PICK(NLO_PREC_FP64, false);
+        NLO_PICK(NLO_PREC_FP64);
     } else if (precision == NLO_PREC_FP32) {
-        if (want_hist) NLO_PICK(NLO_PREC_FP32, true);
-        else NLO_PICK(NLO_PREC_FP32, false);
+        NLO_PICK(NLO_PREC_FP32);
     } else {
-        if (want_hist) NLO_PICK(NLO_PREC_FP64_STABLE, true);
-        else NLO_PICK(NLO_PREC_FP64_STABLE, false);
+        NLO_PICK(NLO_PREC_FP64_STABLE);
     }
 #undef NLO_PICK
     NLO_LAUNCH_CHECK();

@@ -269,3 +269,39 @@ def test_solve_bk_fp32_vs_fp64(dev):
     print(f"max |dN| fp32 vs fp64_stable: {dmax:.3e}")
     assert dmax <= 1e-5
     assert bool(torch.all((t64.N >= 0) & (t64.N <= 1)))
+
+
+def test_largest_workload_shape_one_rk4_step(dev):
+    """The largest configuration's shape (R = 1024 on [1e-7, 1e2], 2^24 Sobol
+    points per parent per right-hand side, classical RK4, dY = 0.05) for one
+    step: FP32 against the FP64 evaluation of the same formula at the full
+    size, where the oracle would take hours (size-independent checks: the
+    step stays in [0, 1] and small, and both precisions agree)."""
+
+    from nlo_bk_optimized_amd import MV
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+    from nlo_bk_optimized_amd.bk import evolution as evo
+
+    base = dict(RPOINTS=1023, MINR=1e-7, MAXR=1e2, MCINTPOINTS=1 << 24, DE_SOLVER_STEP=0.05, RK_METHOD="rk4",
+                VEGAS_ADAPT_ITERATIONS=1, VEGAS_ADAPT_POINTS=1 << 16)
+    ic = MV(qs0sqr=0.2, gamma=1.0)
+    out = {}
+    for precision in ("fp32", "fp64_stable"):
+        config = BKConfig(PRECISION=precision, **base)
+        setup = evo.prepare(ic, config, device=torch.device("cuda"), seed=0)
+        ev = dev.Evolver(setup.r, setup.edges, setup.table, config, n_samples=1 << 24, precision=precision,
+                         rk_method="rk4", step=0.05)
+        ev.state.copy_(setup.N0)
+        ev.step()
+        out[precision] = ev.state.cpu().numpy().copy()
+        assert int(ev.flag.item()) == 0
+        ev.close()
+        N0 = setup.N0.cpu().numpy()
+    a, b = out["fp32"], out["fp64_stable"]
+    assert np.all((a >= 0) & (a <= 1)) and np.all((b >= 0) & (b <= 1))
+    assert np.abs(b - N0).max() < 0.05  # one small step
+    d = np.abs(a - b).max()
+    k = int(np.argmax(np.abs(a - b)))
+    print(f"R=1024, 2^24 points, one RK4 step: max |dN| fp32 vs fp64 = {d:.3e} at r = {setup.r[k].item():.3e}, "
+          f"N = {b[k]:.6f}, step dN = {b[k] - N0[k]:.3e}")
+    assert d <= 1e-5
