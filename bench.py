@@ -36,6 +36,22 @@ METRIC = "NLO-BK evolution wall time + integrand samples/s, FP32 vs FP64 at fixe
 FLOPS_PER_SAMPLE = 263
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per launch of the FP32 K2 + Kf kernel on this workload, from the
+# FETCH_SIZE / WRITE_SIZE counter runs calibrated on a device copy.
+COUNTERS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "nlo_kernel_fp32_256r_counters.md")
+
+
+def measured_traffic(rpoints: int, samples_log2: int) -> float | None:
+    """traffic_bytes_per_launch from the counter summary (default workload only)."""
+
+    if rpoints != 255 or samples_log2 != 20 or not os.path.exists(COUNTERS):
+        return None
+    with open(COUNTERS) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def parse() -> argparse.Namespace:
@@ -50,6 +66,7 @@ def parse() -> argparse.Namespace:
     p.add_argument("--kernel-launches", type=int, default=5)
     p.add_argument("--cpu-samples-log2", type=int, default=20,
                    help="Sobol points per parent in the CPU baseline sample")
+    p.add_argument("--no-dis", action="store_true", help="skip the DIS VEGAS-pass measurements")
     p.add_argument("--kernel-probe", action="store_true",
                    help="only run the FP32 K2+Kf launch and a 256 MiB device copy (for counter runs)")
     return p.parse_args()
@@ -173,6 +190,34 @@ def main() -> int:
             "max_abs_dN_fp64_formulas": float((fp64s["N"] - fp64["N"]).abs().max()),
         }
 
+    if not args.no_dis:
+        # DIS: one VEGAS pass (in-kernel Sobol, map, integrand, FP64 sums and
+        # histogram) of the qqbar-g terms at the reference's DIS benchmark point
+        # (charm, GBW Qs0^2 = 1, Q^2 = 9, x = 1e-3, maxr = 5, 32-bin grids).
+        from nlo_bk_optimized_amd.dipole import GBW
+        from nlo_bk_optimized_amd.dis.config import DISConfig, Quark, QuarkType
+
+        gbw = GBW(Qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0)
+        dcfg = DISConfig(quarks=(Quark(QuarkType.C, 1.4),), maxr=5.0, nf_alphas=1)
+        desc = dev.dis_desc(9.0, 1e-3, True, dcfg.quarks, gbw, dcfg)
+        nd = 1 << 22
+        for kind, dim in (("I1", 5), ("I2_nested", 5), ("I3_nested", 5)):
+            edges_d = torch.linspace(0, 1, 33, dtype=torch.float64, device=device).expand(dim, -1).contiguous()
+            table_d = dev.sobol_table(dim, 0)
+            for prec in ("fp32", "fp64"):
+                dev.dis_vegas(desc, kind, edges_d, table_d, nd, precision=prec)
+                st = torch.cuda.current_stream()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(st)
+                reps = 3
+                for _ in range(reps):
+                    dev.dis_vegas(desc, kind, edges_d, table_d, nd, precision=prec)
+                e1.record(st)
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                extra[f"dis_{kind}_T_{prec}_samples_per_s"] = nd / (ms * 1e-3)
+                log(f"dis {kind} T {prec}: {ms:.3f} ms per 2^22-sample VEGAS pass")
+
     cpu = None
     if not args.no_cpu_baseline:
         # The FP64 oracle (C restatement of the reference CPU algorithm) on a
@@ -240,7 +285,7 @@ def main() -> int:
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(args.rpoints, args.samples_log2),
             "kernel": "nlo_kernel<FP32> + per-parent reduce",
             "launch_ms": launch_ms,
             "flops_per_sample": FLOPS_PER_SAMPLE,

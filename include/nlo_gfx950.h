@@ -249,6 +249,8 @@ int nlo_dis_special(int32_t fn, int64_t n, const double* d_x, int32_t precision,
 typedef struct nlo_dis_dipole {
     int32_t kind;
     int32_t initialized;
+    int32_t bilinear; /* BK table: AmplitudeInterpolation.LINEAR_LINEAR (else SPLINE_LINEAR) */
+    int32_t reserved;
     double qs0sqr, lambda, gamma, x0; /* GBW */
     int32_t R, T;
     const double* d_r;

@@ -83,6 +83,8 @@ class DisDipole(ctypes.Structure):
     _fields_ = [
         ("kind", ctypes.c_int32),
         ("initialized", ctypes.c_int32),
+        ("bilinear", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
         ("qs0sqr", ctypes.c_double),
         ("lambda_", ctypes.c_double),
         ("gamma", ctypes.c_double),

@@ -1,39 +1,46 @@
 // J1 and I1 in FP64 for the double-log resummation of K1
 // (S/bk_kernels/k1.py:108-132 uses torch.special.bessel_j1 / modified_bessel_i1).
 // Both return f(x) / (x / 2) so that the caller's J1(2a)/a and I1(2a)/a are
-// direct; written for x in [0, 200] (the resummation arguments stay below ~80).
-//   I1: the ascending series sum (x^2/4)^k / (k! (k+1)!), all terms positive.
-//   J1: x < 2, the alternating series (at most 2x smaller than its first
-//       term); otherwise Miller's backward recurrence
-//       J_{k-1} = (2k / x) J_k - J_{k+1} from k = N ~ x + 32, normalised by
-//       J_0 + 2 sum_k J_2k = 1 (Abramowitz & Stegun 9.1.27 and 9.1.46).
+// direct.
+//   J1, x < 128: the piecewise Chebyshev table of bessel_j1_i1_tables.inc
+//       (pieces of width 4, degree 16; the caller passes the table, staged in
+//       LDS); beyond: Miller's backward recurrence J_{k-1} = (2k / x) J_k -
+//       J_{k+1} from k = N ~ 1.25 x + 32, normalised by J_0 + 2 sum_k J_2k = 1
+//       (Abramowitz & Stegun 9.1.27 and 9.1.46).
+//   I1, x < 8: the ascending series sum (x^2/4)^k / (k! (k+1)!), all terms
+//       positive; x >= 8: e^x / sqrt(x) times the Chebyshev series of
+//       e^-x sqrt(x) I1(x) in 16/x - 1.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace nlo {
 
-__host__ __device__ inline double i1_over_half_x(double x) {
-    const double t = 0.25 * x * x;
-    double term = 1.0, sum = 1.0;
-    for (int k = 1; k < 400; ++k) {
-        term *= t / ((double)k * (double)(k + 1));
-        sum += term;
-        if (term < 1e-17 * sum) break;
-    }
-    return sum;
-}
+#include "bessel_j1_i1_tables.inc"
 
-__host__ __device__ inline double j1_over_half_x(double x) {
-    if (x < 2.0) {
-        const double t = -0.25 * x * x;
+__host__ __device__ inline double i1_over_half_x(double x) {
+    if (x < 8.0) {
+        const double t = 0.25 * x * x;
         double term = 1.0, sum = 1.0;
-        for (int k = 1; k < 30; ++k) {
+        for (int k = 1; k < 40; ++k) {
             term *= t / ((double)k * (double)(k + 1));
             sum += term;
+            if (term < 1e-17 * sum) break;
         }
         return sum;
     }
+    const double s = 16.0 / x - 1.0, s2 = s + s;
+    double b0 = 0.0, b1 = 0.0;
+    for (int m = kI1Deg; m >= 1; --m) {
+        const double t = s2 * b0 - b1 + kI1Cheb[m];
+        b1 = b0;
+        b0 = t;
+    }
+    const double g = s * b0 - b1 + kI1Cheb[0];
+    return g * exp(x) / sqrt(x) / (0.5 * x);
+}
+
+__host__ __device__ inline __attribute__((noinline)) double j1_miller_over_half_x(double x) {
     const int N = 2 * ((int)(0.625 * x) + 16);  // even, > 1.25 x + 30
     const double two_over_x = 2.0 / x;
     double jp = 0.0, j = 1e-30;  // J_{k+1}, J_k (unnormalised)
@@ -42,7 +49,7 @@ __host__ __device__ inline double j1_over_half_x(double x) {
         const double jm = (double)k * two_over_x * j - jp;  // J_{k-1}
         jp = j;
         j = jm;
-        if (k == 2) j1 = j;                       // J_1, produced by the step from k = 2
+        if (k == 2) j1 = j;                                    // J_1, produced by the step from k = 2
         if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * jm;  // even order >= 2
         if (fabs(j) > 1e250) {  // rescale to stay in range
             j *= 1e-250;
@@ -53,6 +60,21 @@ __host__ __device__ inline double j1_over_half_x(double x) {
     }
     norm += j;  // J_0
     return (j1 / norm) / (0.5 * x);
+}
+
+// table: kJ1Cheb (kJ1Pieces x (kJ1Deg + 1) doubles), e.g. staged in LDS.
+__host__ __device__ inline double j1_over_half_x(double x, const double* table) {
+    if (!(x < 4.0 * kJ1Pieces)) return j1_miller_over_half_x(x);
+    const int k = (int)(0.25 * x);
+    const double s = 0.5 * (x - 4.0 * k) - 1.0, s2 = s + s;
+    const double* c = table + k * (kJ1Deg + 1);
+    double b0 = 0.0, b1 = 0.0;
+    for (int m = kJ1Deg; m >= 1; --m) {
+        const double t = s2 * b0 - b1 + c[m];
+        b1 = b0;
+        b0 = t;
+    }
+    return s * b0 - b1 + c[0];
 }
 
 }  // namespace nlo

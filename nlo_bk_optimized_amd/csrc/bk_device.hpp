@@ -109,11 +109,13 @@ __device__ __forceinline__ T loglog_from_log(const LdsSpline<T>& s, T log_d, T u
 // X accurate near X -> 0).  Logarithms of distances are formed once and
 // shared by the spline, the coupling and the resummation logs.
 
-// alpha(d) from ln d (S/coupling.py:81-97): ln(4 C2 / (d^2 Lambda^2)) = log_scale0 - 2 ln d.
-__device__ __forceinline__ double alpha_s64_log(double log_d, const BkConsts& c) {
+// alpha(d) from ln d (S/coupling.py:81-97): ln(4 C2 / (d^2 Lambda^2)) =
+// log_scale0 - 2 ln d.  The denominator is returned too, so that ratios of
+// couplings need no division: alpha_a / alpha_b = alpha_a * den_b.
+__device__ __forceinline__ double alpha_s64_den(double log_d, const BkConsts& c) {
     const double right = (c.log_scale0 - 2.0 * log_d) / 0.2;
     const double hi = fmax(c.log_mu0, right);
-    return 1.0 / (c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right))))));
+    return c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right)))));
 }
 
 struct K1Parent {
@@ -122,10 +124,11 @@ struct K1Parent {
     double fin_coef;    // alpha(r) Nc / (4 pi) (67/9 - pi^2/3 - 10/9 Nf/Nc)
     double log_ksub;
     double resum_on;    // 1 where r > 1.01 MINR (double-log resummation active)
+    const double* j1tab;  // J1 Chebyshev pieces (LDS)
 };
 
 struct K1Z {
-    double z, log_z, n_y, a_y, ly;
+    double z, log_z, n_y, a_y, den_y, ly, inv_y2;
 };
 
 __device__ __forceinline__ double k1_theta_term(const K1Parent& P, const K1Z& Z, double s2,
@@ -137,23 +140,27 @@ __device__ __forceinline__ double k1_theta_term(const K1Parent& P, const K1Z& Z,
     const double lnX = log(X);
     const double n_x = fmin(loglog_from_log(sN, lnX, 0.0, 1.0), 1.0);
     const double dipole = n_x + Z.n_y - P.n_r - n_x * Z.n_y;
-    const double a_x = alpha_s64_log(lnX, c);
+    const double den_x = alpha_s64_den(lnX, c);
+    const double a_x = 1.0 / den_x;
     const double Y = z;
     const double a_min = fmin(X, Y) <= r ? (X <= Y ? a_x : Z.a_y) : P.a_r;
-    double lo = P.lo_coef * (r * r / (xsq * Y * Y) + (Z.a_y / a_x - 1.0) / (Y * Y) + (a_x / Z.a_y - 1.0) / xsq);
+    const double inv_x2 = 1.0 / xsq;
+    const double rxy2 = r * r * inv_x2 * Z.inv_y2;  // r^2 / (X^2 Y^2)
+    // Kernel_lo: alpha_r (r^2/(X^2 Y^2) + (alpha_Y/alpha_X - 1)/Y^2 + (alpha_X/alpha_Y - 1)/X^2)
+    double lo = P.lo_coef * (rxy2 + (Z.a_y * den_x - 1.0) * Z.inv_y2 + (a_x * Z.den_y - 1.0) * inv_x2);
     lo = isfinite(lo) ? lo : 0.0;
     const double xl = 4.0 * (lnX - P.log_r) * Z.ly;
     const double abar = a_min * c.nc / kPi;
     const double as_x = sqrt(abar * fabs(xl));
     double dlog = 1.0;
     // J1(2a)/a or I1(2a)/a (k1.py:108-132)
-    if (P.resum_on != 0.0 && as_x != 0.0) dlog = xl >= 0.0 ? j1_over_half_x(2.0 * as_x) : i1_over_half_x(2.0 * as_x);
+    if (P.resum_on != 0.0 && as_x != 0.0)
+        dlog = xl >= 0.0 ? j1_over_half_x(2.0 * as_x, P.j1tab) : i1_over_half_x(2.0 * as_x);
     if (!isfinite(dlog)) return 0.0;
     // ln(KSUB r^2 / min(X, Y)^2): the single-log exponent of both slog terms
     const double lmin = X <= Y ? lnX : Z.log_z;
     const double slog_exp = -abar * (11.0 / 12.0) * fabs(P.log_ksub + 2.0 * (P.log_r - lmin));
     const double slog = exp(slog_exp);
-    const double rxy2 = r * r / (xsq * Y * Y);
     const double lo_single = P.lo_coef * rxy2;
     const double k1fin = lo * P.fin_coef;
     return (dlog * slog * lo - lo_single * slog_exp + k1fin) * dipole;

@@ -62,16 +62,21 @@ __device__ void stage_spline(const T* table, int R, int row, T* x, T* rec, LdsSp
 // -------------------------------------------------------------- splines
 
 // S/numerics/interpolation.py:27-48, :124-149 and S/bk/evolution.py:548-560:
-// one wave; lanes 0 and 1 run the Thomas sweeps of the N and S rows.
+// one wave.  The tridiagonal system of the natural spline depends on the
+// grid only, so lane 0 factorises it once (u_i and 1/den_i, the only
+// divisions of the sweep) while the right-hand sides of both rows are formed
+// in parallel; lanes 0 and 1 then run the division-free forward and back
+// substitutions of the N and S rows.
 __global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* __restrict__ r,
                                                           const double* __restrict__ N,
                                                           int force_positive, double* __restrict__ t64,
                                                           float* __restrict__ t32) {
     extern __shared__ double sh[];
     double* x = sh;
-    double* ly = x + R;       // 2 R
-    double* up = ly + 2 * R;  // 2 R
-    double* so = up + 2 * R;  // 2 R
+    double* ly = x + R;        // 2 R
+    double* up = ly + 2 * R;   // R: u_i
+    double* inv = up + R;      // R: 1 / den_i
+    double* so = inv + R;      // 2 R: right-hand sides, then solutions
     for (int i = threadIdx.x; i < R; i += blockDim.x) {
         x[i] = log(r[i]);
         double v = N[i] > 1.0 ? 1.0 : N[i];
@@ -84,25 +89,27 @@ __global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* _
     }
     __syncthreads();
     const int m = R - 2;
-    if (threadIdx.x < 2) {
-        const double* y = ly + threadIdx.x * R;
-        double* u = up + threadIdx.x * R;
-        double* sol = so + threadIdx.x * R;
-        if (m == 1) {
-            sol[0] = 6.0 * ((y[2] - y[1]) / (x[2] - x[1]) - (y[1] - y[0]) / (x[1] - x[0])) /
-                     (2.0 * ((x[1] - x[0]) + (x[2] - x[1])));
-        } else if (m > 1) {
-            double h0 = x[1] - x[0], h1 = x[2] - x[1];
-            u[0] = h1 / (2.0 * (h0 + h1));
-            sol[0] = 6.0 * ((y[2] - y[1]) / h1 - (y[1] - y[0]) / h0) / (2.0 * (h0 + h1));
-            for (int i = 1; i < m; ++i) {
-                const double hi = x[i + 1] - x[i], hi1 = x[i + 2] - x[i + 1];
-                const double den = 2.0 * (hi + hi1) - hi * u[i - 1];
-                if (i < m - 1) u[i] = hi1 / den;
-                sol[i] = (6.0 * ((y[i + 2] - y[i + 1]) / hi1 - (y[i + 1] - y[i]) / hi) - hi * sol[i - 1]) / den;
-            }
-            for (int i = m - 2; i >= 0; --i) sol[i] = sol[i] - u[i] * sol[i + 1];
+    for (int item = threadIdx.x; item < 2 * m; item += blockDim.x) {
+        const int row = item / m, i = item - row * m;
+        const double* y = ly + row * R;
+        const double hi = x[i + 1] - x[i], hi1 = x[i + 2] - x[i + 1];
+        so[row * R + i] = 6.0 * ((y[i + 2] - y[i + 1]) / hi1 - (y[i + 1] - y[i]) / hi);
+    }
+    if (threadIdx.x == 0 && m >= 1) {
+        for (int i = 0; i < m; ++i) {
+            const double hi = x[i + 1] - x[i], hi1 = x[i + 2] - x[i + 1];
+            const double den = 2.0 * (hi + hi1) - (i > 0 ? hi * up[i - 1] : 0.0);
+            const double iv = 1.0 / den;
+            inv[i] = iv;
+            up[i] = hi1 * iv;
         }
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && m >= 1) {
+        double* sol = so + threadIdx.x * R;
+        sol[0] = sol[0] * inv[0];
+        for (int i = 1; i < m; ++i) sol[i] = (sol[i] - (x[i + 1] - x[i]) * sol[i - 1]) * inv[i];
+        for (int i = m - 2; i >= 0; --i) sol[i] = sol[i] - up[i] * sol[i + 1];
     }
     __syncthreads();
     const int iv = R - 1;
@@ -179,11 +186,13 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
     extern __shared__ double sh[];
     __shared__ double red[kK1Threads / 64];
     __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], s2t[kMaxGaussT], wtt[kMaxGaussT];
+    __shared__ double j1tab[kJ1Pieces * (kJ1Deg + 1)];
     const int p = blockIdx.x;
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
         if (threadIdx.x == 0) k1[p] = 0.0;
         return;
     }
+    for (int i = threadIdx.x; i < kJ1Pieces * (kJ1Deg + 1); i += blockDim.x) j1tab[i] = kJ1Cheb[i];
     LdsSpline<double> s;
     stage_spline<double>(t64, R, 0, sh, sh + R, s);
     for (int i = threadIdx.x; i < g.nr; i += blockDim.x) {
@@ -207,6 +216,7 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
     P.fin_coef = P.a_r * c.nc / (4.0 * kPi) * (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
     P.log_ksub = log(c.ksub);
     P.resum_on = P.r > 1.01 * c.minr ? 1.0 : 0.0;
+    P.j1tab = j1tab;
     const int znodes = (R - 1) * g.nr;
     double total = 0.0;
     for (int zi = threadIdx.x; zi < znodes; zi += blockDim.x) {
@@ -218,8 +228,10 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
         Z.log_z = center + half * gxr[i];
         Z.z = exp(Z.log_z);
         Z.n_y = fmin(loglog_from_log(s, Z.log_z, 0.0, 1.0), 1.0);
-        Z.a_y = alpha_s64_log(Z.log_z, c);
+        Z.den_y = alpha_s64_den(Z.log_z, c);
+        Z.a_y = 1.0 / Z.den_y;
         Z.ly = Z.log_z - P.log_r;
+        Z.inv_y2 = 1.0 / (Z.z * Z.z);
         double acc = 0.0;
         for (int j = 0; j < g.nt; ++j) acc += k1_theta_term(P, Z, s2t[j], s, c) * wtt[j];
         total += acc * ((2.0 * Z.z * Z.z) * (half * gwr[i]));

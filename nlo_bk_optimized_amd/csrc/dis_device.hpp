@@ -176,6 +176,7 @@ struct DisDipole {
     int initialized;
     double y_init, maxr_interpolate;
     const double* init_coef;
+    int bilinear;  // AmplitudeInterpolation.LINEAR_LINEAR
 };
 
 // GBW: 1 - exp(-(r^2 Qs^2(Y))^gamma / 4) (as arg below 1e-7), Qs^2 =
@@ -254,6 +255,26 @@ __device__ __forceinline__ double bk_amplitude(double r, double Y, const DisDipo
         const double* c = d.init_coef + 4 * k;
         v = c[0] + t * (c[1] + t * (c[2] + t * c[3]));
         if (d.maxr_interpolate > 0.0 && r >= d.maxr_interpolate) v = 1.0;
+    } else if (d.bilinear) {
+        // _bilinear_amplitude (S/dipole/amplitude.py:176-202)
+        int rind = search_right(d.r, d.R, r);
+        int yind = search_right(d.Y, d.T, Y);
+        rind = rind < 0 ? 0 : rind;
+        yind = yind < 0 ? 0 : yind;
+        const int rind2 = rind + 1 < d.R ? rind + 1 : d.R - 1;
+        const int yind2 = yind + 1 < d.T ? yind + 1 : d.T - 1;
+        const double r1 = d.r[rind], r2 = d.r[rind2], Y1 = d.Y[yind], Y2 = d.Y[yind2];
+        const double dr = rind2 != rind ? r2 - r1 : 1.0, dY = yind2 != yind ? Y2 - Y1 : 1.0;
+        const double N11 = d.N[(size_t)yind * d.R + rind], N12 = d.N[(size_t)yind * d.R + rind2];
+        const double N21 = d.N[(size_t)yind2 * d.R + rind], N22 = d.N[(size_t)yind2 * d.R + rind2];
+        if (rind2 == rind) {
+            v = 1.0;
+        } else if (yind2 == yind) {
+            v = N11 + (r - r1) / dr * (N12 - N11);
+        } else {
+            v = (N11 * (r2 - r) * (Y2 - Y) + N12 * (r - r1) * (Y2 - Y) + N21 * (r2 - r) * (Y - Y1) +
+                 N22 * (r - r1) * (Y - Y1)) / (dr * dY);
+        }
     } else {
         const int rind = search_right(d.r, d.R, r);
         const int yind = search_right(d.Y, d.T, Y);

@@ -563,6 +563,7 @@ DisDipole make_dipole(const nlo_dis_dipole& d) {
     o.y_init = d.y_init;
     o.maxr_interpolate = d.maxr_interpolate;
     o.init_coef = d.d_init_coef;
+    o.bilinear = d.bilinear;
     return o;
 }
 

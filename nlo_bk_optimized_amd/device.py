@@ -343,6 +343,7 @@ def dis_dipole_struct(dipole) -> _lib.DisDipole:
     d.R, d.T = dipole.device_r.numel(), dipole.device_Y.numel()
     d.d_r, d.d_Y, d.d_N = ptr(dipole.device_r), ptr(dipole.device_Y), ptr(dipole.device_N)
     d.x0 = 1.0
+    d.bilinear = 1 if dipole.bilinear else 0
     if dipole.device_coef is not None:
         d.initialized = 1
         d.y_init = dipole.interpolator_Y

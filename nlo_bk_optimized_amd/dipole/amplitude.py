@@ -74,7 +74,8 @@ class BKDipole:
 
     The table is held on the device in FP64.  Until ``initialize_interpolation``
     is called every query uses the local 7-point (9 at the top of the grid)
-    natural spline in r of the Y-linear table rows; afterwards queries whose Y
+    natural spline in r of the Y-linear table rows (SPLINE_LINEAR) or bilinear
+    interpolation in r and Y (LINEAR_LINEAR); afterwards queries whose Y
     matches the initialised rapidity (BKDipole.interpolator_initialized) use
     the global spline at that Y, set to 1 beyond maxr_interpolate.
     """
@@ -91,10 +92,9 @@ class BKDipole:
                                                                 device=table.r.device))
         if not bool((torch.abs(expected / table.r - 1) <= 0.01).all()):
             raise ValueError("BKDipole requires a logarithmic r grid")
-        if interpolation is not AmplitudeInterpolation.SPLINE_LINEAR:
-            raise NotImplementedError("only AmplitudeInterpolation.SPLINE_LINEAR is implemented on the device")
         self.table = table
         self.interpolation = interpolation
+        self.bilinear = interpolation is AmplitudeInterpolation.LINEAR_LINEAR
         self.device = torch.device(device) if device is not None else (
             table.r.device if table.r.is_cuda else torch.device("cuda"))
         y0 = max(0.0, math.log(1 / table.x0))

@@ -159,10 +159,30 @@ def natural_spline_eval(x_nodes, y_nodes, x):
 class BKDipoleOracle:
     """N(r, Y) from a BK table (S/dipole/amplitude.py:49-270)."""
 
-    def __init__(self, r, y, N, x0):
+    def __init__(self, r, y, N, x0, bilinear=False):
         self.r = np.asarray(r, dtype=np.float64)
         self.N = np.asarray(N, dtype=np.float64)
         self.Y = np.asarray(y, dtype=np.float64) + max(0.0, math.log(1 / x0))
+        self.bilinear = bilinear
+
+    def _bilinear(self, r, Y):
+        """Bilinear interpolation in r and Y (:176-202)."""
+
+        R, T = self.r.size, self.Y.size
+        ri = np.searchsorted(self.r, r, side="right") - 1
+        yi = np.searchsorted(self.Y, Y, side="right") - 1
+        ri2 = np.minimum(ri + 1, R - 1)
+        yi2 = np.minimum(yi + 1, T - 1)
+        r1, r2, Y1, Y2 = self.r[ri], self.r[ri2], self.Y[yi], self.Y[yi2]
+        dr = np.where(ri2 != ri, r2 - r1, 1.0)
+        dY = np.where(yi2 != yi, Y2 - Y1, 1.0)
+        N11, N12 = self.N[yi, ri], self.N[yi, ri2]
+        N21, N22 = self.N[yi2, ri], self.N[yi2, ri2]
+        v = (N11 * (r2 - r) * (Y2 - Y) + N12 * (r - r1) * (Y2 - Y) + N21 * (r2 - r) * (Y - Y1)
+             + N22 * (r - r1) * (Y - Y1)) / (dr * dY)
+        last = N11 + (r - r1) / dr * (N12 - N11)
+        v = np.where(yi2 == yi, last, v)
+        return np.where(ri2 == ri, 1.0, v)
 
     def _local(self, r, Y):
         """Local natural spline over 7 r nodes (9 at the top) of the Y-linear
@@ -198,7 +218,7 @@ class BKDipoleOracle:
         Y = np.where((Y >= 0) & (Y < self.Y[0]), self.Y[0], Y)
         if np.any((Y < self.Y[0]) | (Y > self.Y[-1])):
             raise ValueError("Y outside the table")
-        return np.clip(self._local(r, Y), 0.0, 1.0)
+        return np.clip(self._bilinear(r, Y) if self.bilinear else self._local(r, Y), 0.0, 1.0)
 
     def initialized(self, Y):
         """Initialised path at one Y (:111-131, :170-175, :242-260): a natural spline

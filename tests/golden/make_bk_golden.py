@@ -14,7 +14,15 @@ FP64 and evaluates only functions that no closed form or library routine pins:
 * ``_adapt_vegas_edges`` (S/numerics/integration.py:804-866, eager branch);
 * ``MV.dipole_amplitude`` (S/bk/initial_conditions.py:29-35) and the
   evolution splines (S/bk/evolution.py:536-560), as the inputs the composites
-  above consume.
+  above consume;
+* ``Kernel_lo``, ``_lo_integrand`` and ``_nlo_integrand`` for the other
+  couplings, orders and NF (S/bk_kernels/k1.py:37-150,
+  S/bk/derivatives.py:596-632) and ``bk_alpha_s`` with the NF = 5 flavour
+  thresholds (S/coupling.py:11-46);
+* ``_interpolate_history_N`` and the kinematically constrained
+  ``_lo_integrand`` (S/bk/derivatives.py:445-468, :712-758);
+* ``_adaptive_rk2_to`` (S/bk/evolution.py:239-326) driven by a closed-form
+  right-hand side.
 
 Usage: ``python tests/golden/make_bk_golden.py`` (rewrites the .npz).
 """
@@ -178,8 +186,181 @@ def main() -> None:
         out[f"adapt{bins}_count"] = count
         out[f"adapt{bins}_new"] = new.numpy()
 
+    physics_options(out, rng, r, N, interp_N, interp_S)
+    kinematical(out, rng, r)
+    adaptive_rk2(out)
+
     np.savez_compressed(OUT, **out)
     print(f"wrote {OUT} ({OUT.stat().st_size} bytes, {len(out)} arrays)")
+
+
+# (name, RC_LO, RC_NLO, RESUM_RC, Order, NF, FIXED_AS, C2, KSUB) of the
+# non-production physics variants; the row order fixes the vector names.
+VARIANTS = (
+    ("parent_dls", "PARENT_LO", "PARENT_NLO", "RESUM_RC_PARENT", "NLO_RESUM_DLOG_SLOG", 3, 0.2, 1.0, 0.65),
+    ("smallest_lod", "SMALLEST_LO", "SMALLEST_NLO", "RESUM_RC_SMALLEST", "LO_RESUM_DLOG", 3, 0.2, 2.0, 0.65),
+    ("fac_nlod", "FAC_LO", "SMALLEST_NLO", "RESUM_RC_FIXED", "NLO_RESUM_DLOG", 3, 0.25, 1.0, 0.65),
+    ("beuf_dls", "BEUF_LO", "SMALLEST_NLO", "RESUM_RC_SMALLEST", "NLO_RESUM_DLOG_SLOG", 3, 0.2, 1.0, 0.4),
+    ("fixed_dls", "FIXED_LO", "FIXED_NLO", "RESUM_RC_PARENT", "NLO_RESUM_DLOG_SLOG", 3, 0.21, 1.0, 0.65),
+    ("nf5_dls", "BALITSKY_LO", "SMALLEST_NLO", "RESUM_RC_SMALLEST", "NLO_RESUM_DLOG_SLOG", 5, 0.2, 1.0, 0.65),
+    ("nf0_lo", "BALITSKY_LO", "PARENT_NLO", "RESUM_RC_SMALLEST", "LO", 0, 0.2, 1.0, 0.65),
+    ("bal_nlo", "BALITSKY_LO", "SMALLEST_NLO", "RESUM_RC_PARENT", "NLO", 3, 0.2, 1.0, 0.65),
+    ("bal_lodls_nf5", "BALITSKY_LO", "PARENT_NLO", "RESUM_RC_PARENT", "LO_RESUM_DLOG_SLOG", 5, 0.2, 1.5, 0.65),
+)
+
+
+def variant_config(v, **extra):
+    from nlo_torch.bk.config import (
+        BKConfig,
+        BKOrder,
+        ResummationCoupling,
+        RunningCouplingLO,
+        RunningCouplingNLO,
+    )
+
+    _, lo, nlo, resum, order, nf, fixed_as, c2, ksub = v
+    return BKConfig(RPOINTS=63, MINR=1e-6, MAXR=1e2, CUDA_FUSION=False, RC_LO=RunningCouplingLO[lo],
+                    RC_NLO=RunningCouplingNLO[nlo], RESUM_RC=ResummationCoupling[resum],
+                    Order=BKOrder[order], NF=nf, FIXED_AS=fixed_as, C2=c2, KSUB=ksub, **extra)
+
+
+def physics_options(out, rng, r, N, interp_N, interp_S):
+    """Kernel_lo, _lo_integrand, _nlo_integrand and bk_alpha_s for every
+    coupling / order / NF choice other than the production one, with the
+    accurate J1 (scipy) in the double-log resummation."""
+
+    import scipy.special
+
+    import nlo_torch.bk_kernels.k1 as k1_module
+    from nlo_torch.bk.derivatives import _lo_integrand, _nlo_integrand
+    from nlo_torch.coupling import bk_alpha_s
+
+    # columns: RC_LO, RC_NLO, RESUM_RC, Order (as the NLO_RC_*, NLO_RESUM_RC_*,
+    # NLO_ORDER_* codes of the C interface), NF, FIXED_AS, C2, KSUB
+    out["var_table"] = np.array([[
+        ["BALITSKY_LO", "FIXED_LO", "PARENT_LO", "SMALLEST_LO", "FAC_LO", "BEUF_LO"].index(v[1]),
+        ["SMALLEST_NLO", "FIXED_NLO", "PARENT_NLO"].index(v[2]),
+        ["RESUM_RC_SMALLEST", "RESUM_RC_PARENT", "RESUM_RC_FIXED"].index(v[3]),
+        ["NLO_RESUM_DLOG_SLOG", "NLO_RESUM_DLOG", "NLO", "LO_RESUM_DLOG_SLOG", "LO_RESUM_DLOG", "LO"].index(v[4]),
+        v[5], v[6], v[7], v[8]] for v in VARIANTS], dtype=np.float64)
+
+    # couplings: NF = 5 thresholds (both sides of 1.3 and 4.5 GeV, the IR
+    # cap) and NF = 0, C2 = 2
+    ra = np.concatenate([np.exp(rng.uniform(math.log(1e-4), math.log(40.0), 200)),
+                         2.0 / np.array([1.3, 4.5]) * (1 + np.array([[-1e-9], [1e-9]])).ravel()])
+    out["alpha_r"] = ra
+    out["alpha_nf5"] = bk_alpha_s(torch.tensor(ra), NF=5).numpy()
+    out["alpha_nf5_c2"] = bk_alpha_s(torch.tensor(ra), NF=5, C2=2.0).numpy()
+    out["alpha_nf0_c2"] = bk_alpha_s(torch.tensor(ra), NF=0, C2=2.0).numpy()
+
+    n = 200
+    rr = np.exp(rng.uniform(math.log(2e-6), math.log(80.0), n))
+    zz = np.exp(rng.uniform(math.log(1e-6), math.log(100.0), n))
+    th = rng.uniform(0.0, math.pi, n)
+    rr[:2] = [0.5, 1.005e-6]
+    zz[:2] = [0.3, 2e-3]
+    out["var_klo_in"] = np.stack((rr, zz, th))
+    k = 200
+    lo_r = math.log(0.5 * r[0].item())
+    hi_r = math.log(2.0 * r[-1].item())
+    nlo_in = np.stack((r.numpy()[rng.integers(1, r.numel(), k)], rng.uniform(lo_r, hi_r, k),
+                       rng.uniform(lo_r, hi_r, k), rng.uniform(0.0, 2 * math.pi, k),
+                       rng.uniform(0.0, 2 * math.pi, k)))
+    out["var_nlo_in"] = nlo_in
+
+    raw_j1 = k1_module.bessel_J1
+    k1_module.bessel_J1 = lambda x: torch.from_numpy(scipy.special.j1(x.numpy()))
+    try:
+        for v in VARIANTS:
+            cfg = variant_config(v)
+            name = v[0]
+            out[f"var_{name}_klo"] = k1_module.Kernel_lo(
+                torch.tensor(rr), torch.tensor(zz), torch.tensor(th), cfg).numpy()
+            out[f"var_{name}_lo_integrand"] = np.array([
+                _lo_integrand(torch.tensor(a), torch.tensor([b]), torch.tensor([c]), interp_N, r, cfg,
+                              None, None, None).item() for a, b, c in zip(rr, zz, th)])
+            vals = []
+            for a, b, c, d, e in nlo_in.T:
+                vv = _nlo_integrand(torch.tensor(a), torch.tensor([math.exp(b)]), torch.tensor([d]),
+                                    torch.tensor([math.exp(c)]), torch.tensor([e]), interp_S, r, cfg)
+                vals.append(vv.item() * math.exp(2 * b + 2 * c))
+            out[f"var_{name}_nlo"] = np.array(vals)
+    finally:
+        k1_module.bessel_J1 = raw_j1
+
+
+def kinematical(out, rng, r):
+    """_interpolate_history_N and the kinematically constrained
+    _lo_integrand (S/bk/derivatives.py:445-468, :712-758) on a synthetic
+    history: MV amplitudes with growing saturation scale at y = 0, 0.2, ..."""
+
+    from nlo_torch.bk.config import BKConfig
+    from nlo_torch.bk.derivatives import _interpolate_history_N, _lo_integrand
+    from nlo_torch.bk.evolution import _evolution_interpolators
+    from nlo_torch.bk.initial_conditions import MV
+
+    rows = 6
+    hy = torch.tensor([0.2 * k for k in range(rows)])
+    hN = torch.stack([MV(qs0sqr=0.2 * math.exp(0.35 * k * 0.2), x0=0.01, ec=1.0, lambdaqcd=0.241,
+                         gamma=1.0).dipole_amplitude(r) for k in range(rows)])
+    out["kin_hist_y"] = hy.numpy()
+    out["kin_hist_N"] = hN.numpy()
+    cfg = BKConfig(RPOINTS=63, MINR=1e-6, MAXR=1e2, CUDA_FUSION=False, KINEMATICAL_CONSTRAINT=True,
+                   EULER_METHOD=True)
+    interp_N, _ = _evolution_interpolators(hN[-1], r, cfg)
+    # history lookups: rapidities on and between rows, r on and off the grid
+    m = 300
+    qr = np.exp(rng.uniform(math.log(3e-7), math.log(300.0), m))
+    qy = rng.uniform(0.0, float(hy[-1]), m)
+    qy[:20] = hy.numpy()[rng.integers(0, rows, 20)]
+    qr[20:30] = r.numpy()[rng.integers(0, r.numel(), 10)]
+    qy[-5:] = float(hy[-1])
+    out["kin_q_r"], out["kin_q_y"] = qr, qy
+    out["kin_q_N"] = _interpolate_history_N(torch.tensor(qr), torch.tensor(qy), interp_N, r, hy,
+                                            hN).numpy()
+    n = 300
+    rr = np.exp(rng.uniform(math.log(2e-6), math.log(80.0), n))
+    zz = np.exp(rng.uniform(math.log(1e-6), math.log(100.0), n))
+    th = rng.uniform(0.0, math.pi, n)
+    out["kin_lo_in"] = np.stack((rr, zz, th))
+    y_now = hy[-1]
+    out["kin_lo_value"] = np.array([
+        _lo_integrand(torch.tensor(a), torch.tensor([b]), torch.tensor([c]), interp_N, r, cfg,
+                      y_now, hy, hN).item() for a, b, c in zip(rr, zz, th)])
+
+
+def adaptive_rk2(out):
+    """_adaptive_rk2_to (S/bk/evolution.py:239-326) with the BK derivative
+    replaced by a closed-form logistic right-hand side, so that the vectors
+    pin the step-size controller alone: accepted steps, rejections, the
+    returned step size and the final state over several output steps."""
+
+    import nlo_torch.bk.evolution as evolution
+    from nlo_torch.bk.config import BKConfig
+
+    rate = torch.tensor([0.0, 0.3, 1.1, 2.7, 6.0, 15.0, 40.0])
+
+    def derivative(N, r, rapidity, history_y, history_N, config, *, seed, vegas_states=None):
+        return rate * N * (1 - N) * (1 + 0.5 * torch.sin(3 * rapidity))
+
+    saved = evolution._evolve_derivative
+    evolution._evolve_derivative = derivative
+    try:
+        for case, (step, h0, abserr, relerr) in enumerate(((0.2, 0.2, 1e-6, 1e-4), (0.5, 2.0, 1e-8, 1e-6),
+                                                            (0.1, 0.01, 1e-3, 1e-2))):
+            cfg = BKConfig(DE_SOLVER_STEP=step, DE_SOLVER_ABSERR=abserr, DE_SOLVER_RELERR=relerr)
+            N = torch.tensor([0.5, 0.01, 0.02, 0.05, 1e-3, 1e-4, 1e-5])
+            y, h = 0.0, h0
+            traj = []
+            for _ in range(4):
+                N, y, h = evolution._adaptive_rk2_to(N, y, y + step, h, rate, None, None, cfg, None)
+                traj.append(np.concatenate([[y, h], N.numpy()]))
+            out[f"ark2_case{case}"] = np.array([step, h0, abserr, relerr])
+            out[f"ark2_traj{case}"] = np.array(traj)
+        out["ark2_rate"] = rate.numpy()
+        out["ark2_N0"] = np.array([0.5, 0.01, 0.02, 0.05, 1e-3, 1e-4, 1e-5])
+    finally:
+        evolution._evolve_derivative = saved
 
 
 if __name__ == "__main__":

@@ -8,8 +8,9 @@ evaluates, in FP64 at seeded random points:
 * ``Li2`` (S/numerics/special.py:35-97) and the Bessel K0 / K1 / K2 wrappers;
 * ``dis_alpha_s_freeze`` / ``dis_alpha_s_smooth`` (S/coupling.py:49-78);
 * ``GBW.dipole_amplitude`` (S/dipole/amplitude.py:38-43);
-* ``BKDipole.dipole_amplitude`` uninitialised and initialised
-  (S/dipole/amplitude.py:49-270) on the table in ``xb_001_mc50w.dat``;
+* ``BKDipole.dipole_amplitude`` uninitialised and initialised, SPLINE_LINEAR
+  and LINEAR_LINEAR (S/dipole/amplitude.py:49-270) on the table in
+  ``xb_001_mc50w.dat``;
 * ``integrand_photon_target_LO`` (S/dis/lo.py:12-51);
 * ``integrand_dip_massive`` for all six contributions (S/dis/observables.py:107-172);
 * ``integrand_qgunsub_massive`` I1 / I2 / I3 (S/dis/observables.py:175-250);
@@ -37,7 +38,7 @@ TABLE = Path(__file__).with_name("xb_001_mc50w.dat")
 def main() -> None:
     sys.path.insert(0, str(REFERENCE_SRC))
     from nlo_torch.coupling import dis_alpha_s_freeze, dis_alpha_s_smooth
-    from nlo_torch.dipole.amplitude import GBW, BKDipole
+    from nlo_torch.dipole.amplitude import GBW, AmplitudeInterpolation, BKDipole
     from nlo_torch.dipole.table import load_bk_table
     from nlo_torch.dis import observables as obs
     from nlo_torch.dis.config import (
@@ -106,6 +107,11 @@ def main() -> None:
     out["bk_init_Y"] = np.array(y_init)
     out["bk_N_init"] = bkd.dipole_amplitude(T(br), T(np.full_like(br, y_init))).numpy()
     out["bk_maxr_interpolate"] = np.array(bkd.maxr_interpolate)
+    bkl = BKDipole(table, interpolation=AmplitudeInterpolation.LINEAR_LINEAR)
+    out["bk_N_bilinear"] = bkl.dipole_amplitude(T(br), T(by)).numpy()
+    bkl.initialize_interpolation(y_init)
+    out["bk_N_bilinear_init"] = bkl.dipole_amplitude(T(br), T(np.full_like(br, y_init))).numpy()
+    out["bk_bilinear_maxr_interpolate"] = np.array(bkl.maxr_interpolate)
 
     light = Quark(QuarkType.LIGHT)
     charm = Quark(QuarkType.C, 1.4)

@@ -106,8 +106,8 @@ def test_struct_layouts_match_the_header():
     # nlo_sobol_table: dim + 8 x 30 directions + 8 shifts + 8 first points
     assert ctypes.sizeof(_lib.SobolTable) == 4 * (1 + 8 * 30 + 8 + 8)
     assert ctypes.sizeof(_lib.Physics) == 6 * 8 + 2 * 4
-    # nlo_dis_dipole: 2 int32, 4 doubles, 2 int32, 3 pointers, 2 doubles, 1 pointer
-    assert ctypes.sizeof(_lib.DisDipole) == 8 + 32 + 8 + 24 + 16 + 8
+    # nlo_dis_dipole: 4 int32, 4 doubles, 2 int32, 3 pointers, 2 doubles, 1 pointer
+    assert ctypes.sizeof(_lib.DisDipole) == 16 + 32 + 8 + 24 + 16 + 8
     # nlo_dis_desc: Q2, xbj, 2 int32, 8 + 8 doubles, maxr, 4 int32, 2 doubles, 2 int32, dipole
     assert ctypes.sizeof(_lib.DisDesc) == 16 + 8 + 128 + 8 + 16 + 16 + 8 + ctypes.sizeof(_lib.DisDipole)
 

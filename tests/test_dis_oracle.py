@@ -83,6 +83,11 @@ def test_bk_dipole_matches_reference(g):
     fn = bk.initialized(float(g["bk_init_Y"]))
     assert fn.maxr == pytest.approx(float(g["bk_maxr_interpolate"]), rel=1e-15)
     assert _rel(fn(g["bk_r"]), g["bk_N_init"]).max() < 1e-14
+    lin = D.BKDipoleOracle(t.r.numpy(), t.y.numpy(), t.N.numpy(), t.x0, bilinear=True)
+    assert _rel(lin.amplitude(g["bk_r"], g["bk_Y"]), g["bk_N_bilinear"]).max() < 1e-14
+    fl = lin.initialized(float(g["bk_init_Y"]))
+    assert fl.maxr == pytest.approx(float(g["bk_bilinear_maxr_interpolate"]), rel=1e-15)
+    assert _rel(fl(g["bk_r"]), g["bk_N_bilinear_init"]).max() < 1e-14
 
 
 def test_lo_integrand_matches_reference(g):

@@ -132,6 +132,15 @@ def test_dipole_amplitudes_match_oracle(dev, g):
     np.testing.assert_allclose(got, g["bk_N_init"], rtol=1e-13, atol=1e-16)
     with pytest.raises(ValueError):
         bk.dipole_amplitude(cuda([1.0]), cuda([bk.max_Y + 1]))
+    from nlo_bk_optimized_amd.dipole import AmplitudeInterpolation, BKDipole, load_bk_table
+
+    lin = BKDipole(load_bk_table("tests/golden/xb_001_mc50w.dat"), AmplitudeInterpolation.LINEAR_LINEAR)
+    got = lin.dipole_amplitude(cuda(g["bk_r"]), cuda(g["bk_Y"])).cpu().numpy()
+    np.testing.assert_allclose(got, g["bk_N_bilinear"], rtol=1e-13, atol=1e-16)
+    lin.initialize_interpolation(float(g["bk_init_Y"]))
+    assert lin.maxr_interpolate == pytest.approx(float(g["bk_bilinear_maxr_interpolate"]), rel=1e-14)
+    got = lin.dipole_amplitude(cuda(g["bk_r"]), cuda(np.full_like(g["bk_r"], float(g["bk_init_Y"])))).cpu().numpy()
+    np.testing.assert_allclose(got, g["bk_N_bilinear_init"], rtol=1e-13, atol=1e-16)
 
 
 # --------------------------------------------------------------- integrands
