@@ -6,8 +6,9 @@
  *   - pointers named d_* are device pointers; the caller owns every buffer,
  *     including scratch ("workspace"), whose size is queried first;
  *   - every launch goes on the stream argument (hipStream_t); nothing
- *     synchronises the host; there is no hidden global state except a
- *     thread-local last-error string;
+ *     synchronises the host except the adaptive RK2 step of the rapidity
+ *     driver (one error ratio read per attempt); there is no hidden global
+ *     state except a thread-local last-error string;
  *   - return value: NLO_OK (0) or an NLO_ERR_* code; nlo_last_error() gives
  *     the message.  The Python layer turns a non-zero code into RuntimeError,
  *     as the reference's TORCH_CHECK failures do.
@@ -34,7 +35,7 @@ extern "C" {
 #define NLO_ERR_WORKSPACE 3   /* workspace too small */
 #define NLO_ERR_UNSUPPORTED 4 /* configuration outside the implemented set */
 
-#define NLO_API_VERSION 1
+#define NLO_API_VERSION 2
 
 /* ------------------------------------------------------------------ misc */
 
@@ -48,19 +49,46 @@ int nlo_gauss_legendre(int32_t n, double* x, double* w);
 
 /* --------------------------------------------------------------- physics */
 
-/* BKConfig physics constants (S/bk/config.py:66-101) for the implemented
- * production choice: RC_LO = BALITSKY_LO, RC_NLO = SMALLEST_NLO,
- * RESUM_RC = RESUM_RC_SMALLEST, Order = NLO_RESUM_DLOG_SLOG, NF <= 3 (smooth
- * coupling, S/coupling.py:81-97). */
+/* BKConfig physics choices (S/bk/config.py:9-64) and constants
+ * (S/bk/config.py:66-101).  Zero is the production choice in every code
+ * field: Balitsky LO coupling, smallest-distance NLO and resummation
+ * couplings, NLO_RESUM_DLOG_SLOG.  RC_LO and RC_NLO are fixed together or not
+ * at all (S/bk/config.py:119-122); with them every coupling is FIXED_AS.
+ * NF = 0 or 3 selects the smooth coupling (S/coupling.py:81-97), NF = 5 the
+ * one-loop coupling with flavour thresholds (S/coupling.py:23-46). */
+#define NLO_RC_LO_BALITSKY 0
+#define NLO_RC_LO_FIXED 1
+#define NLO_RC_LO_PARENT 2
+#define NLO_RC_LO_SMALLEST 3
+#define NLO_RC_LO_FAC 4
+#define NLO_RC_LO_BEUF 5
+#define NLO_RC_NLO_SMALLEST 0
+#define NLO_RC_NLO_FIXED 1
+#define NLO_RC_NLO_PARENT 2
+#define NLO_RESUM_RC_SMALLEST 0
+#define NLO_RESUM_RC_PARENT 1
+#define NLO_RESUM_RC_FIXED 2
+#define NLO_ORDER_NLO_RESUM_DLOG_SLOG 0
+#define NLO_ORDER_NLO_RESUM_DLOG 1
+#define NLO_ORDER_NLO 2
+#define NLO_ORDER_LO_RESUM_DLOG_SLOG 3
+#define NLO_ORDER_LO_RESUM_DLOG 4
+#define NLO_ORDER_LO 5
+
 typedef struct nlo_bk_physics {
     double nc;          /* NC */
-    double nf;          /* NF (0 or 3) */
-    double lambda_qcd;  /* LambdaQCD */
+    double nf;          /* NF: 0, 3 or 5 */
+    double lambda_qcd;  /* LambdaQCD (smooth coupling) */
     double c2;          /* C2 */
     double ksub;        /* KSUB */
     double minr;        /* MINR (double-log resummation only for r > 1.01 MINR) */
     int32_t symmetrize; /* SYMMETRIZE_Z_Z2_INTEGRATION */
-    int32_t reserved;
+    int32_t order;      /* NLO_ORDER_*: Order */
+    int32_t rc_lo;      /* NLO_RC_LO_*: RC_LO */
+    int32_t rc_nlo;     /* NLO_RC_NLO_*: RC_NLO */
+    int32_t resum_rc;   /* NLO_RESUM_RC_*: RESUM_RC */
+    int32_t reserved;   /* 0 */
+    double fixed_as;    /* FIXED_AS */
 } nlo_bk_physics;
 
 /* ----------------------------------------------------------------- Sobol */
@@ -159,6 +187,19 @@ int nlo_bk_k1(int32_t R, const double* d_r, const double* d_N, const double* d_t
               const nlo_bk_physics* physics, int32_t nr, int32_t nt, double* d_k1,
               hipStream_t stream);
 
+/* The same with the kinematical constraint (KINEMATICAL_CONSTRAINT,
+ * S/bk/derivatives.py:445-468): the dipole S_01 - S_02 S_12 with S_02, S_12
+ * read from the amplitude history at the rapidity y - max(0, ln(min(X, Y)^2 /
+ * r^2)) (_interpolate_history_N, S/bk/derivatives.py:712-758), where y is the
+ * last history row's rapidity.  d_hist_y: capacity doubles, increasing,
+ * d_hist_N: capacity x R amplitudes on the grid d_r, *d_rows (device int32,
+ * read by the kernel and clamped to [1, capacity]) the rows in use; d_N must
+ * be the last row (the current amplitude). */
+int nlo_bk_k1_history(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+                      const nlo_bk_physics* physics, int32_t nr, int32_t nt,
+                      const double* d_hist_y, const double* d_hist_N, const int32_t* d_rows,
+                      int32_t capacity, double* d_k1, hipStream_t stream);
+
 /* K2 + Kf VEGAS sums for every active parent over Sobol points
  * [first_index, first_index + n) mapped through the parent's grid
  * (d_edges: R x 4 x (bins + 1); row 0 unused) on the box
@@ -186,6 +227,8 @@ int nlo_bk_rhs_finalize(int32_t R, const double* d_N, const double* d_k1, const 
 #define NLO_RK_MIDPOINT 1 /* N + h K(N + h/2 K(N))  (S/bk/evolution.py:265-276)  */
 #define NLO_RK_KUTTA3 2   /* (K1 + 4 K2 + K3)/6      (S/bk/evolution.py:190-236) */
 #define NLO_RK_CLASSIC4 3 /* classical fourth order                              */
+#define NLO_RK_ADAPTIVE2 4 /* midpoint RK2 with step doubling and step-size control
+                              (_adaptive_rk2_to, S/bk/evolution.py:239-326)      */
 
 typedef struct nlo_bk_evolver_desc {
     int32_t R;            /* r nodes */
@@ -200,6 +243,16 @@ typedef struct nlo_bk_evolver_desc {
     int32_t nlo;          /* include K2 + Kf */
     nlo_bk_physics physics;
     nlo_sobol_table sobol;
+    /* NLO_RK_ADAPTIVE2: DE_SOLVER_ABSERR, DE_SOLVER_RELERR and the machine
+     * epsilon of the caller's dtype (step underflow test) */
+    double abserr, relerr, eps;
+    /* kinematical constraint (NLO_RK_EULER only): the evolver appends each
+     * step's amplitude and rapidity to the history, which K1 reads.  The
+     * caller fills row 0 (initial rapidity and amplitude); capacity rows. */
+    int32_t kinematical;
+    int32_t history_capacity;
+    double* d_history_y;  /* capacity doubles */
+    double* d_history_N;  /* capacity x R doubles */
 } nlo_bk_evolver_desc;
 
 typedef struct nlo_bk_evolver nlo_bk_evolver;
@@ -223,7 +276,20 @@ uint64_t* nlo_bk_evolver_active_evaluations(nlo_bk_evolver* ev);
 /* Last step's dN/dy stages: d_k1 (R) and d_k2kf (R) of the final stage. */
 const double* nlo_bk_evolver_k1(nlo_bk_evolver* ev);
 const double* nlo_bk_evolver_k2kf(nlo_bk_evolver* ev);
+/* One output step of the rapidity grid: y -> y + step.  Fixed-step methods
+ * replay the captured graph asynchronously on `stream`.  NLO_RK_ADAPTIVE2
+ * runs the step-size controller on the host: per attempt one graph of four
+ * right-hand sides and the error reduction, then a synchronisation to read
+ * the error ratio (as the reference's per-attempt .item()); it returns
+ * NLO_ERR_INVALID with "step underflow" in nlo_last_error when the step
+ * collapses.  With the kinematical constraint the call fails once the
+ * history is full. */
 int nlo_bk_evolver_step(nlo_bk_evolver* ev, hipStream_t stream);
+/* Controller state of NLO_RK_ADAPTIVE2: current rapidity, next step size and
+ * attempt counts (any pointer may be NULL).  For fixed-step methods y
+ * advances by step per call and h = step. */
+int nlo_bk_evolver_adaptive_state(const nlo_bk_evolver* ev, double* y, double* h, int64_t* accepted,
+                                  int64_t* rejected);
 int nlo_bk_evolver_destroy(nlo_bk_evolver* ev);
 
 /* ================================================================== DIS */

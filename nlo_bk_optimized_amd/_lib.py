@@ -17,7 +17,13 @@ LIBRARY = Path(__file__).resolve().parent / "libnlo_gfx950.so"
 NLO_OK = 0
 NLO_SOBOL_U32, NLO_SOBOL_F32, NLO_SOBOL_F64 = 0, 1, 2
 NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
-NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4 = 0, 1, 2, 3
+NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4, NLO_RK_ADAPTIVE2 = 0, 1, 2, 3, 4
+# physics choice codes of nlo_bk_physics (zero = production)
+RC_LO_CODES = {"BALITSKY_LO": 0, "FIXED_LO": 1, "PARENT_LO": 2, "SMALLEST_LO": 3, "FAC_LO": 4, "BEUF_LO": 5}
+RC_NLO_CODES = {"SMALLEST_NLO": 0, "FIXED_NLO": 1, "PARENT_NLO": 2}
+RESUM_RC_CODES = {"RESUM_RC_SMALLEST": 0, "RESUM_RC_PARENT": 1, "RESUM_RC_FIXED": 2}
+ORDER_CODES = {"NLO_RESUM_DLOG_SLOG": 0, "NLO_RESUM_DLOG": 1, "NLO": 2, "LO_RESUM_DLOG_SLOG": 3,
+               "LO_RESUM_DLOG": 4, "LO": 5}
 SOBOL_MAXDIM = 8
 SOBOL_BITS = 30
 NLO_FN_K0, NLO_FN_K1, NLO_FN_K2, NLO_FN_LI2 = 0, 1, 2, 3
@@ -46,7 +52,12 @@ class Physics(ctypes.Structure):
         ("ksub", ctypes.c_double),
         ("minr", ctypes.c_double),
         ("symmetrize", ctypes.c_int32),
+        ("order", ctypes.c_int32),
+        ("rc_lo", ctypes.c_int32),
+        ("rc_nlo", ctypes.c_int32),
+        ("resum_rc", ctypes.c_int32),
         ("reserved", ctypes.c_int32),
+        ("fixed_as", ctypes.c_double),
     ]
 
 
@@ -74,6 +85,13 @@ class EvolverDesc(ctypes.Structure):
         ("nlo", ctypes.c_int32),
         ("physics", Physics),
         ("sobol", SobolTable),
+        ("abserr", ctypes.c_double),
+        ("relerr", ctypes.c_double),
+        ("eps", ctypes.c_double),
+        ("kinematical", ctypes.c_int32),
+        ("history_capacity", ctypes.c_int32),
+        ("d_history_y", ctypes.c_void_p),
+        ("d_history_N", ctypes.c_void_p),
     ]
 
 
@@ -141,6 +159,10 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_vegas_adapt": (ctypes.c_int, [_i32, _i32, _i32, _vp, _vp, _vp, _f64, _vp, _vp]),
     "nlo_bk_spline_build": (ctypes.c_int, [_i32, _vp, _vp, _i32, _vp, _vp, _vp]),
     "nlo_bk_k1": (ctypes.c_int, [_i32, _vp, _vp, _vp, ctypes.POINTER(Physics), _i32, _i32, _vp, _vp]),
+    "nlo_bk_k1_history": (
+        ctypes.c_int,
+        [_i32, _vp, _vp, _vp, ctypes.POINTER(Physics), _i32, _i32, _vp, _vp, _vp, _i32, _vp, _vp],
+    ),
     "nlo_bk_nlo_workspace": (_size, [_i32, _i32, _i64, _i32]),
     "nlo_bk_nlo": (
         ctypes.c_int,
@@ -177,6 +199,10 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_bk_evolver_active_evaluations": (_vp, [_vp]),
     "nlo_bk_evolver_k2kf": (_vp, [_vp]),
     "nlo_bk_evolver_step": (ctypes.c_int, [_vp, _vp]),
+    "nlo_bk_evolver_adaptive_state": (
+        ctypes.c_int,
+        [_vp, ctypes.POINTER(_f64), ctypes.POINTER(_f64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
+    ),
     "nlo_bk_evolver_destroy": (ctypes.c_int, [_vp]),
     "nlo_dis_special": (ctypes.c_int, [_i32, _i64, _vp, _i32, _vp, _vp]),
     "nlo_dis_dipole_amplitude": (ctypes.c_int, [ctypes.POINTER(DisDipole), _i64, _vp, _vp, _vp, _vp]),

@@ -69,7 +69,7 @@ class BKOrder(Enum):
 
 
 PRECISIONS = ("fp32", "fp64", "fp64_stable")
-RK_METHODS = ("euler", "rk2", "kutta3", "rk4")
+RK_METHODS = ("auto", "euler", "rk2", "kutta3", "rk4", "adaptive_rk2")
 
 
 @dataclass(frozen=True, slots=True)
@@ -117,9 +117,13 @@ class BKConfig:
     # "fp64": the reference's K2/Kf formula in FP64; "fp64_stable": FP64 with
     # the cancellation-free kernels.
     PRECISION: str = "fp32"
-    # Rapidity stepping: "kutta3" is the reference's production step
-    # (S/bk/evolution.py:190-236); "rk2" the midpoint rule; "euler"; "rk4".
-    RK_METHOD: str = "kutta3"
+    # Rapidity stepping: "auto" chooses as the reference's _bk_steps does
+    # (S/bk/evolution.py:108-146): Euler for EULER_METHOD, the three-stage
+    # Kutta step (S/bk/evolution.py:190-236) where the reference takes its
+    # fixed RK2(3) step, otherwise the adaptive midpoint RK2 with step
+    # doubling (S/bk/evolution.py:239-326).  Or explicitly "euler", "rk2"
+    # (midpoint), "kutta3", "rk4", "adaptive_rk2".
+    RK_METHOD: str = "auto"
     # K1 Gauss-Legendre orders per log-r panel (ln z) and in v (theta = pi v^2).
     K1_RADIAL_ORDER: int = 8
     K1_ANGULAR_ORDER: int = 24
@@ -162,34 +166,43 @@ class BKConfig:
 
 
 def require_gpu_supported(config: BKConfig) -> None:
-    """Reject configurations outside the implemented production physics.
+    """Reject configurations the device path does not implement: the NLO
+    terms are integrated by VEGAS only (MISER and the nested adaptive
+    MULTIPLE rule of S/bk/derivatives.py:313-420, :635-709 are not built)."""
 
-    The GPU path implements the reference's production choice
-    (S/bk/derivatives.py:145-154): Balitsky LO coupling, smallest-distance
-    NLO and resummation couplings, NLO_RESUM_DLOG_SLOG order, NF <= 3, no
-    kinematical constraint, VEGAS for the NLO terms.
-    """
-
-    problems = []
-    if config.RC_LO is not RunningCouplingLO.BALITSKY_LO:
-        problems.append(f"RC_LO={config.RC_LO.name}")
-    if config.RC_NLO is not RunningCouplingNLO.SMALLEST_NLO:
-        problems.append(f"RC_NLO={config.RC_NLO.name}")
-    if config.RESUM_RC is not ResummationCoupling.RESUM_RC_SMALLEST:
-        problems.append(f"RESUM_RC={config.RESUM_RC.name}")
-    if config.Order not in {BKOrder.NLO_RESUM_DLOG_SLOG, BKOrder.LO_RESUM_DLOG_SLOG}:
-        problems.append(f"Order={config.Order.name}")
-    if config.NF > 3:
-        problems.append(f"NF={config.NF}")
-    if config.KINEMATICAL_CONSTRAINT:
-        problems.append("KINEMATICAL_CONSTRAINT=True")
     if config.INTMETHOD_NLO is not IntegrationMethod.VEGAS:
-        problems.append(f"INTMETHOD_NLO={config.INTMETHOD_NLO.name}")
-    if problems:
         raise NotImplementedError(
-            "the MI355X BK path implements the reference's production physics only; "
-            "unsupported: " + ", ".join(problems)
+            f"the MI355X BK path integrates K2 + Kf by VEGAS only; unsupported: "
+            f"INTMETHOD_NLO={config.INTMETHOD_NLO.name}"
         )
+
+
+def resolve_rk_method(config: BKConfig, dtype) -> str:
+    """The stepping method RK_METHOD="auto" stands for (S/bk/evolution.py:108-146,
+    :180-187): the reference takes its fixed RK2(3) step only for float32 with
+    the validated fixed K1 rule (production physics, RPOINTS = 100,
+    RINTPOINTS = THETAINTPOINTS = 85, INTACCURACY = 0.001); else the adaptive RK2."""
+
+    import torch
+
+    if config.EULER_METHOD:
+        return "euler"
+    if config.RK_METHOD != "auto":
+        return config.RK_METHOD
+    fixed_k1 = (
+        not config.KINEMATICAL_CONSTRAINT
+        and config.RC_LO is RunningCouplingLO.BALITSKY_LO
+        and config.RESUM_RC is ResummationCoupling.RESUM_RC_SMALLEST
+        and config.Order is BKOrder.NLO_RESUM_DLOG_SLOG
+        and config.NF <= 3
+        and config.RPOINTS == 100
+        and config.RINTPOINTS == 85
+        and config.THETAINTPOINTS == 85
+        and config.INTACCURACY == 0.001
+    )
+    if dtype is torch.float32 and config.INTMETHOD_NLO is IntegrationMethod.VEGAS and fixed_k1:
+        return "kutta3"
+    return "adaptive_rk2"
 
 
 __all__ = [
@@ -198,6 +211,7 @@ __all__ = [
     "IntegrationMethod",
     "PRECISIONS",
     "RK_METHODS",
+    "resolve_rk_method",
     "ResummationCoupling",
     "RunningCouplingLO",
     "RunningCouplingNLO",

@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import torch
 
 from nlo_bk_optimized_amd import device as dev
-from nlo_bk_optimized_amd.bk.config import BKConfig, require_gpu_supported
+from nlo_bk_optimized_amd.bk.config import BKConfig, require_gpu_supported, resolve_rk_method
 from nlo_bk_optimized_amd.bk.initial_conditions import MV, ICDataFile
 from nlo_bk_optimized_amd.dipole.table import DipoleTable
 
@@ -125,11 +125,13 @@ def prepare(
     return BKSetup(r=r, N0=N0, table=table, edges=edges, n_samples=n)
 
 
-def rhs(setup: BKSetup, N: torch.Tensor, config: BKConfig) -> tuple[torch.Tensor, torch.Tensor]:
-    """K1 and K2 + Kf at amplitude N (S/bk/evolution.py:354-488)."""
+def rhs(setup: BKSetup, N: torch.Tensor, config: BKConfig,
+        history: tuple[torch.Tensor, torch.Tensor] | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+    """K1 and K2 + Kf at amplitude N (S/bk/evolution.py:354-488); with the
+    kinematical constraint ``history`` = (rapidities, amplitudes), N its last row."""
 
     t64, t32 = dev.spline_build(setup.r, N, config.FORCE_POSITIVE_N)
-    k1 = dev.bk_k1(setup.r, N, t64, config)
+    k1 = dev.bk_k1(setup.r, N, t64, config, history=history if config.KINEMATICAL_CONSTRAINT else None)
     if config.Order.has_nlo_kernels:
         sums, _, _ = dev.bk_nlo(
             setup.r, N, t64, t32, setup.edges, setup.table, setup.n_samples, config,
@@ -158,38 +160,56 @@ def solve_bk(
         dtype = initial_condition.r.dtype
     d = _resolve_device(device)
     setup = prepare(initial_condition, config, device=d, seed=seed)
-    steps = 0
-    y = 0.0
-    ys = [0.0]
+    rk = resolve_rk_method(config, dtype)
     tolerance = 1e-12 * max(1.0, abs(maxy), abs(config.DE_SOLVER_STEP))
-    while True:  # output rapidities as S/bk/evolution.py:108-159 produces them
+    # output rapidities of fixed steps, as S/bk/evolution.py:108-159 produces them
+    steps, y = 0, 0.0
+    while True:
         y = y + config.DE_SOLVER_STEP
         steps += 1
-        ys.append(y)
         if y >= maxy - tolerance:
             break
-    rk = "euler" if config.EULER_METHOD else config.RK_METHOD
+    R = setup.r.numel()
+    history_y = torch.zeros(steps + 1, dtype=torch.float64, device=d)
+    history = torch.empty((steps + 1, R), dtype=torch.float64, device=d)
+    history[0].copy_(setup.N0)
+    kinematical = bool(config.KINEMATICAL_CONSTRAINT)
     evolver = dev.Evolver(
         setup.r, setup.edges, setup.table, config,
         n_samples=setup.n_samples, precision=config.PRECISION, rk_method=rk,
         step=config.DE_SOLVER_STEP, nlo=config.Order.has_nlo_kernels,
+        eps=float(torch.finfo(dtype).eps), history=(history_y, history) if kinematical else None,
     )
+    ys = [0.0]
     try:
-        history = torch.empty((steps + 1, setup.r.numel()), dtype=torch.float64, device=d)
-        history[0].copy_(setup.N0)
         evolver.state.copy_(setup.N0)
-        for t in range(steps):
+        rows = [history]
+        t = 0
+        while True:
             evolver.step()
-            history[t + 1].copy_(evolver.state)
+            if rk == "adaptive_rk2":
+                y = evolver.adaptive_state()[0]  # the controller's own sum of sub-steps
+            else:
+                y = ys[-1] + config.DE_SOLVER_STEP
+            t += 1
+            if t <= steps:
+                if not kinematical:  # the kinematical evolver appends its rows itself
+                    history[t].copy_(evolver.state)
+            else:  # an adaptive run whose rounded rapidity needs one more output step
+                rows.append(evolver.state.clone().unsqueeze(0))
+            ys.append(y)
+            if y >= maxy - tolerance:
+                break
         if int(evolver.flag.item()) != 0:
             raise FloatingPointError("non-finite BK amplitude during the rapidity evolution")
     finally:
         evolver.close()
+    N_out = torch.cat([history[: min(t, steps) + 1]] + rows[1:], dim=0)
     r_out = setup.r.to(dtype)
     return DipoleTable(
         r=r_out,
         y=torch.tensor(ys, dtype=dtype, device=d),
-        N=history.to(dtype),
+        N=N_out.to(dtype),
         x0=initial_condition.x0,
     )
 
@@ -208,7 +228,8 @@ def compute_dndy(
         dtype = initial_condition.r.dtype
     d = _resolve_device(device)
     setup = prepare(initial_condition, config, device=d, seed=seed)
-    k1, k2kf = rhs(setup, setup.N0, config)
+    history = (torch.zeros(1, dtype=torch.float64, device=d), setup.N0.unsqueeze(0).contiguous())
+    k1, k2kf = rhs(setup, setup.N0, config, history=history)
     keep = torch.arange(setup.r.numel(), device=d) > 0
     keep = keep & (setup.N0 <= 0.99999)
     return DNDYResult(

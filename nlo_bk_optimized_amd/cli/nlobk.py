@@ -78,7 +78,8 @@ def build_parser() -> argparse.ArgumentParser:
     p.add_argument("--mc-bins", type=int, default=50)
     p.add_argument("--step", type=float, default=0.2)
     p.add_argument("--euler", action="store_true")
-    p.add_argument("--rk-method", choices=("euler", "rk2", "kutta3", "rk4"), default="kutta3")
+    p.add_argument("--rk-method", choices=("auto", "euler", "rk2", "kutta3", "rk4", "adaptive_rk2"),
+                   default="auto")
     p.add_argument("--precision", choices=("fp32", "fp64", "fp64_stable"), default="fp32")
     p.add_argument("--kinematical-constraint", action="store_true")
     p.add_argument("--nolimit", action="store_true")

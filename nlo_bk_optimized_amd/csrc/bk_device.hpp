@@ -23,6 +23,14 @@ struct BkConsts {
     double log_scale0;  // ln(4 C2 / Lambda^2)
     double norm;        // 1 / (8 pi^4)
     int symmetrize;
+    int rc_lo, rc_nlo, resum_rc;  // NLO_RC_LO_*, NLO_RC_NLO_*, NLO_RESUM_RC_*
+    int has_resum;      // Order resums the double logs (all but LO and NLO)
+    int slog;           // ... and the single logs
+    int fixed;          // every coupling is FIXED_AS (RC_LO or RC_NLO fixed)
+    int thresholds;     // NF > 3: one-loop coupling with flavour thresholds
+    int production;     // Balitsky LO, smallest resummation, DLOG_SLOG, smooth coupling
+    double fixed_as;
+    double log_4c2;     // ln(4 C2)
 };
 
 // -------------------------------------------------------------- coupling
@@ -34,6 +42,98 @@ __device__ __forceinline__ double alpha_s64(double r, const BkConsts& c) {
     const double right = log(scale) / 0.2;
     const double hi = fmax(c.log_mu0, right);
     return 1.0 / (c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right))))));
+}
+
+// bk_alpha_s for NF > 3 (S/coupling.py:23-46): Nf and Lambda switched at
+// 4 C2 / r^2 = 1.3^2 and 4.5^2, b0 = 11 - 2 Nf / 3, alpha = 4 pi / (b0 ln(4 C2 /
+// (r^2 Lambda^2))), 1 where the logarithm's argument is below 1 or alpha > 1.
+__device__ __forceinline__ double alpha_thresholds64(double r, const BkConsts& c) {
+    const double dipolescale = c.scale_num / (r * r);
+    double nf, lambda;
+    if (dipolescale < 1.3 * 1.3) {
+        nf = 3.0;
+        lambda = 0.146159;
+    } else if (dipolescale < 4.5 * 4.5) {
+        nf = 4.0;
+        lambda = 0.122944;
+    } else {
+        nf = 5.0;
+        lambda = 0.0904389;
+    }
+    const double b0 = 11.0 - 2.0 * nf / 3.0;
+    const double log_arg = c.scale_num / (r * r * (lambda * lambda));
+    const double a = 4.0 * kPi / (b0 * log(log_arg));
+    return (log_arg < 1.0 || a > 1.0) ? 1.0 : a;
+}
+
+// _alpha_s of S/bk_kernels/k1.py:158-169 and S/bk/derivatives.py:775-786:
+// FIXED_AS, the smooth coupling (NF <= 3) or the threshold coupling.
+__device__ __forceinline__ double alpha_any64(double r, const BkConsts& c) {
+    if (c.fixed) return c.fixed_as;
+    return c.thresholds ? alpha_thresholds64(r, c) : alpha_s64(r, c);
+}
+
+// The same threshold coupling from ln d (FP32 NLO integrand):
+// ln(4 C2 / d^2) against 2 ln 1.3 and 2 ln 4.5, ln(arg) = that - 2 ln Lambda.
+template <typename T>
+__device__ __forceinline__ T alpha_thresholds_log(T log_d, T log_4c2) {
+    const T lq = log_4c2 - T(2) * log_d;
+    T nf = T(5), l2 = T(-4.806161588812351);  // 2 ln 0.0904389
+    if (lq < T(0.5247285289349821)) {         // 2 ln 1.3
+        nf = T(3);
+        l2 = T(-3.8461204175059556);          // 2 ln 0.146159
+    } else if (lq < T(3.0081547935525483)) {  // 2 ln 4.5
+        nf = T(4);
+        l2 = T(-4.192052623672091);           // 2 ln 0.122944
+    }
+    const T L = lq - l2;
+    const T a = T(4.0 * kPi) / ((T(11) - T(2) * nf / T(3)) * L);
+    return (L < T(0) || a > T(1)) ? T(1) : a;
+}
+
+// Coupling of the K2 + Kf integrand (_nlo_alpha_s, S/bk/derivatives.py:615-632):
+// mode 0 = smooth coupling at the smallest distance (production), 1 = a
+// constant per parent (FIXED_AS or alpha(r)), 2 = threshold coupling at the
+// smallest distance.
+template <typename T>
+struct NloCoupling {
+    int mode;
+    T b0, log_mu0, log_scale0, log_4c2, as_const;
+};
+
+// ANYC = false: the production coupling (mode 0) compiled alone, so that the
+// production kernels carry no code or registers of the other modes.
+template <bool ANYC>
+__device__ __forceinline__ double nlo_alpha_from_log(double log_min, const NloCoupling<double>& k) {
+    if constexpr (ANYC) {
+        if (k.mode == 1) return k.as_const;
+        if (k.mode == 2) return alpha_thresholds_log<double>(log_min, k.log_4c2);
+    }
+    const double right = 5.0 * (k.log_scale0 - 2.0 * log_min);
+    const double hi = fmax(k.log_mu0, right);
+    return 1.0 / (k.b0 * (0.2 * (hi + log1p(exp(-fabs(k.log_mu0 - right))))));
+}
+
+// NloCoupling for a parent: the mode from the physics choice, the constant
+// for FIXED_NLO / PARENT_NLO evaluated at r in FP64.
+template <typename T>
+__device__ __forceinline__ NloCoupling<T> nlo_coupling(double r, const BkConsts& c) {
+    NloCoupling<T> k;
+    k.b0 = (T)c.b0;
+    k.log_mu0 = (T)c.log_mu0;
+    k.log_scale0 = (T)c.log_scale0;
+    k.log_4c2 = (T)c.log_4c2;
+    k.as_const = T(0);
+    if (c.fixed || c.rc_nlo == NLO_RC_NLO_FIXED) {
+        k.mode = 1;
+        k.as_const = (T)c.fixed_as;
+    } else if (c.rc_nlo == NLO_RC_NLO_PARENT) {
+        k.mode = 1;
+        k.as_const = (T)alpha_any64(r, c);
+    } else {
+        k.mode = c.thresholds ? 2 : 0;
+    }
+    return k;
 }
 
 // Same coupling from ln r (FP32): right = 5 (ln(4 C2 / Lambda^2) - 2 ln r).
@@ -166,6 +266,169 @@ __device__ __forceinline__ double k1_theta_term(const K1Parent& P, const K1Z& Z,
     return (dlog * slog * lo - lo_single * slog_exp + k1fin) * dipole;
 }
 
+// ------------------------------------------ K1 for every physics choice
+
+// Kernel_lo (S/bk_kernels/k1.py:20-155) for any RC_LO / RESUM_RC / Order
+// choice, at valid distances X, Y (logs given).  j1tab: the J1 table in LDS.
+__device__ inline double kernel_lo_any(double r, double log_r, double X, double lnX, double Y,
+                                       double lnY, const BkConsts& c, const double* j1tab) {
+    const double mn = fmin(fmin(X, Y), r);
+    const double a_r = alpha_any64(r, c);
+    const double rxy2 = (r / (X * Y)) * (r / (X * Y));
+    const double pref = c.nc / (2.0 * kPi * kPi);
+    double lo, a_scale;
+    switch (c.rc_lo) {
+        case NLO_RC_LO_SMALLEST:
+            a_scale = alpha_any64(mn, c);
+            lo = pref * a_scale * rxy2;
+            break;
+        case NLO_RC_LO_PARENT:
+            a_scale = a_r;
+            lo = pref * a_r * rxy2;
+            break;
+        case NLO_RC_LO_FAC: {
+            const double ab_r = a_r * c.nc / kPi;
+            const double ab_x = alpha_any64(X, c) * c.nc / kPi;
+            const double ab_y = alpha_any64(Y, c) * c.nc / kPi;
+            a_scale = a_r;
+            lo = 1.0 / (2.0 * kPi) /
+                 (1.0 / ab_r + (X * X - Y * Y) / (r * r) * (ab_x - ab_y) / (ab_x * ab_y)) * rxy2;
+            break;
+        }
+        case NLO_RC_LO_BEUF: {
+            const double reff2 = r * r * pow(Y * Y / (X * X), (X * X - Y * Y) / (r * r));
+            a_scale = alpha_any64(sqrt(reff2), c);
+            const double q = r / (X * Y + 1e-40);
+            lo = pref * a_scale * q * q;
+            break;
+        }
+        default: {  // BALITSKY_LO, FIXED_LO
+            const double a_x = alpha_any64(X, c), a_y = alpha_any64(Y, c);
+            a_scale = a_r;
+            lo = pref * a_r * (rxy2 + (a_y / a_x - 1.0) / (Y * Y) + (a_x / a_y - 1.0) / (X * X));
+            break;
+        }
+    }
+    lo = isfinite(lo) ? lo : 0.0;
+    if (!c.has_resum) return lo;
+    const double a_res = c.resum_rc == NLO_RESUM_RC_PARENT  ? a_r
+                         : c.resum_rc == NLO_RESUM_RC_FIXED ? c.fixed_as
+                                                            : alpha_any64(mn, c);
+    const double abar = a_res * c.nc / kPi;
+    const double xl = 4.0 * (lnX - log_r) * (lnY - log_r);
+    const double as_x = sqrt(abar * fabs(xl));
+    double dlog = 1.0;
+    if (r > 1.01 * c.minr && as_x != 0.0)
+        dlog = xl >= 0.0 ? j1_over_half_x(2.0 * as_x, j1tab) : i1_over_half_x(2.0 * as_x);
+    if (!isfinite(dlog)) return 0.0;
+    double slog = 1.0, slog_exp = 0.0;
+    if (c.slog) {
+        // ln(KSUB r^2 / min(X, Y)^2), shared by both single-log terms
+        slog_exp = -abar * (11.0 / 12.0) * fabs(log(c.ksub) + 2.0 * (log_r - fmin(lnX, lnY)));
+        slog = exp(slog_exp);
+    }
+    const double lo_single = pref * a_scale * rxy2;
+    const double fin = a_scale * c.nc / (4.0 * kPi) * (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
+    return dlog * slog * lo - lo_single * slog_exp + lo * fin;
+}
+
+// Amplitude history for the kinematical constraint: rows N(r_grid, y_k),
+// k < *rows, in HBM; the current rapidity is the last row's.
+struct K1History {
+    const double* y;
+    const double* N;
+    const double* r;
+    const int32_t* rows;  // device row count, read by the kernel (clamped to capacity)
+    int n;
+    int capacity;
+};
+
+// NaturalCubicSpline on six nodes evaluated at v (S/numerics/interpolation.py:10-68).
+__device__ inline double natural6(const double* x, const double* y, double v) {
+    double h[5], up[4], sol[4];
+    for (int i = 0; i < 5; ++i) h[i] = x[i + 1] - x[i];
+    for (int i = 0; i < 4; ++i) {
+        const double diag = 2.0 * (h[i] + h[i + 1]);
+        const double rhs = 6.0 * ((y[i + 2] - y[i + 1]) / h[i + 1] - (y[i + 1] - y[i]) / h[i]);
+        const double den = i == 0 ? diag : diag - h[i] * up[i - 1];
+        up[i] = h[i + 1] / den;
+        sol[i] = (rhs - (i == 0 ? 0.0 : h[i] * sol[i - 1])) / den;
+    }
+    for (int i = 2; i >= 0; --i) sol[i] -= up[i] * sol[i + 1];
+    int k = 0;
+    while (k < 4 && x[k + 1] <= v) ++k;
+    const double m0 = k == 0 ? 0.0 : sol[k - 1], m1 = k == 4 ? 0.0 : sol[k];
+    const double b = (y[k + 1] - y[k]) / h[k] - h[k] * (2.0 * m0 + m1) / 6.0;
+    const double dx = v - x[k];
+    return y[k] + dx * (b + dx * (m0 / 2.0 + dx * (m1 - m0) / (6.0 * h[k])));
+}
+
+// _interpolate_history_N (S/bk/derivatives.py:712-758) at (d, rapidity),
+// 0 <= rapidity <= the last row's rapidity; the last row is read through the
+// current spline sN.
+__device__ inline double history_n(double d, double log_d, double rapidity, const K1History& H,
+                                   int rows, const LdsSpline<double>& sN) {
+    int lo = 0, hi = rows;  // first row with y > rapidity
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (H.y[mid] <= rapidity) lo = mid + 1; else hi = mid;
+    }
+    const int yind = lo - 1 < 0 ? 0 : lo - 1;
+    double out;
+    if (yind == rows - 1) {
+        out = loglog_from_log(sN, log_d, 0.0, 1.0);
+    } else {
+        int a = 0, b = H.n;  // first node with r > d
+        while (a < b) {
+            const int mid = (a + b) >> 1;
+            if (H.r[mid] <= d) a = mid + 1; else b = mid;
+        }
+        const int rind = a - 1;
+        const int start = rind - 3 > 0 ? rind - 3 : 0;
+        if (start + 6 > H.n) {
+            out = 1.0;
+        } else {
+            double x[6], yl[6], yu[6];
+            for (int i = 0; i < 6; ++i) {
+                x[i] = H.r[start + i];
+                yl[i] = H.N[(size_t)yind * H.n + start + i];
+                yu[i] = H.N[(size_t)(yind + 1) * H.n + start + i];
+            }
+            double v = d;
+            if (v < x[0]) v = x[0] * 1.00001;
+            if (v > x[5]) v = x[5] * 0.999999;
+            const double nl = natural6(x, yl, v), nu = natural6(x, yu, v);
+            const double frac = (rapidity - H.y[yind]) / (H.y[yind + 1] - H.y[yind]);
+            out = nl + frac * (nu - nl);
+        }
+    }
+    return fmin(fmax(out, 0.0), 1.0);
+}
+
+// The K1 integrand (_lo_integrand, S/bk/derivatives.py:434-468) for any
+// physics choice, with the kinematical constraint when H.rows is set.
+__device__ inline double k1_theta_term_any(const K1Parent& P, const K1Z& Z, double s2,
+                                           const LdsSpline<double>& sN, const BkConsts& c,
+                                           const K1History& H, int rows, double rapidity) {
+    const double r = P.r, z = Z.z;
+    const double xsq = (r - z) * (r - z) + 4.0 * r * z * s2;
+    if (xsq < 1e-40 || z < 1e-20 || r < 1e-20) return 0.0;
+    const double X = sqrt(xsq);
+    const double lnX = log(X);
+    const double k = kernel_lo_any(r, P.log_r, X, lnX, z, Z.log_z, c, P.j1tab);
+    if (rows == 0) {
+        const double n_x = fmin(loglog_from_log(sN, lnX, 0.0, 1.0), 1.0);
+        return k * (n_x + Z.n_y - P.n_r - n_x * Z.n_y);
+    }
+    // rapidity shifted by max(0, ln(min(X, Y)^2 / r^2)); cut below zero
+    const double delta = fmax(2.0 * (fmin(lnX, Z.log_z) - P.log_r), 0.0);
+    const double shifted = rapidity - delta;
+    if (shifted < 0.0) return 0.0;
+    const double s02 = 1.0 - history_n(X, lnX, shifted, H, rows, sN);
+    const double s12 = 1.0 - history_n(z, Z.log_z, shifted, H, rows, sN);
+    return k * (-s02 * s12 + (1.0 - P.n_r));
+}
+
 // -------------------------------------------------- K2 + Kf, FP64 direct
 
 struct NloValue64 {
@@ -197,6 +460,7 @@ __device__ __forceinline__ double kernel_nlo_fermion64(double r, double X, doubl
 
 // _nlo_integrand (S/bk/derivatives.py:521-612) times exp(2 ln z + 2 ln z')
 // (S/bk/derivatives.py:335), with the reference's kernels and S dipoles.
+template <bool ANYC>
 __device__ NloValue64 nlo_integrand64(double r, double lz, double lz2, double tz, double tz2,
                                       const LdsSpline<double>& sS, double r_min, double r_max,
                                       const BkConsts& c) {
@@ -215,7 +479,14 @@ __device__ NloValue64 nlo_integrand64(double r, double lz, double lz2, double tz
     const double S_Y2 = loglog64(sS, r_min, r_max, Y2, 1.0, 0.0);
     const double S_Z = loglog64(sS, r_min, r_max, Z, 1.0, 0.0);
     const double dmin = fmin(fmin(fmin(r, X), Y), fmin(fmin(X2, Y2), Z));
-    const double as = alpha_s64(dmin, c);
+    // _nlo_alpha_s (S/bk/derivatives.py:615-632)
+    double as;
+    if constexpr (ANYC)
+        as = (c.fixed || c.rc_nlo == NLO_RC_NLO_FIXED) ? c.fixed_as
+             : c.rc_nlo == NLO_RC_NLO_PARENT           ? alpha_any64(r, c)
+                                                       : alpha_any64(dmin, c);
+    else
+        as = alpha_s64(dmin, c);
     const double z4 = Z * Z * Z * Z;
     double k = kernel_nlo64(r, X, Y, X2, Y2, Z, z4);
     double ks = kernel_nlo64(r, X2, Y2, X, Y, Z, z4);
@@ -322,11 +593,11 @@ struct NloStable {
     T total, k2, kf;
 };
 
-template <typename T>
+template <typename T, bool ANYC>
 __device__ __forceinline__ NloStable<T> nlo_integrand_stable(T r, T log_r, T lz, T lz2, T tz, T tz2,
-                                                             const LdsSpline<T>& sN, T b0, T log_mu0,
-                                                             T log_scale0, T nf_over_nc, T nc,
-                                                             T norm, bool symmetrize) {
+                                                             const LdsSpline<T>& sN,
+                                                             const NloCoupling<T>& cpl, T nf_over_nc,
+                                                             T nc, T norm, bool symmetrize) {
     NloStable<T> out{T(0), T(0), T(0)};
     const T z = exp(lz), z2 = exp(lz2);
     const T c1 = cos(tz), c2 = cos(tz2);
@@ -373,9 +644,7 @@ __device__ __forceinline__ NloStable<T> nlo_integrand_stable(T r, T log_r, T lz,
         kfr = -(symmetrize ? (kf * df + kfs * dfs) * T(0.5) : kf * df);
     }
     const T log_min = fmin(fmin(fmin(log_r, log_x), lz), fmin(fmin(log_x2, lz2), log_zz));
-    const T right = T(5) * (log_scale0 - T(2) * log_min);
-    const T hi = fmax(log_mu0, right);
-    const T as = T(1) / (b0 * (T(0.2) * (hi + log1p(exp(-fabs(log_mu0 - right))))));
+    const T as = nlo_alpha_from_log<ANYC>(log_min, cpl);
     const T scale = (as * nc) * (as * nc) * norm;
     k2r *= scale;
     kfr *= scale;
@@ -490,9 +759,10 @@ __device__ __forceinline__ void fspline_ns(const FastSpline& s, float log_d, flo
 // nlo_integrand_stable, every transcendental on the fast primitives above,
 // cos(theta) = 1 - 2 sin^2(theta / 2), logs of distances from the log1p
 // forms, N from the local-coordinate spline.
+template <bool ANYC>
 __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
-    float r, float log_r, float lz, float lz2, float tz, float tz2, const FastSpline& sp, float b0,
-    float log_mu0, float log_scale0, float nf_over_nc, float nc, float norm, bool symmetrize) {
+    float r, float log_r, float lz, float lz2, float tz, float tz2, const FastSpline& sp,
+    const NloCoupling<float>& cpl, float nf_over_nc, float nc, float norm, bool symmetrize) {
     NloStable<float> out{0.0f, 0.0f, 0.0f};
     const float z = fexp(lz), z2 = fexp(lz2);
     const float s1 = fsin_pi(0.5f * tz), s2 = fsin_pi(0.5f * tz2);
@@ -554,9 +824,16 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
         kfr = -(symmetrize ? (kf * df + kfs * dfs) * 0.5f : kf * df);
     }
     const float log_min = fminf(fminf(fminf(log_r, log_x), lz), fminf(fminf(log_x2, lz2), log_zz));
-    const float right = 5.0f * (log_scale0 - 2.0f * log_min);
-    const float hi = fmaxf(log_mu0, right);
-    const float as = frcp(b0 * (0.2f * (hi + flog(1.0f + fexp(-fabsf(log_mu0 - right))))));
+    float as;
+    if (!ANYC || cpl.mode == 0) {  // smooth coupling at the smallest distance (production)
+        const float right = 5.0f * (cpl.log_scale0 - 2.0f * log_min);
+        const float hi = fmaxf(cpl.log_mu0, right);
+        as = frcp(cpl.b0 * (0.2f * (hi + flog(1.0f + fexp(-fabsf(cpl.log_mu0 - right))))));
+    } else if (cpl.mode == 1) {
+        as = cpl.as_const;
+    } else {
+        as = alpha_thresholds_log<float>(log_min, cpl.log_4c2);
+    }
     const float scale = (as * nc) * (as * nc) * norm;
     k2r *= scale;
     kfr *= scale;

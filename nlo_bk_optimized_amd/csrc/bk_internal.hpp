@@ -29,7 +29,7 @@ NloPlan nlo_plan(int R, int bins, int64_t n, bool hist);
 int launch_spline_build(int R, const double* r, const double* N, int force_positive, double* t64,
                         float* t32, hipStream_t stream);
 int launch_k1(int R, const double* r, const double* N, const double* t64, const BkConsts& c, int nr,
-              int nt, double* k1, hipStream_t stream);
+              int nt, double* k1, hipStream_t stream, const K1History* hist = nullptr);
 int launch_nlo(int R, const double* r, const double* N, const double* t64, const float* t32,
                const double* edges, int bins, const nlo_sobol_table& sobol, int64_t first, int64_t n,
                int precision, const BkConsts& c, double* sums, double* hist, double* count, void* ws,

@@ -24,6 +24,18 @@ BkConsts make_consts(const nlo_bk_physics& p) {
     c.log_scale0 = std::log(4.0 * p.c2 / (p.lambda_qcd * p.lambda_qcd));
     c.norm = 1.0 / (8.0 * kPi * kPi * kPi * kPi);
     c.symmetrize = p.symmetrize;
+    c.rc_lo = p.rc_lo;
+    c.rc_nlo = p.rc_nlo;
+    c.resum_rc = p.resum_rc;
+    c.has_resum = p.order != NLO_ORDER_LO && p.order != NLO_ORDER_NLO;
+    c.slog = p.order == NLO_ORDER_NLO_RESUM_DLOG_SLOG || p.order == NLO_ORDER_LO_RESUM_DLOG_SLOG;
+    c.fixed = p.rc_lo == NLO_RC_LO_FIXED || p.rc_nlo == NLO_RC_NLO_FIXED;
+    c.thresholds = p.nf > 3.0;
+    c.fixed_as = p.fixed_as;
+    c.log_4c2 = std::log(4.0 * p.c2);
+    c.production = p.rc_lo == NLO_RC_LO_BALITSKY && p.resum_rc == NLO_RESUM_RC_SMALLEST &&
+                   (p.order == NLO_ORDER_NLO_RESUM_DLOG_SLOG || p.order == NLO_ORDER_LO_RESUM_DLOG_SLOG) &&
+                   !c.fixed && !c.thresholds;
     return c;
 }
 
@@ -31,11 +43,19 @@ int validate_physics(const nlo_bk_physics* p) {
     NLO_REQUIRE(p, "null physics");
     NLO_REQUIRE(p->nc > 0 && p->lambda_qcd > 0 && p->c2 > 0 && p->ksub > 0 && p->minr > 0,
                 "physics constants must be positive");
-    if (!(p->nf == 0.0 || p->nf == 3.0)) {
-        set_error("NF=%g: only the smooth coupling (NF <= 3, S/coupling.py:20-21) is implemented",
-                  p->nf);
-        return NLO_ERR_UNSUPPORTED;
-    }
+    NLO_REQUIRE(p->nf == 0.0 || p->nf == 3.0 || p->nf == 5.0, "NF=%g: NF must be 0, 3 or 5", p->nf);
+    NLO_REQUIRE(p->rc_lo >= NLO_RC_LO_BALITSKY && p->rc_lo <= NLO_RC_LO_BEUF, "unknown RC_LO code %d",
+                p->rc_lo);
+    NLO_REQUIRE(p->rc_nlo >= NLO_RC_NLO_SMALLEST && p->rc_nlo <= NLO_RC_NLO_PARENT,
+                "unknown RC_NLO code %d", p->rc_nlo);
+    NLO_REQUIRE(p->resum_rc >= NLO_RESUM_RC_SMALLEST && p->resum_rc <= NLO_RESUM_RC_FIXED,
+                "unknown RESUM_RC code %d", p->resum_rc);
+    NLO_REQUIRE(p->order >= NLO_ORDER_NLO_RESUM_DLOG_SLOG && p->order <= NLO_ORDER_LO,
+                "unknown Order code %d", p->order);
+    NLO_REQUIRE((p->rc_lo == NLO_RC_LO_FIXED) == (p->rc_nlo == NLO_RC_NLO_FIXED),
+                "fixed coupling must be selected for both RC_LO and RC_NLO");
+    NLO_REQUIRE(!(p->rc_lo == NLO_RC_LO_FIXED || p->resum_rc == NLO_RESUM_RC_FIXED) || p->fixed_as > 0,
+                "FIXED_AS must be positive");
     return NLO_OK;
 }
 
@@ -179,10 +199,15 @@ __device__ __forceinline__ double block_sum_256(double v, double* red) {
 // whose sin^2(theta/2) and weights sit in LDS.
 constexpr int kK1Threads = 1024;
 
+// ANY = false: the production physics with the hoisted per-z-node terms;
+// ANY = true: every coupling / order choice and the kinematical constraint
+// (H.rows set), the full Kernel_lo per theta node.
+template <bool ANY>
 __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __restrict__ r,
                                                         const double* __restrict__ N,
                                                         const double* __restrict__ t64, BkConsts c,
-                                                        GaussRule g, double* __restrict__ k1) {
+                                                        GaussRule g, K1History H,
+                                                        double* __restrict__ k1) {
     extern __shared__ double sh[];
     __shared__ double red[kK1Threads / 64];
     __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], s2t[kMaxGaussT], wtt[kMaxGaussT];
@@ -211,13 +236,16 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
     P.r = r[p];
     P.log_r = log(P.r);
     P.n_r = fmin(loglog64(s, r_min, r_max, P.r, 0.0, 1.0), 1.0);
-    P.a_r = alpha_s64(P.r, c);
+    P.a_r = ANY ? 0.0 : alpha_s64(P.r, c);
     P.lo_coef = c.nc / (2.0 * kPi * kPi) * P.a_r;
     P.fin_coef = P.a_r * c.nc / (4.0 * kPi) * (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
     P.log_ksub = log(c.ksub);
     P.resum_on = P.r > 1.01 * c.minr ? 1.0 : 0.0;
     P.j1tab = j1tab;
     const int znodes = (R - 1) * g.nr;
+    // kinematical constraint: history rows and the current rapidity (last row)
+    const int rows = (ANY && H.rows) ? min(max(*H.rows, 1), H.capacity) : 0;
+    const double rapidity = rows > 0 ? H.y[rows - 1] : 0.0;
     double total = 0.0;
     for (int zi = threadIdx.x; zi < znodes; zi += blockDim.x) {
         const int panel = zi / g.nr;
@@ -228,12 +256,17 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
         Z.log_z = center + half * gxr[i];
         Z.z = exp(Z.log_z);
         Z.n_y = fmin(loglog_from_log(s, Z.log_z, 0.0, 1.0), 1.0);
-        Z.den_y = alpha_s64_den(Z.log_z, c);
-        Z.a_y = 1.0 / Z.den_y;
-        Z.ly = Z.log_z - P.log_r;
-        Z.inv_y2 = 1.0 / (Z.z * Z.z);
         double acc = 0.0;
-        for (int j = 0; j < g.nt; ++j) acc += k1_theta_term(P, Z, s2t[j], s, c) * wtt[j];
+        if constexpr (ANY) {
+            for (int j = 0; j < g.nt; ++j)
+                acc += k1_theta_term_any(P, Z, s2t[j], s, c, H, rows, rapidity) * wtt[j];
+        } else {
+            Z.den_y = alpha_s64_den(Z.log_z, c);
+            Z.a_y = 1.0 / Z.den_y;
+            Z.ly = Z.log_z - P.log_r;
+            Z.inv_y2 = 1.0 / (Z.z * Z.z);
+            for (int j = 0; j < g.nt; ++j) acc += k1_theta_term(P, Z, s2t[j], s, c) * wtt[j];
+        }
         total += acc * ((2.0 * Z.z * Z.z) * (half * gwr[i]));
     }
     total = wave_sum(total);
@@ -301,7 +334,9 @@ struct NloArgs {
 // into a wave-uniform high part and a per-lane low part.
 // FULL = false (the evolver): only sum w is accumulated; the sum of w^2 and
 // the K2 / Kf split read as 0.
-template <int MODE, bool HIST, bool FULL>
+// ANYC: any coupling choice (NloCoupling modes); false = the production
+// coupling only (smooth, smallest distance), with no other code paths.
+template <int MODE, bool HIST, bool FULL, bool ANYC>
 __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     using T = typename std::conditional<MODE == NLO_PREC_FP32, float, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -407,6 +442,15 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     }
     const T r_t = (T)rp;
     const T log_r_t = (T)log(rp);
+    NloCoupling<float> cpl32;
+    NloCoupling<double> cpl64;
+    if constexpr (ANYC) {
+        cpl32 = nlo_coupling<float>(rp, a.c);
+        cpl64 = nlo_coupling<double>(rp, a.c);
+    } else {
+        cpl32 = NloCoupling<float>{0, (float)a.c.b0, (float)a.c.log_mu0, (float)a.c.log_scale0, 0.0f, 0.0f};
+        cpl64 = NloCoupling<double>{0, a.c.b0, a.c.log_mu0, a.c.log_scale0, 0.0, 0.0};
+    }
     const double r_min64 = a.t64[0], r_max64 = a.t64[1];
 
     const int lane = threadIdx.x & 63;
@@ -484,23 +528,22 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
         double wv = 0.0, k2 = 0.0, kf = 0.0;
         if (valid) {
             if constexpr (MODE == NLO_PREC_FP64) {
-                const NloValue64 f = nlo_integrand64(rp, x[0], x[1], x[2], x[3], spl, r_min64,
-                                                     r_max64, a.c);
+                const NloValue64 f = nlo_integrand64<ANYC>(rp, x[0], x[1], x[2], x[3], spl, r_min64,
+                                                           r_max64, a.c);
                 wv = f.total * (double)jac;
                 k2 = f.k2 * (double)jac;
                 kf = f.kf * (double)jac;
             } else if constexpr (MODE == NLO_PREC_FP32) {
-                const NloStable<float> f = nlo_integrand_fp32(
-                    r_t, log_r_t, x[0], x[1], x[2], x[3], fsp, (float)a.c.b0, (float)a.c.log_mu0,
-                    (float)a.c.log_scale0, (float)(a.c.nf / a.c.nc), (float)a.c.nc, (float)a.c.norm,
-                    a.c.symmetrize != 0);
+                const NloStable<float> f = nlo_integrand_fp32<ANYC>(
+                    r_t, log_r_t, x[0], x[1], x[2], x[3], fsp, cpl32, (float)(a.c.nf / a.c.nc),
+                    (float)a.c.nc, (float)a.c.norm, a.c.symmetrize != 0);
                 wv = (double)(f.total * jac);
                 k2 = (double)(f.k2 * jac);
                 kf = (double)(f.kf * jac);
             } else {
-                const NloStable<double> f = nlo_integrand_stable<double>(
-                    rp, log(rp), x[0], x[1], x[2], x[3], spl, a.c.b0, a.c.log_mu0, a.c.log_scale0,
-                    a.c.nf / a.c.nc, a.c.nc, a.c.norm, a.c.symmetrize != 0);
+                const NloStable<double> f = nlo_integrand_stable<double, ANYC>(
+                    rp, log(rp), x[0], x[1], x[2], x[3], spl, cpl64, a.c.nf / a.c.nc, a.c.nc,
+                    a.c.norm, a.c.symmetrize != 0);
                 wv = f.total * jac;
                 k2 = f.k2 * jac;
                 kf = f.kf * jac;
@@ -700,16 +743,21 @@ int init_kernel_attributes() {
         if (dyn <= 65536) return;
         const void* fns[] = {
             (const void*)spline_build_kernel,
-            (const void*)k1_kernel,
-            (const void*)nlo_kernel<NLO_PREC_FP64, false, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP64, false, false>,
-            (const void*)nlo_kernel<NLO_PREC_FP64, true, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP32, false, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP32, false, false>,
-            (const void*)nlo_kernel<NLO_PREC_FP32, true, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false, true>,
-            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false, false>,
-            (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, true, true>,
+            (const void*)k1_kernel<false>,
+            (const void*)k1_kernel<true>,
+#define NLO_INSTANCES(ANYC)                                                   \
+    (const void*)nlo_kernel<NLO_PREC_FP64, false, true, ANYC>,                \
+        (const void*)nlo_kernel<NLO_PREC_FP64, false, false, ANYC>,           \
+        (const void*)nlo_kernel<NLO_PREC_FP64, true, true, ANYC>,             \
+        (const void*)nlo_kernel<NLO_PREC_FP32, false, true, ANYC>,            \
+        (const void*)nlo_kernel<NLO_PREC_FP32, false, false, ANYC>,           \
+        (const void*)nlo_kernel<NLO_PREC_FP32, true, true, ANYC>,             \
+        (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false, true, ANYC>,     \
+        (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, false, false, ANYC>,    \
+        (const void*)nlo_kernel<NLO_PREC_FP64_STABLE, true, true, ANYC>
+            NLO_INSTANCES(false),
+            NLO_INSTANCES(true),
+#undef NLO_INSTANCES
         };
         for (const void* f : fns) {
             if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) != hipSuccess)
@@ -731,7 +779,7 @@ int launch_spline_build(int R, const double* r, const double* N, int force_posit
 }
 
 int launch_k1(int R, const double* r, const double* N, const double* t64, const BkConsts& c, int nr,
-              int nt, double* k1, hipStream_t stream) {
+              int nt, double* k1, hipStream_t stream, const K1History* hist) {
     GaussRule g{};
     g.nr = nr;
     g.nt = nt;
@@ -740,7 +788,14 @@ int launch_k1(int R, const double* r, const double* N, const double* t64, const 
     const size_t smem = ((size_t)R + 4 * (size_t)(R - 1)) * sizeof(double);
     int rc = init_kernel_attributes();
     if (rc != NLO_OK) return rc;
-    hipLaunchKernelGGL(k1_kernel, dim3(R), dim3(kK1Threads), smem, stream, R, r, N, t64, c, g, k1);
+    K1History H{};
+    if (hist) H = *hist;
+    if (c.production && !hist)
+        hipLaunchKernelGGL(k1_kernel<false>, dim3(R), dim3(kK1Threads), smem, stream, R, r, N, t64, c,
+                           g, H, k1);
+    else
+        hipLaunchKernelGGL(k1_kernel<true>, dim3(R), dim3(kK1Threads), smem, stream, R, r, N, t64, c, g,
+                           H, k1);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }
@@ -800,13 +855,21 @@ int launch_nlo(int R, const double* r, const double* N, const double* t64, const
     const dim3 grid(pl.bpp, R - 1);
     int rc = init_kernel_attributes();
     if (rc != NLO_OK) return rc;
-#define NLO_PICK(MODE)                                                                        \
-    if (want_hist)                                                                            \
-        hipLaunchKernelGGL((nlo_kernel<MODE, true, true>), grid, dim3(256), smem, stream, a);  \
-    else if (components)                                                                      \
-        hipLaunchKernelGGL((nlo_kernel<MODE, false, true>), grid, dim3(256), smem, stream, a); \
-    else                                                                                      \
-        hipLaunchKernelGGL((nlo_kernel<MODE, false, false>), grid, dim3(256), smem, stream, a)
+#define NLO_PICK2(MODE, ANYC)                                                                        \
+    if (want_hist)                                                                                   \
+        hipLaunchKernelGGL((nlo_kernel<MODE, true, true, ANYC>), grid, dim3(256), smem, stream, a);  \
+    else if (components)                                                                             \
+        hipLaunchKernelGGL((nlo_kernel<MODE, false, true, ANYC>), grid, dim3(256), smem, stream, a); \
+    else                                                                                             \
+        hipLaunchKernelGGL((nlo_kernel<MODE, false, false, ANYC>), grid, dim3(256), smem, stream, a)
+#define NLO_PICK(MODE)             \
+    if (production_coupling) {     \
+        NLO_PICK2(MODE, false);    \
+    } else {                       \
+        NLO_PICK2(MODE, true);     \
+    }
+    // the production coupling: smooth (NF <= 3), not fixed, at the smallest distance
+    const bool production_coupling = !c.fixed && !c.thresholds && c.rc_nlo == NLO_RC_NLO_SMALLEST;
     if (precision == NLO_PREC_FP64) {
         NLO_PICK(NLO_PREC_FP64);
     } else if (precision == NLO_PREC_FP32) {
@@ -815,6 +878,7 @@ int launch_nlo(int R, const double* r, const double* N, const double* t64, const
         NLO_PICK(NLO_PREC_FP64_STABLE);
     }
 #undef NLO_PICK
+#undef NLO_PICK2
     NLO_LAUNCH_CHECK();
     hipLaunchKernelGGL(nlo_reduce_kernel, dim3(R), dim3(256), 0, stream, R, pl.bpp, bins,
                        want_hist ? 1 : 0, a.part, a.part_hist, a.part_cnt, sums, hist, count);
@@ -885,6 +949,25 @@ int nlo_bk_k1(int32_t R, const double* d_r, const double* d_N, const double* d_t
     int rc = nlo::validate_physics(physics);
     if (rc != NLO_OK) return rc;
     rc = nlo::launch_k1(R, d_r, d_N, d_table64, nlo::make_consts(*physics), nr, nt, d_k1, stream);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+int nlo_bk_k1_history(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+                      const nlo_bk_physics* physics, int32_t nr, int32_t nt,
+                      const double* d_hist_y, const double* d_hist_N, const int32_t* d_rows,
+                      int32_t capacity, double* d_k1, hipStream_t stream) {
+    NLO_REQUIRE(R >= 3 && R <= nlo::kMaxR, "nlo_bk_k1_history: R=%d outside [3, %d]", R, nlo::kMaxR);
+    NLO_REQUIRE(nr >= 1 && nr <= nlo::kMaxGaussR && nt >= 1 && nt <= nlo::kMaxGaussT,
+                "nlo_bk_k1_history: GL orders (%d, %d) outside [1, %d] x [1, %d]", nr, nt,
+                nlo::kMaxGaussR, nlo::kMaxGaussT);
+    NLO_REQUIRE(d_r && d_N && d_table64 && d_k1 && d_hist_y && d_hist_N && d_rows,
+                "nlo_bk_k1_history: null pointer");
+    NLO_REQUIRE(capacity >= 1, "nlo_bk_k1_history: capacity must be positive");
+    int rc = nlo::validate_physics(physics);
+    if (rc != NLO_OK) return rc;
+    nlo::K1History H{d_hist_y, d_hist_N, d_r, d_rows, R, capacity};
+    rc = nlo::launch_k1(R, d_r, d_N, d_table64, nlo::make_consts(*physics), nr, nt, d_k1, stream, &H);
     if (rc == NLO_OK) nlo::clear_error();
     return rc;
 }

@@ -476,12 +476,35 @@ __global__ void __launch_bounds__(256) vegas_kernel(DisArgs a, VegasArgs v) {
     }
 }
 
+// Histogram partials, first level: block (x, g) sums the per-block cells
+// [64 x, 64 x + 64) over the g-th contiguous range of blocks, in block order
+// (coalesced across lanes: consecutive cells).
+constexpr int kHistGroups = 64;
+
+__global__ void __launch_bounds__(64) vegas_hist_groups_kernel(int blocks, int cells, int per_group,
+                                                              const double* __restrict__ ph,
+                                                              const double* __restrict__ pc,
+                                                              double* __restrict__ gh, double* __restrict__ gc) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int g = blockIdx.y;
+    if (k >= cells) return;
+    const int b0 = g * per_group, b1 = min(b0 + per_group, blocks);
+    double h = 0.0, c = 0.0;
+    for (int b = b0; b < b1; ++b) {
+        h += ph[(size_t)b * cells + k];
+        c += pc[(size_t)b * cells + k];
+    }
+    gh[(size_t)g * cells + k] = h;
+    gc[(size_t)g * cells + k] = c;
+}
+
 // Fixed-order sums over the per-block partials: sums (2) by strided thread
-// sums, a lane butterfly and a wave sum; each histogram cell by one thread in
-// block order.
-__global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells, const double* __restrict__ part,
-                                                           const double* __restrict__ ph,
-                                                           const double* __restrict__ pc, double* __restrict__ sums,
+// sums, a lane butterfly and a wave sum in block 0; each histogram cell by
+// one thread over the group partials in group order.
+__global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells, int groups,
+                                                           const double* __restrict__ part,
+                                                           const double* __restrict__ gh,
+                                                           const double* __restrict__ gc, double* __restrict__ sums,
                                                            double* __restrict__ hist, double* __restrict__ cnt) {
     __shared__ double red[4][2];
     if (blockIdx.x == 0) {
@@ -509,9 +532,9 @@ __global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells
     const int k = (blockIdx.x - 1) * blockDim.x + threadIdx.x;
     if (k >= cells) return;
     double h = 0.0, c = 0.0;
-    for (int b = 0; b < blocks; ++b) {
-        h += ph[(size_t)b * cells + k];
-        c += pc[(size_t)b * cells + k];
+    for (int g = 0; g < groups; ++g) {
+        h += gh[(size_t)g * cells + k];
+        c += gc[(size_t)g * cells + k];
     }
     hist[k] = h;
     cnt[k] = c;
@@ -653,8 +676,8 @@ int make_args(const nlo_dis_desc* desc, DisArgs& a) {
     }
 
 struct VegasPlan {
-    int spt, blocks;
-    size_t part, hist, workspace, lds;
+    int spt, blocks, groups, per_group;
+    size_t part, hist, group, workspace, lds;
 };
 
 VegasPlan vegas_plan(int contribution, int bins, int64_t n) {
@@ -667,7 +690,10 @@ VegasPlan vegas_plan(int contribution, int bins, int64_t n) {
     p.blocks = (int)((groups + 4 * spt - 1) / (4 * spt));
     p.part = align_up((size_t)p.blocks * 2 * sizeof(double), 256);
     p.hist = align_up((size_t)p.blocks * D * bins * sizeof(double), 256);
-    p.workspace = p.part + 2 * p.hist;
+    p.per_group = (p.blocks + kHistGroups - 1) / kHistGroups;
+    p.groups = (p.blocks + p.per_group - 1) / p.per_group;
+    p.group = align_up((size_t)p.groups * D * bins * sizeof(double), 256);
+    p.workspace = p.part + 2 * p.hist + 2 * p.group;
     p.lds = vegas_layout(D, bins, spt, true).total;
     return p;
 }
@@ -816,9 +842,16 @@ int nlo_dis_vegas(const nlo_dis_desc* desc, int32_t contribution, int32_t bins, 
 #undef DIS_VEGAS
     NLO_LAUNCH_CHECK();
     const int cells = D * bins;
+    double* gh = reinterpret_cast<double*>(ws + pl.part + 2 * pl.hist);
+    double* gc = reinterpret_cast<double*>(ws + pl.part + 2 * pl.hist + pl.group);
+    if (hist) {
+        hipLaunchKernelGGL(vegas_hist_groups_kernel, dim3((cells + 63) / 64, pl.groups), dim3(64), 0, stream,
+                           pl.blocks, cells, pl.per_group, v.part_hist, v.part_cnt, gh, gc);
+        NLO_LAUNCH_CHECK();
+    }
     const int rblocks = 1 + (hist ? (cells + 255) / 256 : 0);
-    hipLaunchKernelGGL(vegas_reduce_kernel, dim3(rblocks), dim3(256), 0, stream, pl.blocks, cells, v.part,
-                       v.part_hist, v.part_cnt, d_sums, d_hist, d_count);
+    hipLaunchKernelGGL(vegas_reduce_kernel, dim3(rblocks), dim3(256), 0, stream, pl.blocks, cells, pl.groups,
+                       v.part, gh, gc, d_sums, d_hist, d_count);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }

@@ -147,6 +147,11 @@ def physics(config) -> _lib.Physics:
     p.ksub = float(config.KSUB)
     p.minr = float(config.MINR)
     p.symmetrize = int(bool(config.SYMMETRIZE_Z_Z2_INTEGRATION))
+    p.order = _lib.ORDER_CODES[config.Order.name]
+    p.rc_lo = _lib.RC_LO_CODES[config.RC_LO.name]
+    p.rc_nlo = _lib.RC_NLO_CODES[config.RC_NLO.name]
+    p.resum_rc = _lib.RESUM_RC_CODES[config.RESUM_RC.name]
+    p.fixed_as = float(config.FIXED_AS)
     return p
 
 
@@ -166,16 +171,36 @@ def spline_build(r: torch.Tensor, N: torch.Tensor, force_positive: bool = True) 
     return t64, t32
 
 
-def bk_k1(r: torch.Tensor, N: torch.Tensor, t64: torch.Tensor, config) -> torch.Tensor:
-    """K1 for every r node (0 for index 0 and N > 0.99999)."""
+def bk_k1(r: torch.Tensor, N: torch.Tensor, t64: torch.Tensor, config,
+          history: tuple[torch.Tensor, torch.Tensor] | None = None) -> torch.Tensor:
+    """K1 for every r node (0 for index 0 and N > 0.99999).  ``history``
+    (rapidities [rows], amplitudes [rows, R], the last row equal to N) turns
+    the kinematical constraint on at the last row's rapidity."""
 
     R = r.numel()
     out = torch.empty(R, dtype=torch.float64, device=r.device)
     ph = physics(config)
+    if history is None:
+        check(
+            lib().nlo_bk_k1(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
+                            config.K1_ANGULAR_ORDER, ptr(out), stream_handle(r.device)),
+            "nlo_bk_k1",
+        )
+        return out
+    hy, hN = history
+    require_cuda(hy, torch.float64, "history rapidities")
+    require_cuda(hN, torch.float64, "history amplitudes")
+    rows = hy.numel()
+    if rows < 1 or hN.shape != (rows, R):
+        raise ValueError("history must be rapidities [rows] and amplitudes [rows, R]")
+    if not bool((hy[1:] > hy[:-1]).all()):
+        raise ValueError("history rapidities must increase")
+    d_rows = torch.tensor([rows], dtype=torch.int32, device=r.device)
     check(
-        lib().nlo_bk_k1(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
-                        config.K1_ANGULAR_ORDER, ptr(out), stream_handle(r.device)),
-        "nlo_bk_k1",
+        lib().nlo_bk_k1_history(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
+                                config.K1_ANGULAR_ORDER, ptr(hy), ptr(hN), ptr(d_rows), rows, ptr(out),
+                                stream_handle(r.device)),
+        "nlo_bk_k1_history",
     )
     return out
 
@@ -235,7 +260,7 @@ def rhs_finalize(N: torch.Tensor, k1: torch.Tensor, sums: torch.Tensor, n: int) 
 
 
 RK_CODES = {"euler": _lib.NLO_RK_EULER, "rk2": _lib.NLO_RK_MIDPOINT, "kutta3": _lib.NLO_RK_KUTTA3,
-            "rk4": _lib.NLO_RK_CLASSIC4}
+            "rk4": _lib.NLO_RK_CLASSIC4, "adaptive_rk2": _lib.NLO_RK_ADAPTIVE2}
 
 
 class Evolver:
@@ -254,11 +279,18 @@ class Evolver:
         step: float,
         nlo: bool = True,
         first_index: int = 0,
+        eps: float = float(torch.finfo(torch.float64).eps),
+        history: tuple[torch.Tensor, torch.Tensor] | None = None,
     ) -> None:
+        """``history``: (rapidities [capacity], amplitudes [capacity, R]) with
+        row 0 filled, for the kinematical constraint (Euler only); each step
+        appends a row.  ``eps``: the step-underflow epsilon of adaptive RK2."""
+
         require_cuda(r, torch.float64, "r")
         require_cuda(edges, torch.float64, "edges")
         self.r = r
         self.edges = edges
+        self.history = history
         d = _lib.EvolverDesc()
         d.R = r.numel()
         d.bins = edges.shape[-1] - 1
@@ -273,6 +305,19 @@ class Evolver:
         d.nlo = int(nlo)
         d.physics = physics(config)
         d.sobol = table
+        d.abserr = float(config.DE_SOLVER_ABSERR)
+        d.relerr = float(config.DE_SOLVER_RELERR)
+        d.eps = float(eps)
+        if history is not None:
+            hy, hN = history
+            require_cuda(hy, torch.float64, "history rapidities")
+            require_cuda(hN, torch.float64, "history amplitudes")
+            if hN.shape != (hy.numel(), r.numel()):
+                raise ValueError("history must be rapidities [capacity] and amplitudes [capacity, R]")
+            d.kinematical = 1
+            d.history_capacity = hy.numel()
+            d.d_history_y = hy.data_ptr()
+            d.d_history_N = hN.data_ptr()
         self.desc = d
         nbytes = lib().nlo_bk_evolver_workspace(ctypes.byref(d))
         if nbytes == 0:
@@ -303,6 +348,16 @@ class Evolver:
 
     def step(self) -> None:
         check(lib().nlo_bk_evolver_step(self.handle, stream_handle(self.r.device)), "nlo_bk_evolver_step")
+
+    def adaptive_state(self) -> tuple[float, float, int, int]:
+        """(y, next step size, accepted, rejected attempts) of the controller."""
+
+        y, h = ctypes.c_double(), ctypes.c_double()
+        acc, rej = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().nlo_bk_evolver_adaptive_state(self.handle, ctypes.byref(y), ctypes.byref(h),
+                                                   ctypes.byref(acc), ctypes.byref(rej)),
+              "nlo_bk_evolver_adaptive_state")
+        return y.value, h.value, acc.value, rej.value
 
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:

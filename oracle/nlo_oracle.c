@@ -27,16 +27,39 @@ int orc_num_threads(void) {
 
 /* ---------------------------------------------------------------- coupling */
 
-/* S/coupling.py:81-97 (_smooth_alpha_s): alpha = 1 / (b0 * 0.2 *
- * logaddexp(2/0.2 ln 2.5, ln(4 C2 / (r^2 Lambda^2)) / 0.2)). */
+/* _alpha_s (S/bk_kernels/k1.py:158-169, S/bk/derivatives.py:775-786) and
+ * bk_alpha_s (S/coupling.py:11-46): FIXED_AS when either coupling choice is
+ * fixed; for NF <= 3 the smooth coupling of S/coupling.py:81-97, alpha =
+ * 1 / (b0 * 0.2 * logaddexp(2/0.2 ln 2.5, ln(4 C2 / (r^2 Lambda^2)) / 0.2));
+ * for NF > 3 one-loop running with the flavour number and Lambda switched at
+ * the scales 1.3 and 4.5 (4 C2 / r^2 in GeV^2), capped at 1. */
 double orc_alpha_s(double r, const orc_bk_params* p) {
-    const double b0 = (11.0 * p->nc - 2.0 * p->nf) / (12.0 * M_PI);
-    const double left = 2.0 / 0.2 * log(2.5);
-    const double scale = 4.0 * p->c2 / (r * r * p->lambda_qcd * p->lambda_qcd);
-    const double right = log(scale) / 0.2;
-    const double hi = left > right ? left : right;
-    const double lae = hi + log1p(exp(-fabs(left - right)));
-    return 1.0 / (b0 * (0.2 * lae));
+    if (p->rc_lo == ORC_RC_LO_FIXED || p->rc_nlo == ORC_RC_NLO_FIXED) return p->fixed_as;
+    if (p->nf <= 3.0) {
+        const double b0 = (11.0 * p->nc - 2.0 * p->nf) / (12.0 * M_PI);
+        const double left = 2.0 / 0.2 * log(2.5);
+        const double scale = 4.0 * p->c2 / (r * r * p->lambda_qcd * p->lambda_qcd);
+        const double right = log(scale) / 0.2;
+        const double hi = left > right ? left : right;
+        const double lae = hi + log1p(exp(-fabs(left - right)));
+        return 1.0 / (b0 * (0.2 * lae));
+    }
+    const double dipolescale = 4.0 * p->c2 / (r * r);
+    double nf, lambda;
+    if (dipolescale < 1.3 * 1.3) {
+        nf = 3.0;
+        lambda = 0.146159;
+    } else if (dipolescale < 4.5 * 4.5) {
+        nf = 4.0;
+        lambda = 0.122944;
+    } else {
+        nf = 5.0;
+        lambda = 0.0904389;
+    }
+    const double b0 = 11.0 - 2.0 * nf / 3.0;
+    const double log_arg = 4.0 * p->c2 / (r * r * (lambda * lambda));
+    const double a = 4.0 * M_PI / (b0 * log(log_arg));
+    return (log_arg < 1.0 || a > 1.0) ? 1.0 : a;
 }
 
 /* S/bk/initial_conditions.py:29-35. The constant e is truncated to 2.7182818. */
@@ -174,8 +197,17 @@ static double dist_sq(double r, double z, double theta, int cos_form) {
     return (r - z) * (r - z) + 4.0 * r * z * s * s;
 }
 
-/* S/bk_kernels/k1.py:20-155 for RC_LO = BALITSKY_LO, RESUM_RC = SMALLEST,
- * Order = NLO_RESUM_DLOG_SLOG (dlog = 0 in k1fin). */
+static int order_has_resummation(int order) {
+    return order != ORC_ORDER_LO && order != ORC_ORDER_NLO;
+}
+
+static int order_resums_slog(int order) {
+    return order == ORC_ORDER_NLO_RESUM_DLOG_SLOG || order == ORC_ORDER_LO_RESUM_DLOG_SLOG;
+}
+
+/* Kernel_lo, S/bk_kernels/k1.py:20-155, for every RC_LO / RESUM_RC / Order
+ * choice.  The k1fin double-log term carries dlog = 0 for every order that
+ * reaches it (all of them resum the double logs). */
 static double kernel_lo_impl(double r, double z, double theta, const orc_bk_params* p,
                              double* magnitude) {
     const double Y = z;
@@ -188,42 +220,101 @@ static double kernel_lo_impl(double r, double z, double theta, const orc_bk_para
     double mn = sX < sY ? sX : sY;
     mn = mn < r ? mn : r;
     const double a_r = orc_alpha_s(r, p);
-    const double a_Y = orc_alpha_s(sY, p);
-    const double a_X = orc_alpha_s(sX, p);
-    double lo = p->nc / (2.0 * M_PI * M_PI) * a_r *
-                (r * r / (sX * sX * sY * sY) + (a_Y / a_X - 1.0) / (sY * sY) +
-                 (a_X / a_Y - 1.0) / (sX * sX));
+    const double pref = p->nc / (2.0 * M_PI * M_PI);
+    const double rxy2 = sqr(r / (sX * sY));
+    double lo, lo_abs, a_scale;
+    switch (p->rc_lo) {
+        case ORC_RC_LO_SMALLEST:
+            a_scale = orc_alpha_s(mn, p);
+            lo = p->nc * a_scale / (2.0 * M_PI * M_PI) * rxy2;
+            lo_abs = fabs(lo);
+            break;
+        case ORC_RC_LO_PARENT:
+            a_scale = a_r;
+            lo = p->nc * a_r / (2.0 * M_PI * M_PI) * rxy2;
+            lo_abs = fabs(lo);
+            break;
+        case ORC_RC_LO_FAC: {
+            const double ab_r = a_r * p->nc / M_PI;
+            const double ab_X = orc_alpha_s(sX, p) * p->nc / M_PI;
+            const double ab_Y = orc_alpha_s(sY, p) * p->nc / M_PI;
+            a_scale = a_r;
+            const double den =
+                1.0 / ab_r + (sX * sX - sY * sY) / (r * r) * (ab_X - ab_Y) / (ab_X * ab_Y);
+            lo = 1.0 / (2.0 * M_PI) / den * rxy2;
+            /* rounding scale: the denominator's terms in absolute value, the
+             * differences X^2 - Y^2 and abar_X - abar_Y by their sums */
+            lo_abs = fabs(lo) / fabs(den) *
+                     (1.0 / ab_r + (sX * sX + sY * sY) / (r * r) * (ab_X + ab_Y) / (ab_X * ab_Y));
+            break;
+        }
+        case ORC_RC_LO_BEUF: {
+            const double expo = (sX * sX - sY * sY) / (r * r), base = sY * sY / (sX * sX);
+            const double reff2 = r * r * pow(base, expo);
+            a_scale = orc_alpha_s(sqrt(reff2), p);
+            lo = p->nc * a_scale / (2.0 * M_PI * M_PI) * sqr(r / (sX * sY + 1e-40));
+            /* rounding scale: ln r_eff^2 = ln r^2 + expo ln base carries the
+             * rounding of X^2 times |expo| and that of X^2 - Y^2 times |ln base|
+             * (relative condition numbers; the coupling's log-derivative < 1) */
+            const double kx = (r * r + z * z + 2.0 * r * z) / (sX * sX);
+            lo_abs = fabs(lo) * (1.0 + 2.0 * fabs(expo) * kx +
+                                 fabs(log(base)) * (sX * sX + sY * sY) / (r * r));
+            break;
+        }
+        default: { /* BALITSKY_LO, FIXED_LO */
+            const double a_Y = orc_alpha_s(sY, p);
+            const double a_X = orc_alpha_s(sX, p);
+            a_scale = a_r;
+            lo = pref * a_r *
+                 (r * r / (sX * sX * sY * sY) + (a_Y / a_X - 1.0) / (sY * sY) +
+                  (a_X / a_Y - 1.0) / (sX * sX));
+            lo_abs = pref * a_r *
+                     (r * r / (sX * sX * sY * sY) + (a_Y / a_X + 1.0) / (sY * sY) +
+                      (a_X / a_Y + 1.0) / (sX * sX));
+            break;
+        }
+    }
     if (!isfinite(lo)) lo = 0.0;
     if (invalid) lo = 0.0;
+    if (!order_has_resummation(p->order)) {
+        if (magnitude) *magnitude = invalid ? 0.0 : (isfinite(lo_abs) ? lo_abs : 0.0);
+        return lo;
+    }
 
-    const double a_min = orc_alpha_s(mn, p);
+    double a_res;
+    if (p->resum_rc == ORC_RESUM_RC_PARENT)
+        a_res = a_r;
+    else if (p->resum_rc == ORC_RESUM_RC_FIXED)
+        a_res = p->fixed_as;
+    else
+        a_res = orc_alpha_s(mn, p);
     const double xl = 4.0 * log(sX / r) * log(sY / r);
-    const double as_x = sqrt(a_min * p->nc / M_PI * fabs(xl));
+    const double as_x = sqrt(a_res * p->nc / M_PI * fabs(xl));
     const double safe = as_x != 0.0 ? as_x : 1.0;
     double dlog = xl >= 0.0 ? orc_bessel_j1(2.0 * as_x) / safe : orc_bessel_i1(2.0 * as_x) / safe;
     if (as_x == 0.0) dlog = 1.0;
     if (!(r > 1.01 * p->minr)) dlog = 1.0;
     const int resum_valid = isfinite(dlog);
 
-    const double minxy = sX < sY ? sX : sY;
-    const double abar = a_min * p->nc / M_PI;
-    const double A1 = 11.0 / 12.0;
-    const double slog = exp(-abar * A1 * fabs(log(p->ksub * sqr(r / minxy))));
-    const double slog_exp = -abar * A1 * fabs(2.0 * log(sqrt(p->ksub) * r / minxy));
-    const double lo_single = a_r * p->nc / (2.0 * M_PI * M_PI) * sqr(r / (sX * sY));
+    double slog = 1.0, slog_exp = 0.0;
+    if (order_resums_slog(p->order)) {
+        const double minxy = sX < sY ? sX : sY;
+        const double abar = a_res * p->nc / M_PI;
+        const double A1 = 11.0 / 12.0;
+        slog = exp(-abar * A1 * fabs(log(p->ksub * sqr(r / minxy))));
+        slog_exp = -abar * A1 * fabs(2.0 * log(sqrt(p->ksub) * r / minxy));
+    }
+    const double lo_single = a_scale * p->nc / (2.0 * M_PI * M_PI) * rxy2;
     const double subtract = lo_single * slog_exp;
-    const double k1fin = lo * a_r * p->nc / (4.0 * M_PI) *
-                         (67.0 / 9.0 - M_PI * M_PI / 3.0 - 10.0 / 9.0 * p->nf / p->nc);
+    const double fin = a_scale * p->nc / (4.0 * M_PI) *
+                       (67.0 / 9.0 - M_PI * M_PI / 3.0 - 10.0 / 9.0 * p->nf / p->nc);
+    const double k1fin = lo * fin;
     const double result = dlog * slog * lo - subtract + k1fin;
     if (magnitude) {
-        const double lo_abs = p->nc / (2.0 * M_PI * M_PI) * a_r *
-                              (r * r / (sX * sX * sY * sY) + (a_Y / a_X + 1.0) / (sY * sY) +
-                               (a_X / a_Y + 1.0) / (sX * sX));
         *magnitude = (invalid || !resum_valid)
                          ? 0.0
                          : fabs(dlog * slog) * lo_abs + fabs(subtract) + fabs(k1fin) +
-                               fabs(lo_abs * a_r * p->nc / (4.0 * M_PI) *
-                                    (67.0 / 9.0 - M_PI * M_PI / 3.0 - 10.0 / 9.0 * p->nf / p->nc));
+                               fabs(lo_abs * fin);
     }
     return (invalid || !resum_valid) ? 0.0 : result;
 }
@@ -274,12 +365,90 @@ double orc_k1_integrand_magnitude(double r, double z, double theta, const orc_sp
     return orc_kernel_lo_magnitude(r, z, theta, p) * (nX + nY + nr + nX * nY);
 }
 
+/* NaturalCubicSpline(x, y)(v) on six nodes (S/numerics/interpolation.py:10-68):
+ * second derivatives from the Thomas algorithm, interval by
+ * searchsorted(right) - 1 clamped to [0, 4]. */
+static double natural6(const double* x, const double* y, double v) {
+    double h[5], sec[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0}, up[4], sol[4];
+    for (int i = 0; i < 5; ++i) h[i] = x[i + 1] - x[i];
+    for (int i = 0; i < 4; ++i) {
+        const double diag = 2.0 * (h[i] + h[i + 1]);
+        const double rhs = 6.0 * ((y[i + 2] - y[i + 1]) / h[i + 1] - (y[i + 1] - y[i]) / h[i]);
+        if (i == 0) {
+            up[0] = h[1] / diag;
+            sol[0] = rhs / diag;
+        } else {
+            const double den = diag - h[i] * up[i - 1];
+            up[i] = h[i + 1] / den;
+            sol[i] = (rhs - h[i] * sol[i - 1]) / den;
+        }
+    }
+    for (int i = 2; i >= 0; --i) sol[i] -= up[i] * sol[i + 1];
+    for (int i = 0; i < 4; ++i) sec[i + 1] = sol[i];
+    int k = 0;
+    while (k < 5 && x[k + 1] <= v) ++k;
+    k = k > 4 ? 4 : k;
+    const double b = (y[k + 1] - y[k]) / h[k] - h[k] * (2.0 * sec[k] + sec[k + 1]) / 6.0;
+    const double c = sec[k] / 2.0, d = (sec[k + 1] - sec[k]) / (6.0 * h[k]);
+    const double dx = v - x[k];
+    return y[k] + dx * (b + dx * (c + dx * d));
+}
+
+double orc_history_N(double r, double rapidity, const orc_spline* sN, const orc_history* h) {
+    if (rapidity < 0.0) rapidity = 0.0;
+    if (rapidity > h->y[h->rows - 1]) return NAN;
+    int yind = -1; /* searchsorted(history_y, rapidity, right=True) - 1 */
+    while (yind + 1 < h->rows && h->y[yind + 1] <= rapidity) ++yind;
+    yind = yind < 0 ? 0 : yind;
+    double result;
+    if (yind == h->rows - 1) {
+        result = orc_loglog_eval(sN, r, 0.0, 1.0);
+    } else {
+        int rind = -1;
+        while (rind + 1 < h->n && h->r[rind + 1] <= r) ++rind;
+        const int start = rind - 3 > 0 ? rind - 3 : 0;
+        if (start + 6 > h->n) {
+            result = 1.0;
+        } else {
+            const double* x = h->r + start;
+            double v = r;
+            if (v < x[0]) v = x[0] * 1.00001;
+            if (v > x[5]) v = x[5] * 0.999999;
+            const double lower = natural6(x, h->N + (size_t)yind * h->n + start, v);
+            const double upper = natural6(x, h->N + (size_t)(yind + 1) * h->n + start, v);
+            const double frac = (rapidity - h->y[yind]) / (h->y[yind + 1] - h->y[yind]);
+            result = lower + frac * (upper - lower);
+        }
+    }
+    return result < 0.0 ? 0.0 : (result > 1.0 ? 1.0 : result);
+}
+
+double orc_k1_integrand_history(double r, double z, double theta, const orc_spline* sN,
+                                const orc_bk_params* p, const orc_history* h) {
+    const double Xsqr = dist_sq(r, z, theta, p->geometry_cos);
+    const int invalid = Xsqr < 1e-40 || z < 1e-20 || r < 1e-20;
+    const double X = sqrt(Xsqr > 0.0 ? Xsqr : 0.0);
+    const double sX = invalid ? 1.0 : X;
+    const double sY = invalid ? 1.0 : z;
+    double nr = orc_loglog_eval(sN, r, 0.0, 1.0);
+    nr = nr < 1.0 ? nr : 1.0;
+    const double m2 = sX * sX < sY * sY ? sX * sX : sY * sY;
+    double delta = log(m2 / (r * r));
+    delta = delta > 0.0 ? delta : 0.0;
+    const double shifted = h->rapidity - delta;
+    if (invalid || shifted < 0.0) return 0.0;
+    const double s02 = 1.0 - orc_history_N(sX, shifted, sN, h);
+    const double s12 = 1.0 - orc_history_N(sY, shifted, sN, h);
+    return orc_kernel_lo(r, z, theta, p) * (-s02 * s12 + (1.0 - nr));
+}
+
 /* K/bk/k1/kernel.cu:2040-2092: sum over panels [x_i, x_{i+1}] of the log-r
  * grid, GL(nr) in log z and GL(nt) in v in [0, 1] with theta = pi v^2,
  * integrand * 2 z^2 * (half_width w_r) * (w_t pi v). */
-void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN, int nr,
-                  const double* gr_x, const double* gr_w, int nt, const double* gt_x,
-                  const double* gt_w, const orc_bk_params* p, double* out) {
+void orc_k1_fixed_history(int parents, const double* parent_r, const orc_spline* sN, int nr,
+                          const double* gr_x, const double* gr_w, int nt, const double* gt_x,
+                          const double* gt_w, const orc_bk_params* p, const orc_history* h,
+                          double* out) {
 #pragma omp parallel for schedule(dynamic, 1)
     for (int pi = 0; pi < parents; ++pi) {
         const double r = parent_r[pi];
@@ -295,13 +464,20 @@ void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN, int
                     const double v = (gt_x[j] + 1.0) / 2.0;
                     const double theta = M_PI * v * v;
                     const double wt = gt_w[j] * M_PI * v;
-                    total += (long double)(orc_k1_integrand(r, z, theta, sN, p) * (2.0 * z * z) *
-                                           wr * wt);
+                    const double f = h ? orc_k1_integrand_history(r, z, theta, sN, p, h)
+                                       : orc_k1_integrand(r, z, theta, sN, p);
+                    total += (long double)(f * (2.0 * z * z) * wr * wt);
                 }
             }
         }
         out[pi] = (double)total;
     }
+}
+
+void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN, int nr,
+                  const double* gr_x, const double* gr_w, int nt, const double* gt_x,
+                  const double* gt_w, const orc_bk_params* p, double* out) {
+    orc_k1_fixed_history(parents, parent_r, sN, nr, gr_x, gr_w, nt, gt_x, gt_w, p, NULL, out);
 }
 
 /* Same rule over |integrand| magnitudes: the rounding scale of K1. */
@@ -456,6 +632,14 @@ void orc_nlo_kernels_stable(double r, double z, double theta_z, double z2, doubl
     out[3] = (-2.0 * A * A * g + hm * D * D) * nf / nc;
 }
 
+/* _nlo_alpha_s (S/bk/derivatives.py:615-632): FIXED_AS, the parent's
+ * coupling or the coupling at the smallest of the six distances. */
+static double nlo_alpha_s(double r, double smallest, const orc_bk_params* p) {
+    if (p->rc_nlo == ORC_RC_NLO_FIXED) return p->fixed_as;
+    if (p->rc_nlo == ORC_RC_NLO_PARENT) return orc_alpha_s(r, p);
+    return orc_alpha_s(smallest, p);
+}
+
 /* S/bk/derivatives.py:490-612 (_nlo_integrand, eager branch) times the
  * exp(2 log z + 2 log z') Jacobian of S/bk/derivatives.py:322-335. */
 static double nlo_impl(double r, double log_z, double log_z2, double theta_z,
@@ -485,7 +669,7 @@ static double nlo_impl(double r, double log_z, double log_z2, double theta_z,
     double mn = r;
     const double cand[5] = {X, Y, X2, Y2, zz};
     for (int i = 0; i < 5; ++i) mn = cand[i] < mn ? cand[i] : mn;
-    const double as = orc_alpha_s(mn, p);
+    const double as = nlo_alpha_s(r, mn, p);
 
     double k = orc_kernel_nlo(r, X, Y, X2, Y2, zz);
     double ks = orc_kernel_nlo(r, X2, Y2, X, Y, zz);
@@ -593,11 +777,16 @@ double orc_nlo_integrand_stable(double r, double log_z, double log_z2, double th
     double lmin = log(r);
     const double cand[5] = {log_x, log_z, log_x2, log_z2, log_zz};
     for (int i = 0; i < 5; ++i) lmin = cand[i] < lmin ? cand[i] : lmin;
-    const double b0 = (11.0 * p->nc - 2.0 * p->nf) / (12.0 * M_PI);
-    const double left = 2.0 / 0.2 * log(2.5);
-    const double right = 5.0 * (log(4.0 * p->c2 / (p->lambda_qcd * p->lambda_qcd)) - 2.0 * lmin);
-    const double hi = left > right ? left : right;
-    const double as = 1.0 / (b0 * (0.2 * (hi + log1p(exp(-fabs(left - right))))));
+    double as;
+    if (p->rc_nlo != ORC_RC_NLO_SMALLEST || p->rc_lo == ORC_RC_LO_FIXED || p->nf > 3.0) {
+        as = nlo_alpha_s(r, exp(lmin), p);
+    } else { /* the smooth coupling from ln d */
+        const double b0 = (11.0 * p->nc - 2.0 * p->nf) / (12.0 * M_PI);
+        const double left = 2.0 / 0.2 * log(2.5);
+        const double right = 5.0 * (log(4.0 * p->c2 / (p->lambda_qcd * p->lambda_qcd)) - 2.0 * lmin);
+        const double hi = left > right ? left : right;
+        as = 1.0 / (b0 * (0.2 * (hi + log1p(exp(-fabs(left - right))))));
+    }
     const double scale = sqr(as * p->nc) / (8.0 * pow(M_PI, 4.0));
     k2r *= scale;
     kfr *= scale;

@@ -21,12 +21,31 @@
 extern "C" {
 #endif
 
-/* Physics choices of BKConfig (S/bk/config.py:66-101) used on the hot path:
- * Balitsky LO coupling, smallest-distance NLO coupling and resummation
- * coupling, NLO_RESUM_DLOG_SLOG order, smooth coupling for NF <= 3. */
+/* Physics choices of BKConfig (S/bk/config.py:9-64, :66-101).  The choice
+ * codes are those of the C interface (include/nlo_gfx950.h); zero is the
+ * production choice in every field. */
+#define ORC_RC_LO_BALITSKY 0
+#define ORC_RC_LO_FIXED 1
+#define ORC_RC_LO_PARENT 2
+#define ORC_RC_LO_SMALLEST 3
+#define ORC_RC_LO_FAC 4
+#define ORC_RC_LO_BEUF 5
+#define ORC_RC_NLO_SMALLEST 0
+#define ORC_RC_NLO_FIXED 1
+#define ORC_RC_NLO_PARENT 2
+#define ORC_RESUM_RC_SMALLEST 0
+#define ORC_RESUM_RC_PARENT 1
+#define ORC_RESUM_RC_FIXED 2
+#define ORC_ORDER_NLO_RESUM_DLOG_SLOG 0
+#define ORC_ORDER_NLO_RESUM_DLOG 1
+#define ORC_ORDER_NLO 2
+#define ORC_ORDER_LO_RESUM_DLOG_SLOG 3
+#define ORC_ORDER_LO_RESUM_DLOG 4
+#define ORC_ORDER_LO 5
+
 typedef struct orc_bk_params {
     double nc;          /* NC */
-    double nf;          /* NF */
+    double nf;          /* NF (0, 3: smooth coupling; 5: flavour thresholds) */
     double lambda_qcd;  /* LambdaQCD */
     double c2;          /* C2 */
     double ksub;        /* KSUB */
@@ -36,7 +55,24 @@ typedef struct orc_bk_params {
                            0: X^2 = (r - z)^2 + 4 r z sin^2(theta / 2) */
     int stable;         /* orc_bk_nlo_sums: 0 = reference integrand on the S
                            spline, 1 = orc_nlo_integrand_stable on the N spline */
+    int rc_lo;          /* ORC_RC_LO_* */
+    int rc_nlo;         /* ORC_RC_NLO_* */
+    int resum_rc;       /* ORC_RESUM_RC_* */
+    int order;          /* ORC_ORDER_* */
+    int reserved;
+    double fixed_as;    /* FIXED_AS */
 } orc_bk_params;
+
+/* Amplitude history for the kinematical constraint (S/bk/derivatives.py:712-758):
+ * rows N(r_grid, y_k), k < rows, and the rapidity of the current derivative. */
+typedef struct orc_history {
+    int rows;
+    int n;               /* r grid points */
+    const double* r;     /* r grid */
+    const double* y;     /* rows rapidities, increasing */
+    const double* N;     /* rows x n */
+    double rapidity;
+} orc_history;
 
 /* Natural cubic spline of log(y) against log(r) (S/numerics/interpolation.py:95-121). */
 typedef struct orc_spline {
@@ -104,6 +140,22 @@ void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN,
                   int nr, const double* gr_x, const double* gr_w,
                   int nt, const double* gt_x, const double* gt_w,
                   const orc_bk_params* p, double* out);
+
+/* The same rule with the kinematical constraint: the integrand of
+ * orc_k1_integrand_history at h->rapidity. */
+void orc_k1_fixed_history(int parents, const double* parent_r, const orc_spline* sN,
+                          int nr, const double* gr_x, const double* gr_w,
+                          int nt, const double* gt_x, const double* gt_w,
+                          const orc_bk_params* p, const orc_history* h, double* out);
+
+/* _interpolate_history_N (S/bk/derivatives.py:712-758): N at (r, rapidity)
+ * from the history rows; the last row is read through sN, the spline of the
+ * current amplitude.  NaN if rapidity is above the last row. */
+double orc_history_N(double r, double rapidity, const orc_spline* sN, const orc_history* h);
+
+/* _lo_integrand with KINEMATICAL_CONSTRAINT (S/bk/derivatives.py:445-468). */
+double orc_k1_integrand_history(double r, double z, double theta, const orc_spline* sN,
+                                const orc_bk_params* p, const orc_history* h);
 
 void orc_k1_fixed_magnitude(int parents, const double* parent_r, const orc_spline* sN,
                             int nr, const double* gr_x, const double* gr_w,

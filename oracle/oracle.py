@@ -50,6 +50,23 @@ class BKParams(ctypes.Structure):
         ("symmetrize", ctypes.c_int),
         ("geometry_cos", ctypes.c_int),
         ("stable", ctypes.c_int),
+        ("rc_lo", ctypes.c_int),
+        ("rc_nlo", ctypes.c_int),
+        ("resum_rc", ctypes.c_int),
+        ("order", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("fixed_as", ctypes.c_double),
+    ]
+
+
+class _History(ctypes.Structure):
+    _fields_ = [
+        ("rows", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("r", _dp),
+        ("y", _dp),
+        ("N", _dp),
+        ("rapidity", ctypes.c_double),
     ]
 
 
@@ -155,6 +172,19 @@ def lib() -> ctypes.CDLL:
             _dp,
         ]
         L.orc_k1_fixed_magnitude.argtypes = L.orc_k1_fixed.argtypes
+        L.orc_k1_fixed_history.argtypes = L.orc_k1_fixed.argtypes[:-1] + [
+            ctypes.POINTER(_History),
+            _dp,
+        ]
+        L.orc_history_N.restype = ctypes.c_double
+        L.orc_history_N.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.POINTER(_Spline),
+                                    ctypes.POINTER(_History)]
+        L.orc_k1_integrand_history.restype = ctypes.c_double
+        L.orc_k1_integrand_history.argtypes = [ctypes.c_double] * 3 + [
+            ctypes.POINTER(_Spline),
+            ctypes.POINTER(BKParams),
+            ctypes.POINTER(_History),
+        ]
         L.orc_sobol_u32.argtypes = [
             ctypes.c_int,
             _u32p,
@@ -226,10 +256,34 @@ def params(
     symmetrize: bool = True,
     geometry_cos: bool = False,
     stable: bool = False,
+    rc_lo: int = 0,
+    rc_nlo: int = 0,
+    resum_rc: int = 0,
+    order: int = 0,
+    fixed_as: float = 0.2,
 ) -> BKParams:
+    """Physics choices as the ORC_* codes of nlo_oracle.h (0 = production)."""
+
     return BKParams(
-        nc, nf, lambda_qcd, c2, ksub, minr, int(symmetrize), int(geometry_cos), int(stable)
+        nc, nf, lambda_qcd, c2, ksub, minr, int(symmetrize), int(geometry_cos), int(stable),
+        int(rc_lo), int(rc_nlo), int(resum_rc), int(order), 0, float(fixed_as),
     )
+
+
+@dataclass
+class History:
+    """Amplitude history rows N(r_grid, y_k) for the kinematical constraint."""
+
+    r: np.ndarray
+    y: np.ndarray
+    N: np.ndarray  # rows x n
+    rapidity: float
+
+    def struct(self) -> _History:
+        self.r = np.ascontiguousarray(self.r, dtype=np.float64)
+        self.y = np.ascontiguousarray(self.y, dtype=np.float64)
+        self.N = np.ascontiguousarray(self.N, dtype=np.float64)
+        return _History(self.y.size, self.r.size, _d(self.r), _d(self.y), _d(self.N), float(self.rapidity))
 
 
 @dataclass
@@ -414,9 +468,11 @@ def nlo_integrand_stable(r, log_z, log_z2, theta_z, theta_z2, sN: Spline, p: BKP
 
 
 def k1_fixed(
-    parent_r: np.ndarray, sN: Spline, nr=8, nt=24, p: BKParams | None = None, magnitude: bool = False
+    parent_r: np.ndarray, sN: Spline, nr=8, nt=24, p: BKParams | None = None, magnitude: bool = False,
+    history: History | None = None,
 ) -> np.ndarray:
-    """K1 on the fixed rule; ``magnitude`` integrates |integrand| (rounding scale)."""
+    """K1 on the fixed rule; ``magnitude`` integrates |integrand| (rounding scale);
+    ``history`` switches the kinematical constraint on."""
 
     p = p or params()
     parent_r = np.ascontiguousarray(parent_r, dtype=np.float64)
@@ -425,12 +481,32 @@ def k1_fixed(
     gx, gw, tx, tw = (np.ascontiguousarray(a) for a in (gx, gw, tx, tw))
     out = np.empty(parent_r.size)
     st = sN.struct()
+    args = (parent_r.size, _d(parent_r), ctypes.byref(st), nr, _d(gx), _d(gw), nt, _d(tx), _d(tw),
+            ctypes.byref(p))
+    if history is not None:
+        if magnitude:
+            raise ValueError("no magnitude rule with the kinematical constraint")
+        hs = history.struct()
+        lib().orc_k1_fixed_history(*args, ctypes.byref(hs), _d(out))
+        return out
     fn = lib().orc_k1_fixed_magnitude if magnitude else lib().orc_k1_fixed
-    fn(
-        parent_r.size, _d(parent_r), ctypes.byref(st), nr, _d(gx), _d(gw), nt, _d(tx), _d(tw),
-        ctypes.byref(p), _d(out),
-    )
+    fn(*args, _d(out))
     return out
+
+
+def history_N(r, rapidity, sN: Spline, history: History) -> np.ndarray:
+    st, hs = sN.struct(), history.struct()
+    return np.array([lib().orc_history_N(float(a), float(b), ctypes.byref(st), ctypes.byref(hs))
+                     for a, b in zip(np.ravel(r), np.ravel(rapidity))])
+
+
+def k1_integrand_history(r, z, theta, sN: Spline, history: History, p: BKParams | None = None):
+    p = p or params()
+    st, hs = sN.struct(), history.struct()
+    return np.array([
+        lib().orc_k1_integrand_history(float(a), float(b), float(c), ctypes.byref(st), ctypes.byref(p),
+                                       ctypes.byref(hs))
+        for a, b, c in zip(np.ravel(r), np.ravel(z), np.ravel(theta))])
 
 
 @dataclass

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_lib.LIBRARY))
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
-    assert _lib.lib().nlo_api_version() == 1
+    assert _lib.lib().nlo_api_version() == 2
 
 
 def _gauss_legendre_mp(n: int):
@@ -81,10 +81,19 @@ def test_invalid_arguments_return_status_and_message():
     assert b"dim" in L.nlo_last_error()
     assert L.nlo_vegas_map(4, 1, 10, None, None, None, None, None, None, None, None) == 1
     assert b"bins" in L.nlo_last_error()
-    ph = _lib.Physics(3.0, 5.0, 0.241, 1.0, 0.65, 1e-6, 1, 0)
-    # NF = 5 needs the threshold coupling, which is outside the implemented set
-    rc = L.nlo_bk_k1(8, 1, 1, 1, ctypes.byref(ph), 8, 24, 1, None)
-    assert rc == 4 and b"NF" in L.nlo_last_error()
+    # physics is validated before any launch: NF must be 0, 3 or 5, fixed
+    # couplings come in pairs, codes must be known
+    for kwargs, word in ((dict(nf=4.0), b"NF"), (dict(rc_lo=1), b"fixed coupling"),
+                         (dict(order=9), b"Order"), (dict(rc_lo=1, rc_nlo=1, fixed_as=0.0), b"FIXED_AS")):
+        ph = _lib.Physics(nc=3.0, nf=3.0, lambda_qcd=0.241, c2=1.0, ksub=0.65, minr=1e-6, symmetrize=1,
+                          fixed_as=0.2)
+        for k, v in kwargs.items():
+            setattr(ph, k, v)
+        rc = L.nlo_bk_k1(8, 1, 1, 1, ctypes.byref(ph), 8, 24, 1, None)
+        assert rc == 1 and word in L.nlo_last_error(), kwargs
+    ph = _lib.Physics(nc=3.0, nf=3.0, lambda_qcd=0.241, c2=1.0, ksub=0.65, minr=1e-6, symmetrize=1)
+    assert L.nlo_bk_k1_history(8, 1, 1, 1, ctypes.byref(ph), 8, 24, 1, 1, 1, 0, 1, None) == 1
+    assert b"capacity" in L.nlo_last_error()
     with pytest.raises(RuntimeError, match="status 1"):
         _lib.check(L.nlo_vegas_adapt(0, 4, 50, None, None, None, 1e-16, None, None), "adapt")
 
@@ -102,14 +111,45 @@ def test_workspace_queries_are_host_only_and_monotone():
     assert L.nlo_bk_evolver_workspace(ctypes.byref(d)) > a
 
 
-def test_struct_layouts_match_the_header():
-    # nlo_sobol_table: dim + 8 x 30 directions + 8 shifts + 8 first points
-    assert ctypes.sizeof(_lib.SobolTable) == 4 * (1 + 8 * 30 + 8 + 8)
-    assert ctypes.sizeof(_lib.Physics) == 6 * 8 + 2 * 4
-    # nlo_dis_dipole: 4 int32, 4 doubles, 2 int32, 3 pointers, 2 doubles, 1 pointer
-    assert ctypes.sizeof(_lib.DisDipole) == 16 + 32 + 8 + 24 + 16 + 8
-    # nlo_dis_desc: Q2, xbj, 2 int32, 8 + 8 doubles, maxr, 4 int32, 2 doubles, 2 int32, dipole
-    assert ctypes.sizeof(_lib.DisDesc) == 16 + 8 + 128 + 8 + 16 + 16 + 8 + ctypes.sizeof(_lib.DisDipole)
+STRUCTS = {
+    "nlo_bk_physics": _lib.Physics,
+    "nlo_sobol_table": _lib.SobolTable,
+    "nlo_bk_evolver_desc": _lib.EvolverDesc,
+    "nlo_dis_dipole": _lib.DisDipole,
+    "nlo_dis_desc": _lib.DisDesc,
+}
+
+
+C_FIELD = {"lambda_": "lambda"}  # Python keyword avoided in the ctypes mirror
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    # sizeof and every field's offsetof from the header itself (host gcc)
+    import shutil
+    import subprocess
+
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "nlo_gfx950.h"', "int main(void) {"]
+    for cname, ct in STRUCTS.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in ct._fields_:
+            cfield = C_FIELD.get(field, field)
+            lines.append(f'printf("{cname} {field} %zu\\n", offsetof({cname}, {cfield}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", f"-I{HEADER.parent}", "-I/opt/rocm/include", str(src),
+                    "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, ct in STRUCTS.items():
+        assert got[(cname, "size")] == ctypes.sizeof(ct), cname
+        for field, _ in ct._fields_:
+            assert got[(cname, field)] == getattr(ct, field).offset, (cname, field)
 
 
 def test_dis_dimensions_and_argument_errors():

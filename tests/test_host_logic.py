@@ -39,11 +39,26 @@ def test_config_validation(kwargs):
         BKConfig(**kwargs)
 
 
-def test_unsupported_physics_is_rejected_loudly():
-    with pytest.raises(NotImplementedError, match="RC_NLO"):
-        require_gpu_supported(BKConfig(RC_NLO=RunningCouplingNLO.PARENT_NLO))
-    with pytest.raises(NotImplementedError, match="NF=5"):
-        require_gpu_supported(BKConfig(NF=5))
+def test_unsupported_integration_is_rejected_loudly():
+    from nlo_bk_optimized_amd.bk.config import IntegrationMethod
+
+    for method in (IntegrationMethod.MISER, IntegrationMethod.MULTIPLE):
+        with pytest.raises(NotImplementedError, match="INTMETHOD_NLO"):
+            require_gpu_supported(BKConfig(INTMETHOD_NLO=method))
+    # every coupling, order and NF choice runs on the device
+    require_gpu_supported(BKConfig(RC_NLO=RunningCouplingNLO.PARENT_NLO, NF=5))
+
+
+def test_auto_rk_method_follows_the_reference_selection():
+    from nlo_bk_optimized_amd.bk.config import BKOrder, resolve_rk_method
+
+    f32, f64 = torch.float32, torch.float64
+    assert resolve_rk_method(BKConfig(), f32) == "kutta3"        # fixed RK2(3) path
+    assert resolve_rk_method(BKConfig(), f64) == "adaptive_rk2"  # float64: adaptive
+    assert resolve_rk_method(BKConfig(RPOINTS=255), f32) == "adaptive_rk2"
+    assert resolve_rk_method(BKConfig(Order=BKOrder.LO), f32) == "adaptive_rk2"
+    assert resolve_rk_method(BKConfig(EULER_METHOD=True), f32) == "euler"
+    assert resolve_rk_method(BKConfig(RK_METHOD="rk4", RPOINTS=255), f32) == "rk4"
 
 
 def test_r_grid_matches_oracle():

@@ -174,11 +174,16 @@ def test_loglog_spline_range_rules():
     np.testing.assert_allclose(O.loglog_eval(sN, r[1:-1], 0.0, 1.0), N[1:-1], rtol=1e-13)
 
 
-def test_k1_fixed_rule_against_adaptive_quadrature():
+# production; BEUF with parent resummation and Nf = 5; FAC with the LO order
+K1_RULE_CASES = ({}, {"rc_lo": 5, "resum_rc": 1, "nf": 5.0}, {"rc_lo": 4, "order": 5})
+
+
+@pytest.mark.parametrize("options", K1_RULE_CASES)
+def test_k1_fixed_rule_against_adaptive_quadrature(options):
     r, N = workload(R=24)
     sN, _ = O.evolution_splines(r, N)
-    p = O.params(minr=1e-6)
-    for idx in (8, 16):
+    p = O.params(minr=1e-6, **options)
+    for idx in ((8, 16) if not options else (16,)):
         rp = r[idx]
         fixed = O.k1_fixed(np.array([rp]), sN, p=p)[0]
 
