@@ -67,6 +67,11 @@ def parse() -> argparse.Namespace:
     p.add_argument("--cpu-samples-log2", type=int, default=20,
                    help="Sobol points per parent in the CPU baseline sample")
     p.add_argument("--no-dis", action="store_true", help="skip the DIS VEGAS-pass measurements")
+    p.add_argument("--no-hbm", action="store_true", help="skip the Sobol / VEGAS map / histogram bandwidth runs")
+    p.add_argument("--batch-runs", type=int, default=64,
+                   help="evolutions in the batched uneven-grid run (0 skips it)")
+    p.add_argument("--dis-points", type=int, default=32,
+                   help="(Q^2, x) points of the NLO DIS batch (512 = the full 32 x 16 grid; 0 skips it)")
     p.add_argument("--kernel-probe", action="store_true",
                    help="only run the FP32 K2+Kf launch and a 256 MiB device copy (for counter runs)")
     return p.parse_args()
@@ -217,6 +222,189 @@ def main() -> int:
                 ms = e0.elapsed_time(e1) / reps
                 extra[f"dis_{kind}_T_{prec}_samples_per_s"] = nd / (ms * 1e-3)
                 log(f"dis {kind} T {prec}: {ms:.3f} ms per 2^22-sample VEGAS pass")
+
+    if args.dis_points > 0:
+        # NLO DIS batch: (Q^2, x) on 32 log Q^2 in [1, 100] x 16 log x in
+        # [1e-5, 1e-2] (every k-th point of that grid unless --dis-points 512),
+        # light (m = 0.14) + charm (m = 1.27), T and L, LO + dipole + qqbar-g,
+        # 2^22 VEGAS samples per integral (all eight iterations run: epsrel is
+        # below reach), N(r, Y) from the 256-node evolution taken to Y = 12 so
+        # that every qqbar-g rapidity ln(W^2 z2) lies inside the table.  The
+        # same batch in FP32 and FP64 gives the relative F2 / FL / sigma_r
+        # differences (sigma_r at sqrt(s) = 318 GeV where y <= 1).
+        import numpy as np
+
+        from nlo_bk_optimized_amd import solve_bk
+        from nlo_bk_optimized_amd.dipole import BKDipole
+        from nlo_bk_optimized_amd.dis import FL, FT
+        from nlo_bk_optimized_amd.dis.config import DISConfig, DISOrder, Quark, QuarkType
+
+        log("DIS batch: BK table to Y = 12 (untimed)")
+        table = solve_bk(ic, 12.0, BKConfig(PRECISION="fp32", **base), device=device, seed=0)
+        dipole = BKDipole(table, device=device)
+        dcfg = DISConfig(order=DISOrder.NLO, quarks=(Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)),
+                         maxeval=1 << 22, epsrel=1e-12)
+        grid = [(q, x) for q in np.geomspace(1.0, 100.0, 32) for x in np.geomspace(1e-5, 1e-2, 16)]
+        stride = max(1, len(grid) // args.dis_points)
+        points = grid[::stride][: args.dis_points]
+        res = {}
+        for dtype, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            vals = []
+            for q2, xb in points:
+                Q2, xbj = torch.tensor(q2, dtype=dtype), torch.tensor(xb, dtype=dtype)
+                ft = FT(Q2, xbj, dipole, dcfg, seed=0)
+                fl = FL(Q2, xbj, dipole, dcfg, seed=1000)
+                vals.append((ft.value.item(), fl.value.item(), ft.error.item(), fl.error.item()))
+            torch.cuda.synchronize()
+            res[name] = (np.array(vals), time.perf_counter() - t0)
+            log(f"DIS batch {name}: {len(points)} points in {res[name][1]:.2f} s")
+        v32, v64 = res["fp32"][0], res["fp64"][0]
+        f2_32, f2_64 = v32[:, 0] + v32[:, 1], v64[:, 0] + v64[:, 1]
+        q2a = np.array([p[0] for p in points])
+        xa = np.array([p[1] for p in points])
+        yv = q2a / (xa * 318.0**2)
+        ok = yv <= 1
+        c = 1 - yv**2 / (1 + (1 - yv) ** 2)
+        sr32, sr64 = v32[:, 0] + c * v32[:, 1], v64[:, 0] + c * v64[:, 1]
+        integrals = len(points) * 2 * len(dcfg.quarks) * 5  # VEGAS integrals per precision
+        extra.update({
+            "dis_batch_points": len(points),
+            "dis_batch_fp32_s": res["fp32"][1],
+            "dis_batch_fp64_s": res["fp64"][1],
+            "dis_batch_speedup_fp32_vs_fp64": res["fp64"][1] / res["fp32"][1],
+            "dis_batch_fp32_samples_per_s": integrals * (1 << 22) / res["fp32"][1],
+            "dis_max_rel_dF2_fp32_vs_fp64": float(np.max(np.abs(f2_32 - f2_64) / np.abs(f2_64))),
+            "dis_max_rel_dFL_fp32_vs_fp64": float(np.max(np.abs(v32[:, 1] - v64[:, 1]) / np.abs(v64[:, 1]))),
+            "dis_max_rel_dsigma_r_fp32_vs_fp64": (float(np.max(np.abs(sr32 - sr64)[ok] / np.abs(sr64[ok])))
+                                                  if ok.any() else None),
+            "dis_sigma_r_points": int(ok.sum()),
+            "dis_median_rel_stderr_F2": float(np.median(np.hypot(v64[:, 2], v64[:, 3]) / np.abs(f2_64))),
+        })
+        log(f"DIS batch: max rel dF2 {extra['dis_max_rel_dF2_fp32_vs_fp64']:.2e}, "
+            f"max rel dsigma_r {extra['dis_max_rel_dsigma_r_fp32_vs_fp64']}")
+
+    if args.batch_runs > 0:
+        # Batched uneven evolutions: runs with Qs0^2 ~ U[0.1, 0.5], gamma ~
+        # U[0.9, 1.2], e_c ~ U[1, 20], R ~ U{128..512} (numpy default_rng seed
+        # 20261202), 2^18 Sobol points per parent, midpoint RK2 to Y = 8 with
+        # dY = 0.1, FP32 integrand; all runs in one set of launches per stage
+        # (launch shape tuned during the warmup steps), against the same runs
+        # one after another with the single-run driver.
+        import numpy as np
+
+        rng = np.random.default_rng(20261202)
+        nb = 1 << 18
+        qs = rng.uniform(0.1, 0.5, args.batch_runs)
+        gam = rng.uniform(0.9, 1.2, args.batch_runs)
+        ecs = rng.uniform(1.0, 20.0, args.batch_runs)
+        Rs = rng.integers(128, 513, args.batch_runs)
+        bsteps = 80
+        setups, cfgs = [], []
+        t0 = time.perf_counter()
+        for q, g, ec, R in zip(qs, gam, ecs, Rs):
+            cfg = BKConfig(**{**base, "RPOINTS": int(R) - 1, "MCINTPOINTS": nb, "PRECISION": "fp32"})
+            ic_b = MV(qs0sqr=float(q), x0=0.01, ec=float(ec), lambdaqcd=0.241, gamma=float(g))
+            setups.append(evo.prepare(ic_b, cfg, device=device, seed=0))
+            cfgs.append(cfg)
+        torch.cuda.synchronize()
+        t_prep = time.perf_counter() - t0
+        log(f"batch: {args.batch_runs} runs, sum R = {int(Rs.sum())}, grids adapted in {t_prep:.1f} s")
+        rb = torch.cat([s.r for s in setups])
+        eb = torch.cat([s.edges for s in setups]).contiguous()
+        N0b = torch.cat([s.N0 for s in setups])
+        bev = dev.BatchEvolver(rb, eb, [int(v) for v in Rs], setups[0].table, cfgs[0], n_samples=nb,
+                               precision="fp32", rk_method="rk2", step=0.1)
+        bev.state.copy_(N0b)
+        for _ in range(2):  # the first step tunes the K2 + Kf launch
+            bev.step()
+        bev.state.copy_(N0b)
+        torch.cuda.synchronize()
+        c0 = int(bev.active_evaluations.item())
+        t0 = time.perf_counter()
+        for _ in range(bsteps):
+            bev.step()
+        torch.cuda.synchronize()
+        t_batch = time.perf_counter() - t0
+        b_evals = int(bev.active_evaluations.item()) - c0
+        Nb = bev.state.clone()
+        tuning = bev.tuning()
+        bflag = int(bev.flag.item())
+        bev.close()
+        t_seq, s_evals, off, dmax = 0.0, 0, 0, 0.0
+        for s, cfg in zip(setups, cfgs):
+            ev1 = dev.Evolver(s.r, s.edges, s.table, cfg, n_samples=nb, precision="fp32", rk_method="rk2",
+                              step=0.1)
+            ev1.state.copy_(s.N0)
+            ev1.step()
+            ev1.state.copy_(s.N0)
+            torch.cuda.synchronize()
+            c1 = int(ev1.active_evaluations.item())
+            t0 = time.perf_counter()
+            for _ in range(bsteps):
+                ev1.step()
+            torch.cuda.synchronize()
+            t_seq += time.perf_counter() - t0
+            s_evals += int(ev1.active_evaluations.item()) - c1
+            R = s.r.numel()
+            dmax = max(dmax, float((ev1.state - Nb[off:off + R]).abs().max()))
+            off += R
+            ev1.close()
+        extra.update({
+            "batch_runs": args.batch_runs,
+            "batch_sum_r": int(Rs.sum()),
+            "batch_wall_s": t_batch,
+            "batch_samples_per_s": b_evals * nb / t_batch,
+            "batch_sequential_wall_s": t_seq,
+            "batch_speedup_vs_sequential": t_seq / t_batch,
+            "batch_tuned_spt": tuning[0],
+            "batch_tuned_grid_blocks": tuning[1],
+            "batch_max_abs_dN_vs_sequential": dmax,
+            "batch_nonfinite": bflag,
+            "batch_same_evaluations": b_evals == s_evals,
+        })
+        log(f"batch: {t_batch:.2f} s ({b_evals * nb / t_batch:.3e} samples/s), one at a time {t_seq:.2f} s, "
+            f"tuned spt {tuning[0]} grid {tuning[1]}, max |dN| vs single runs {dmax:.1e}")
+        del setups, rb, eb, N0b, Nb
+
+    if not args.no_hbm:
+        # The HBM-bound sampling kernels on their own, 2^24 points x 4
+        # dimensions: bytes moved per launch / launch time against 8 TB/s.
+        # Sobol f32: writes 16 B per point; VEGAS map: reads u (32 B), writes
+        # x (32 B), jac (8 B) and bins (4 B); histogram: reads bins (4 B) and
+        # |w| (8 B).
+        nh = 1 << 24
+        table_h = dev.sobol_table(4, 0)
+        edges_h = torch.linspace(0, 1, 51, dtype=torch.float64, device=device).expand(4, -1).contiguous()
+        lo_h = torch.zeros(4, dtype=torch.float64, device=device)
+        wd_h = torch.ones(4, dtype=torch.float64, device=device)
+        u_h = dev.sobol_draw(table_h, nh, kind="f64", device=device)
+        x_h, jac_h, b_h = dev.vegas_map(u_h, edges_h, lo_h, wd_h)
+
+        def timed(fn, reps=5):
+            fn()
+            st = torch.cuda.current_stream()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            for _ in range(reps):
+                fn()
+            e1.record(st)
+            e1.synchronize()
+            return e0.elapsed_time(e1) / reps * 1e-3
+
+        for name, fn, nbytes in (
+            ("sobol_f32", lambda: dev.sobol_draw(table_h, nh, kind="f32", device=device), 16 * nh),
+            ("sobol_f64", lambda: dev.sobol_draw(table_h, nh, kind="f64", device=device), 32 * nh),
+            ("vegas_map", lambda: dev.vegas_map(u_h, edges_h, lo_h, wd_h), 76 * nh),
+            ("vegas_histogram", lambda: dev.vegas_histogram(b_h, jac_h, 50), 12 * nh),
+        ):
+            sec = timed(fn)
+            extra[f"hbm_{name}_GBps"] = nbytes / sec / 1e9
+            extra[f"hbm_{name}_frac"] = nbytes / sec / 8e12
+            log(f"{name}: {sec * 1e3:.3f} ms for 2^24 x 4, {nbytes / sec / 1e9:.0f} GB/s "
+                f"({100 * nbytes / sec / 8e12:.1f} % of 8 TB/s)")
+        del u_h, x_h, jac_h, b_h
 
     cpu = None
     if not args.no_cpu_baseline:

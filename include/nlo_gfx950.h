@@ -292,6 +292,55 @@ int nlo_bk_evolver_adaptive_state(const nlo_bk_evolver* ev, double* y, double* h
                                   int64_t* rejected);
 int nlo_bk_evolver_destroy(nlo_bk_evolver* ev);
 
+/* ------------------------------------------------------------- batches */
+
+/* Many evolutions advanced together (the batched workload of 64 runs of
+ * uneven grid sizes): run i has h_R[i] r nodes; d_r (sum R) and d_edges
+ * (sum R x 4 x (bins + 1), row 0 of each run unused) concatenate the runs'
+ * grids and frozen VEGAS grids.  Every run shares the physics, Sobol table,
+ * samples per parent and step.  Per RK stage: one spline-build launch (a
+ * block per run), one K1 launch (a block per node), one persistent K2 + Kf
+ * launch whose blocks take (parent, chunk) items from a device counter, one
+ * fixed-order reduce and one finalize, captured as one hipGraph per step.
+ * spt (samples per lane: 4, 8, 16) and grid_blocks (persistent grid) may be
+ * given or left 0: the first nlo_bk_batch_step then times the candidates on
+ * the state the caller set and keeps the fastest (nlo_bk_batch_tuning reports
+ * the choice).  Fixed-step methods only.  Replaces running solve_bk
+ * (S/bk/evolution.py:33-58) once per initial condition. */
+typedef struct nlo_bk_batch_desc {
+    int32_t runs;
+    int32_t bins;
+    int64_t n_samples;
+    int64_t first_index;
+    int32_t precision;
+    int32_t rk_method;      /* NLO_RK_EULER .. NLO_RK_CLASSIC4 */
+    double step;
+    int32_t force_positive;
+    int32_t nr, nt;
+    int32_t nlo;
+    int32_t spt;            /* 0 = tune */
+    int32_t grid_blocks;    /* 0 = tune */
+    nlo_bk_physics physics;
+    nlo_sobol_table sobol;
+} nlo_bk_batch_desc;
+
+typedef struct nlo_bk_batch nlo_bk_batch;
+
+size_t nlo_bk_batch_workspace(const nlo_bk_batch_desc* desc, const int32_t* h_R);
+int nlo_bk_batch_create(const nlo_bk_batch_desc* desc, const int32_t* h_R, const double* d_r,
+                        const double* d_edges, void* d_workspace, size_t workspace_bytes,
+                        hipStream_t stream, nlo_bk_batch** out);
+/* Concatenated state N (sum R doubles, run order), non-finite flag, active
+ * (parent, right-hand side) evaluation counter. */
+double* nlo_bk_batch_state(nlo_bk_batch* batch);
+int32_t* nlo_bk_batch_nonfinite_flag(nlo_bk_batch* batch);
+uint64_t* nlo_bk_batch_active_evaluations(nlo_bk_batch* batch);
+int nlo_bk_batch_step(nlo_bk_batch* batch, hipStream_t stream);
+/* The K2 + Kf launch shape in use and its measured time per launch. */
+int nlo_bk_batch_tuning(const nlo_bk_batch* batch, int32_t* spt, int32_t* grid_blocks,
+                        double* ms_per_launch);
+int nlo_bk_batch_destroy(nlo_bk_batch* batch);
+
 /* ================================================================== DIS */
 
 /* Special functions on device arrays (n doubles in, n doubles out), in FP64

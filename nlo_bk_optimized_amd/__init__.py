@@ -19,6 +19,7 @@ from nlo_bk_optimized_amd.bk import (
     RunningCouplingNLO,
     compute_dndy,
     solve_bk,
+    solve_bk_batch,
 )
 from nlo_bk_optimized_amd.dipole import GBW, BKDipole, DipoleTable, load_bk_table, save_bk_table
 from nlo_bk_optimized_amd.dis import F2, FL, FT, DISConfig, DISOrder, Polarization, Quark, QuarkType, sigma_r
@@ -37,6 +38,7 @@ __all__ = [
     "Quark",
     "QuarkType",
     "sigma_r",
+    "solve_bk_batch",
     "DNDYResult",
     "DipoleTable",
     "ICDataFile",

@@ -95,6 +95,28 @@ class EvolverDesc(ctypes.Structure):
     ]
 
 
+class BatchDesc(ctypes.Structure):
+    """nlo_bk_batch_desc."""
+
+    _fields_ = [
+        ("runs", ctypes.c_int32),
+        ("bins", ctypes.c_int32),
+        ("n_samples", ctypes.c_int64),
+        ("first_index", ctypes.c_int64),
+        ("precision", ctypes.c_int32),
+        ("rk_method", ctypes.c_int32),
+        ("step", ctypes.c_double),
+        ("force_positive", ctypes.c_int32),
+        ("nr", ctypes.c_int32),
+        ("nt", ctypes.c_int32),
+        ("nlo", ctypes.c_int32),
+        ("spt", ctypes.c_int32),
+        ("grid_blocks", ctypes.c_int32),
+        ("physics", Physics),
+        ("sobol", SobolTable),
+    ]
+
+
 class DisDipole(ctypes.Structure):
     """nlo_dis_dipole."""
 
@@ -204,6 +226,20 @@ SIGNATURES: dict[str, tuple] = {
         [_vp, ctypes.POINTER(_f64), ctypes.POINTER(_f64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)],
     ),
     "nlo_bk_evolver_destroy": (ctypes.c_int, [_vp]),
+    "nlo_bk_batch_workspace": (_size, [ctypes.POINTER(BatchDesc), _vp]),
+    "nlo_bk_batch_create": (
+        ctypes.c_int,
+        [ctypes.POINTER(BatchDesc), _vp, _vp, _vp, _vp, _size, _vp, ctypes.POINTER(_vp)],
+    ),
+    "nlo_bk_batch_state": (_vp, [_vp]),
+    "nlo_bk_batch_nonfinite_flag": (_vp, [_vp]),
+    "nlo_bk_batch_active_evaluations": (_vp, [_vp]),
+    "nlo_bk_batch_step": (ctypes.c_int, [_vp, _vp]),
+    "nlo_bk_batch_tuning": (
+        ctypes.c_int,
+        [_vp, ctypes.POINTER(_i32), ctypes.POINTER(_i32), ctypes.POINTER(_f64)],
+    ),
+    "nlo_bk_batch_destroy": (ctypes.c_int, [_vp]),
     "nlo_dis_special": (ctypes.c_int, [_i32, _i64, _vp, _i32, _vp, _vp]),
     "nlo_dis_dipole_amplitude": (ctypes.c_int, [ctypes.POINTER(DisDipole), _i64, _vp, _vp, _vp, _vp]),
     "nlo_dis_bk_initialize": (ctypes.c_int, [ctypes.POINTER(DisDipole), _f64, _vp, _vp, _vp]),

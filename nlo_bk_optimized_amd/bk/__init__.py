@@ -6,7 +6,7 @@ from nlo_bk_optimized_amd.bk.config import (
     RunningCouplingLO,
     RunningCouplingNLO,
 )
-from nlo_bk_optimized_amd.bk.evolution import DNDYResult, compute_dndy, solve_bk
+from nlo_bk_optimized_amd.bk.evolution import DNDYResult, compute_dndy, solve_bk, solve_bk_batch
 from nlo_bk_optimized_amd.bk.initial_conditions import MV, ICDataFile
 
 __all__ = [
@@ -21,4 +21,5 @@ __all__ = [
     "RunningCouplingNLO",
     "compute_dndy",
     "solve_bk",
+    "solve_bk_batch",
 ]

@@ -214,6 +214,89 @@ def solve_bk(
     )
 
 
+_SHARED_FIELDS = ("NC", "NF", "LambdaQCD", "MCINTPOINTS", "DE_SOLVER_STEP", "FIXED_AS", "RC_LO", "RC_NLO",
+                  "RESUM_RC", "INTMETHOD_NLO", "FORCE_POSITIVE_N", "SYMMETRIZE_Z_Z2_INTEGRATION", "Order", "KSUB",
+                  "KINEMATICAL_CONSTRAINT", "EULER_METHOD", "C2", "MC_BINS", "PRECISION", "RK_METHOD",
+                  "K1_RADIAL_ORDER", "K1_ANGULAR_ORDER")
+
+
+def solve_bk_batch(
+    initial_conditions,
+    maxy: float,
+    configs: BKConfig | list[BKConfig] = BKConfig(),
+    *,
+    device: torch.device | str = "cuda",
+    dtype: torch.dtype = torch.float32,
+    seed: int | None = None,
+    spt: int = 0,
+    grid_blocks: int = 0,
+    stats: dict | None = None,
+) -> list[DipoleTable]:
+    """Evolve many initial conditions together: ``solve_bk`` per run
+    (S/bk/evolution.py:33-58), advanced in one set of launches per RK stage.
+
+    The runs may differ in their r grid (RPOINTS, MINR, MAXR) and initial
+    condition; every other BKConfig field must agree.  Stepping is fixed-step
+    ("auto" means Kutta's third-order step); the kinematical constraint is not
+    batched.  ``spt`` / ``grid_blocks`` fix the K2 + Kf launch shape (0: tuned on
+    the first step); ``stats`` (a dict) receives the tuning and the evaluation count.
+    """
+
+    ics = list(initial_conditions)
+    cfgs = list(configs) if isinstance(configs, (list, tuple)) else [configs] * len(ics)
+    if not ics or len(cfgs) != len(ics):
+        raise ValueError("one config (or one shared config) per initial condition is required")
+    c0 = cfgs[0]
+    for c in cfgs[1:]:
+        diff = [f for f in _SHARED_FIELDS if getattr(c, f) != getattr(c0, f)]
+        if diff:
+            raise ValueError(f"batched runs must share {diff}")
+    if c0.KINEMATICAL_CONSTRAINT:
+        raise NotImplementedError("the kinematical constraint is not batched: use solve_bk per run")
+    rk = "euler" if c0.EULER_METHOD else (c0.RK_METHOD if c0.RK_METHOD != "auto" else "kutta3")
+    if rk == "adaptive_rk2":
+        raise NotImplementedError("the batched driver takes fixed-step methods only")
+    d = _resolve_device(device)
+    setups = [prepare(ic, c, device=d, seed=seed) for ic, c in zip(ics, cfgs)]
+    sizes = [s.r.numel() for s in setups]
+    r = torch.cat([s.r for s in setups])
+    edges = torch.cat([s.edges for s in setups]).contiguous()
+    N0 = torch.cat([s.N0 for s in setups])
+    tolerance = 1e-12 * max(1.0, abs(maxy), abs(c0.DE_SOLVER_STEP))
+    steps, y, ys = 0, 0.0, [0.0]
+    while True:
+        y = y + c0.DE_SOLVER_STEP
+        steps += 1
+        ys.append(y)
+        if y >= maxy - tolerance:
+            break
+    ev = dev.BatchEvolver(r, edges, sizes, setups[0].table, c0, n_samples=setups[0].n_samples,
+                          precision=c0.PRECISION, rk_method=rk, step=c0.DE_SOLVER_STEP,
+                          nlo=c0.Order.has_nlo_kernels, spt=spt, grid_blocks=grid_blocks)
+    try:
+        history = torch.empty((steps + 1, r.numel()), dtype=torch.float64, device=d)
+        history[0].copy_(N0)
+        ev.state.copy_(N0)
+        for t in range(steps):
+            ev.step()
+            history[t + 1].copy_(ev.state)
+        if int(ev.flag.item()) != 0:
+            raise FloatingPointError("non-finite BK amplitude during the rapidity evolution")
+        if stats is not None:
+            stats["spt"], stats["grid_blocks"], stats["ms_per_nlo_launch"] = ev.tuning()
+            stats["active_evaluations"] = int(ev.active_evaluations.item())
+            stats["n_samples"] = setups[0].n_samples
+    finally:
+        ev.close()
+    out, off = [], 0
+    yt = torch.tensor(ys, dtype=dtype, device=d)
+    for s, ic, R in zip(setups, ics, sizes):
+        out.append(DipoleTable(r=s.r.to(dtype), y=yt.clone(), N=history[:, off:off + R].to(dtype),
+                               x0=ic.x0))
+        off += R
+    return out
+
+
 def compute_dndy(
     initial_condition: MV | ICDataFile,
     config: BKConfig = BKConfig(),
@@ -240,4 +323,4 @@ def compute_dndy(
     )
 
 
-__all__ = ["BKSetup", "DNDYResult", "adapt_grids", "compute_dndy", "prepare", "rhs", "solve_bk"]
+__all__ = ["BKSetup", "DNDYResult", "adapt_grids", "compute_dndy", "prepare", "rhs", "solve_bk", "solve_bk_batch"]

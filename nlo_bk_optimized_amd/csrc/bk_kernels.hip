@@ -87,10 +87,9 @@ __device__ void stage_spline(const T* table, int R, int row, T* x, T* rec, LdsSp
 // divisions of the sweep) while the right-hand sides of both rows are formed
 // in parallel; lanes 0 and 1 then run the division-free forward and back
 // substitutions of the N and S rows.
-__global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* __restrict__ r,
-                                                          const double* __restrict__ N,
-                                                          int force_positive, double* __restrict__ t64,
-                                                          float* __restrict__ t32) {
+__device__ __forceinline__ void spline_build_body(int R, const double* __restrict__ r,
+                                                  const double* __restrict__ N, int force_positive,
+                                                  double* __restrict__ t64, float* __restrict__ t32) {
     extern __shared__ double sh[];
     double* x = sh;
     double* ly = x + R;        // 2 R
@@ -174,6 +173,13 @@ __global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* _
     }
 }
 
+__global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* __restrict__ r,
+                                                          const double* __restrict__ N,
+                                                          int force_positive, double* __restrict__ t64,
+                                                          float* __restrict__ t32) {
+    spline_build_body(R, r, N, force_positive, t64, t32);
+}
+
 // ------------------------------------------------------------------- K1
 
 struct GaussRule {
@@ -203,16 +209,14 @@ constexpr int kK1Threads = 1024;
 // ANY = true: every coupling / order choice and the kinematical constraint
 // (H.rows set), the full Kernel_lo per theta node.
 template <bool ANY>
-__global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __restrict__ r,
-                                                        const double* __restrict__ N,
-                                                        const double* __restrict__ t64, BkConsts c,
-                                                        GaussRule g, K1History H,
-                                                        double* __restrict__ k1) {
+__device__ __forceinline__ void k1_body(int R, const double* __restrict__ r, const double* __restrict__ N,
+                                        const double* __restrict__ t64, const BkConsts& c,
+                                        const GaussRule& g, const K1History& H, int p,
+                                        double* __restrict__ k1) {
     extern __shared__ double sh[];
     __shared__ double red[kK1Threads / 64];
     __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], s2t[kMaxGaussT], wtt[kMaxGaussT];
     __shared__ double j1tab[kJ1Pieces * (kJ1Deg + 1)];
-    const int p = blockIdx.x;
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
         if (threadIdx.x == 0) k1[p] = 0.0;
         return;
@@ -279,6 +283,15 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
     }
 }
 
+template <bool ANY>
+__global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __restrict__ r,
+                                                        const double* __restrict__ N,
+                                                        const double* __restrict__ t64, BkConsts c,
+                                                        GaussRule g, K1History H,
+                                                        double* __restrict__ k1) {
+    k1_body<ANY>(R, r, N, t64, c, g, H, blockIdx.x, k1);
+}
+
 // ------------------------------------------------------------- K2 + Kf
 
 // LDS layout of the NLO kernel (bytes), shared by host sizing and device.
@@ -328,6 +341,18 @@ struct NloArgs {
     uint32_t first_q[4];
 };
 
+// One evolution's arrays as an NLO block reads them (a single run: NloArgs'
+// own; a batch: the run's slices of the concatenated arrays).
+struct RunView {
+    int R;
+    const double* r;
+    const double* N;
+    const double* t64;
+    const float* t32;
+    const double* edges;
+    double* part;
+};
+
 // Per (parent, block): spt samples per lane.  Sample ordinal
 // s = (blk * spt * 4 + j * 4 + wave) * 64 + lane, Sobol index first + s, so
 // lanes of a wave hold 64 consecutive indices and the gray-code state splits
@@ -337,17 +362,15 @@ struct NloArgs {
 // ANYC: any coupling choice (NloCoupling modes); false = the production
 // coupling only (smooth, smallest distance), with no other code paths.
 template <int MODE, bool HIST, bool FULL, bool ANYC>
-__global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
+__device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int p, int blk) {
     using T = typename std::conditional<MODE == NLO_PREC_FP32, float, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red[4][4];
     __shared__ double wmax[4];
-    const int p = blockIdx.y + 1;
-    const int blk = blockIdx.x;
     const size_t pidx = (size_t)(p - 1) * a.bpp + blk;
-    const NloLayout lay = nlo_layout(a.R, a.bins, a.spt, HIST, sizeof(T));
-    if (!(a.N[p] <= 0.99999)) {
-        if (threadIdx.x < 4) a.part[pidx * 4 + threadIdx.x] = 0.0;
+    const NloLayout lay = nlo_layout(v.R, a.bins, a.spt, HIST, sizeof(T));
+    if (!(v.N[p] <= 0.99999)) {
+        if (threadIdx.x < 4) v.part[pidx * 4 + threadIdx.x] = 0.0;
         if (HIST)
             for (int k = threadIdx.x; k < 4 * a.bins; k += blockDim.x) {
                 a.part_hist[pidx * 4 * a.bins + k] = 0.0;
@@ -366,37 +389,37 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
         // between a constant record below the grid (N = 2^-126) and one above
         // it (N = 1), so that out-of-range lookups need no selects
         float4* rec4 = reinterpret_cast<float4*>(srec);
-        const float4* src = reinterpret_cast<const float4*>(a.t32 + 2 + a.R);
-        const bool aligned = ((reinterpret_cast<uintptr_t>(a.t32) + (2 + a.R) * 4) & 15) == 0;
-        for (int i = threadIdx.x; i < a.R - 1; i += blockDim.x) {
+        const float4* src = reinterpret_cast<const float4*>(v.t32 + 2 + v.R);
+        const bool aligned = ((reinterpret_cast<uintptr_t>(v.t32) + (2 + v.R) * 4) & 15) == 0;
+        for (int i = threadIdx.x; i < v.R - 1; i += blockDim.x) {
             if (aligned) {
                 rec4[i + 1] = src[i];
             } else {
-                const float* f = a.t32 + 2 + a.R + 4 * i;
+                const float* f = v.t32 + 2 + v.R + 4 * i;
                 rec4[i + 1] = make_float4(f[0], f[1], f[2], f[3]);
             }
         }
         if (threadIdx.x == 0) {
             rec4[0] = make_float4(-126.0f, 0.0f, 0.0f, 0.0f);
-            rec4[a.R] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            rec4[v.R] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
-        const double x0 = a.t64[2], xl = a.t64[2 + a.R - 1];
-        const double inv_h = (double)(a.R - 1) / (xl - x0);
+        const double x0 = v.t64[2], xl = v.t64[2 + v.R - 1];
+        const double inv_h = (double)(v.R - 1) / (xl - x0);
         fsp.rec = rec4;
         fsp.inv_h = (float)inv_h;
         fsp.off = (float)(1.0 - x0 * inv_h);
-        fsp.umax = (float)a.R;
+        fsp.umax = (float)v.R;
     } else {
         // FP64 reference formula reads the S spline (row 1); the stable form reads N (row 0).
-        stage_spline<double>(a.t64, a.R, MODE == NLO_PREC_FP64 ? 1 : 0, reinterpret_cast<double*>(sx),
+        stage_spline<double>(v.t64, v.R, MODE == NLO_PREC_FP64 ? 1 : 0, reinterpret_cast<double*>(sx),
                              reinterpret_cast<double*>(srec), spl);
     }
-    const double r0 = a.r[0], rmax = a.r[a.R - 1];
+    const double r0 = v.r[0], rmax = v.r[v.R - 1];
     // nlo_vegas_bounds (S/bk/derivatives.py:424-431)
     const double lo_r = log(0.5 * r0);
     const double w_r = log(2.0 * rmax) - lo_r;
     const int ne = 4 * (a.bins + 1);
-    const double* ep = a.edges + (size_t)p * ne;
+    const double* ep = v.edges + (size_t)p * ne;
     if constexpr (MODE == NLO_PREC_FP32) {
         // Per bin, the absolute left end lo + w left and width w (right - left)
         // of the mapped interval, formed in FP64 and rounded once: the sample
@@ -430,7 +453,7 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     }
     __syncthreads();
 
-    const double rp = a.r[p];
+    const double rp = v.r[p];
     const T lo[4] = {(T)lo_r, (T)lo_r, T(0), T(0)};
     const T wd[4] = {(T)w_r, (T)w_r, (T)(2.0 * kPi), (T)(2.0 * kPi)};
     T vol = T(1);
@@ -451,7 +474,7 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
         cpl32 = NloCoupling<float>{0, (float)a.c.b0, (float)a.c.log_mu0, (float)a.c.log_scale0, 0.0f, 0.0f};
         cpl64 = NloCoupling<double>{0, a.c.b0, a.c.log_mu0, a.c.log_scale0, 0.0, 0.0};
     }
-    const double r_min64 = a.t64[0], r_max64 = a.t64[1];
+    const double r_min64 = v.t64[0], r_max64 = v.t64[1];
 
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
@@ -574,7 +597,7 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
     if (threadIdx.x < 4) {
         double t = 0.0;
         for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
-        a.part[pidx * 4 + threadIdx.x] = t;
+        v.part[pidx * 4 + threadIdx.x] = t;
     }
     if (HIST) {
         // block max of |w| -> fixed-point scale; integer LDS sums are exact
@@ -603,6 +626,12 @@ __global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
             a.part_cnt[pidx * 4 * a.bins + k] = (double)hc[k];
         }
     }
+}
+
+template <int MODE, bool HIST, bool FULL, bool ANYC>
+__global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
+    const RunView v{a.R, a.r, a.N, a.t64, a.t32, a.edges, a.part};
+    nlo_body<MODE, HIST, FULL, ANYC>(a, v, blockIdx.y + 1, blockIdx.x);
 }
 
 // Fixed-order sum of the per-block partials of each parent: each thread adds
@@ -714,6 +743,134 @@ __global__ void rk_final_kernel(int R, double* __restrict__ N, const double* __r
     N[i] = v;
 }
 
+// ------------------------------------------------------------ batches
+
+__global__ void __launch_bounds__(64) spline_build_batch_kernel(const BatchRun* __restrict__ runs,
+                                                                const double* __restrict__ r,
+                                                                const double* __restrict__ N,
+                                                                int force_positive, double* __restrict__ t64,
+                                                                float* __restrict__ t32) {
+    const BatchRun run = runs[blockIdx.x];
+    spline_build_body(run.R, r + run.node_off, N + run.node_off, force_positive, t64 + run.spline_off,
+                      t32 ? t32 + run.spline_off : nullptr);
+}
+
+// One block per node of every run (node 0 of a run writes 0).
+template <bool ANY>
+__global__ void __launch_bounds__(kK1Threads) k1_batch_kernel(const BatchRun* __restrict__ runs,
+                                                              const int32_t* __restrict__ node_run,
+                                                              const double* __restrict__ r,
+                                                              const double* __restrict__ N,
+                                                              const double* __restrict__ t64, BkConsts c,
+                                                              GaussRule g, double* __restrict__ k1) {
+    const BatchRun run = runs[node_run[blockIdx.x]];
+    const K1History H{};
+    k1_body<ANY>(run.R, r + run.node_off, N + run.node_off, t64 + run.spline_off, c, g, H,
+                 blockIdx.x - run.node_off, k1 + run.node_off);
+}
+
+struct BatchNlo {
+    const BatchRun* runs;
+    const int32_t* node_run;
+    const int32_t* parent_node;
+    const double* r;
+    const double* N;
+    const double* t64;
+    const float* t32;
+    const double* edges;
+    double* part;  // parents x bpp x 4
+    unsigned long long* counter;
+    int64_t items;  // parents x bpp
+};
+
+// Persistent K2 + Kf kernel: every block takes items (parent, chunk) from a
+// device counter and runs the same per-block body as nlo_kernel on the
+// run's slices; each item writes its own partial sums, so the result does not
+// depend on which block took it.
+template <int MODE, bool ANYC>
+__global__ void __launch_bounds__(256) nlo_batch_kernel(NloArgs a, BatchNlo w) {
+    __shared__ int64_t item_s;
+    for (;;) {
+        if (threadIdx.x == 0) item_s = (int64_t)atomicAdd(w.counter, 1ull);
+        __syncthreads();
+        const int64_t item = item_s;
+        if (item >= w.items) break;  // block-uniform
+        const int gp = (int)(item / a.bpp);
+        const int blk = (int)(item - (int64_t)gp * a.bpp);
+        const int node = w.parent_node[gp];
+        const BatchRun run = w.runs[w.node_run[node]];
+        const int p = node - run.node_off;
+        const RunView v{run.R,
+                        w.r + run.node_off,
+                        w.N + run.node_off,
+                        w.t64 + run.spline_off,
+                        w.t32 + run.spline_off,
+                        w.edges + (size_t)run.node_off * 4 * (a.bins + 1),
+                        w.part + (size_t)run.parent_off * a.bpp * 4};
+        nlo_body<MODE, false, false, ANYC>(a, v, p, blk);
+        __syncthreads();  // LDS and item_s are reused by the next item
+    }
+}
+
+// Fixed-order sums of each parent's partials (the order of nlo_reduce_kernel);
+// one block per node, node 0 of a run writes 0.
+__global__ void __launch_bounds__(256) nlo_reduce_batch_kernel(const BatchRun* __restrict__ runs,
+                                                               const int32_t* __restrict__ node_run, int bpp,
+                                                               const double* __restrict__ part,
+                                                               double* __restrict__ sums) {
+    __shared__ double red[4][4];
+    const BatchRun run = runs[node_run[blockIdx.x]];
+    const int p = blockIdx.x - run.node_off;
+    if (p == 0) {
+        if (threadIdx.x < 4) sums[(size_t)blockIdx.x * 4 + threadIdx.x] = 0.0;
+        return;
+    }
+    const size_t base = (size_t)(run.parent_off + p - 1) * bpp;
+    double t[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < bpp; b += blockDim.x) {
+        const double4 v = *reinterpret_cast<const double4*>(part + (base + b) * 4);
+        t[0] += v.x;
+        t[1] += v.y;
+        t[2] += v.z;
+        t[3] += v.w;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < 4; ++c) {
+        const double v = wave_sum(t[c]);
+        if (lane == 0) red[wave][c] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        double v = 0.0;
+        for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) v += red[wv][threadIdx.x];
+        sums[(size_t)blockIdx.x * 4 + threadIdx.x] = v;
+    }
+}
+
+__global__ void rhs_finalize_batch_kernel(int nodes, const BatchRun* __restrict__ runs,
+                                          const int32_t* __restrict__ node_run, const double* __restrict__ N,
+                                          const double* __restrict__ k1, const double* __restrict__ sums,
+                                          double inv_n, double* __restrict__ dndy,
+                                          unsigned long long* __restrict__ counter) {
+    __shared__ unsigned int block_active;
+    if (threadIdx.x == 0) block_active = 0u;
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool active = false;
+    if (i < nodes) {
+        const int p = i - runs[node_run[i]].node_off;
+        active = p >= 1 && N[i] <= 0.99999;
+        const double nlo = active ? sums[(size_t)i * 4] * inv_n : 0.0;
+        const double v = active ? k1[i] + nlo : 0.0;
+        dndy[i] = isfinite(v) ? v : 0.0;
+    }
+    if (counter) {
+        if (active) atomicAdd(&block_active, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0 && block_active) atomicAdd(counter, (unsigned long long)block_active);
+    }
+}
+
 __global__ void fill_i32_kernel(int32_t* p, int32_t v, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -745,6 +902,15 @@ int init_kernel_attributes() {
             (const void*)spline_build_kernel,
             (const void*)k1_kernel<false>,
             (const void*)k1_kernel<true>,
+            (const void*)spline_build_batch_kernel,
+            (const void*)k1_batch_kernel<false>,
+            (const void*)k1_batch_kernel<true>,
+            (const void*)nlo_batch_kernel<NLO_PREC_FP32, false>,
+            (const void*)nlo_batch_kernel<NLO_PREC_FP32, true>,
+            (const void*)nlo_batch_kernel<NLO_PREC_FP64, false>,
+            (const void*)nlo_batch_kernel<NLO_PREC_FP64, true>,
+            (const void*)nlo_batch_kernel<NLO_PREC_FP64_STABLE, false>,
+            (const void*)nlo_batch_kernel<NLO_PREC_FP64_STABLE, true>,
 #define NLO_INSTANCES(ANYC)                                                   \
     (const void*)nlo_kernel<NLO_PREC_FP64, false, true, ANYC>,                \
         (const void*)nlo_kernel<NLO_PREC_FP64, false, false, ANYC>,           \
@@ -907,6 +1073,138 @@ int launch_rk_final(int R, double* N, const double* K0, const double* K1, const 
                     int force_positive, int32_t* flag, hipStream_t stream) {
     hipLaunchKernelGGL(rk_final_kernel, dim3((R + 255) / 256), dim3(256), 0, stream, R, N, K0, K1, K2,
                        K3, b0, b1, b2, b3, h, force_positive, flag);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_spline_build_batch(const BatchView& b, const double* r, const double* N, int force_positive,
+                              double* t64, float* t32, hipStream_t stream) {
+    const size_t smem = (size_t)7 * b.Rmax * sizeof(double);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    hipLaunchKernelGGL(spline_build_batch_kernel, dim3(b.runs), dim3(64), smem, stream, b.runs_d, r, N,
+                       force_positive, t64, t32);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_k1_batch(const BatchView& b, const double* r, const double* N, const double* t64,
+                    const BkConsts& c, int nr, int nt, double* k1, hipStream_t stream) {
+    GaussRule g{};
+    g.nr = nr;
+    g.nt = nt;
+    if (nlo_gauss_legendre(nr, g.xr, g.wr) != NLO_OK) return NLO_ERR_INVALID;
+    if (nlo_gauss_legendre(nt, g.xt, g.wt) != NLO_OK) return NLO_ERR_INVALID;
+    const size_t smem = ((size_t)b.Rmax + 4 * (size_t)(b.Rmax - 1)) * sizeof(double);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    if (c.production)
+        hipLaunchKernelGGL(k1_batch_kernel<false>, dim3(b.nodes), dim3(kK1Threads), smem, stream, b.runs_d,
+                           b.node_run, r, N, t64, c, g, k1);
+    else
+        hipLaunchKernelGGL(k1_batch_kernel<true>, dim3(b.nodes), dim3(kK1Threads), smem, stream, b.runs_d,
+                           b.node_run, r, N, t64, c, g, k1);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+size_t nlo_batch_partials(const BatchView& b, int64_t n, int spt) {
+    const int64_t bpp = (n + 256 * spt - 1) / (256 * spt);
+    return (size_t)b.parents * (size_t)bpp * 4 * sizeof(double);
+}
+
+namespace {
+
+template <int MODE, bool ANYC>
+const void* nlo_batch_fn() {
+    return reinterpret_cast<const void*>(nlo_batch_kernel<MODE, ANYC>);
+}
+
+const void* nlo_batch_pick(int precision, bool production_coupling) {
+    if (precision == NLO_PREC_FP32)
+        return production_coupling ? nlo_batch_fn<NLO_PREC_FP32, false>() : nlo_batch_fn<NLO_PREC_FP32, true>();
+    if (precision == NLO_PREC_FP64)
+        return production_coupling ? nlo_batch_fn<NLO_PREC_FP64, false>() : nlo_batch_fn<NLO_PREC_FP64, true>();
+    return production_coupling ? nlo_batch_fn<NLO_PREC_FP64_STABLE, false>()
+                               : nlo_batch_fn<NLO_PREC_FP64_STABLE, true>();
+}
+
+size_t nlo_batch_lds(int precision, int Rmax, int bins, int spt) {
+    return nlo_layout(Rmax, bins, spt, false, precision == NLO_PREC_FP32 ? sizeof(float) : sizeof(double))
+        .total;
+}
+
+bool production_nlo_coupling(const BkConsts& c) {
+    return !c.fixed && !c.thresholds && c.rc_nlo == NLO_RC_NLO_SMALLEST;
+}
+
+}  // namespace
+
+int nlo_batch_resident_blocks(int precision, const BkConsts& c, int Rmax, int bins, int spt) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    const void* fn = nlo_batch_pick(precision, production_nlo_coupling(c));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, 256, nlo_batch_lds(precision, Rmax, bins, spt)) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return cus * (per_cu > 0 ? per_cu : 1);
+}
+
+int launch_nlo_batch(const BatchView& b, const double* r, const double* N, const double* t64,
+                     const float* t32, const double* edges, int bins, const nlo_sobol_table& sobol,
+                     int64_t first, int64_t n, int precision, const BkConsts& c, int spt, int grid,
+                     unsigned long long* counter, double* part, double* sums, hipStream_t stream) {
+    NloArgs a{};
+    a.R = b.Rmax;
+    a.bins = bins;
+    a.spt = spt;
+    a.bpp = (int)((n + 256 * spt - 1) / (256 * spt));
+    a.n = n;
+    a.first = first;
+    a.c = c;
+    for (int d = 0; d < 4; ++d) {
+        for (int k = 0; k < 30; ++k) a.dir[d][k] = sobol.direction[d][k];
+        a.shift[d] = sobol.shift[d];
+        a.first_q[d] = sobol.first[d];
+    }
+    BatchNlo w{b.runs_d, b.node_run, b.parent_node, r, N, t64, t32, edges, part, counter,
+               (int64_t)b.parents * a.bpp};
+    const size_t smem = nlo_batch_lds(precision, b.Rmax, bins, spt);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    rc = launch_fill_u64(counter, 0ull, 1, stream);  // the item counter
+    if (rc != NLO_OK) return rc;
+    const bool prod = production_nlo_coupling(c);
+#define NLO_BATCH(MODE)                                                                                    \
+    if (prod)                                                                                              \
+        hipLaunchKernelGGL((nlo_batch_kernel<MODE, false>), dim3(grid), dim3(256), smem, stream, a, w);   \
+    else                                                                                                   \
+        hipLaunchKernelGGL((nlo_batch_kernel<MODE, true>), dim3(grid), dim3(256), smem, stream, a, w)
+    if (precision == NLO_PREC_FP32) {
+        NLO_BATCH(NLO_PREC_FP32);
+    } else if (precision == NLO_PREC_FP64) {
+        NLO_BATCH(NLO_PREC_FP64);
+    } else {
+        NLO_BATCH(NLO_PREC_FP64_STABLE);
+    }
+#undef NLO_BATCH
+    NLO_LAUNCH_CHECK();
+    hipLaunchKernelGGL(nlo_reduce_batch_kernel, dim3(b.nodes), dim3(256), 0, stream, b.runs_d, b.node_run,
+                       a.bpp, part, sums);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_rhs_finalize_batch(const BatchView& b, const double* N, const double* k1, const double* sums,
+                              int64_t n, double* dndy, unsigned long long* counter, hipStream_t stream) {
+    hipLaunchKernelGGL(rhs_finalize_batch_kernel, dim3((b.nodes + 255) / 256), dim3(256), 0, stream, b.nodes,
+                       b.runs_d, b.node_run, N, k1, sums, 1.0 / (double)n, dndy, counter);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }

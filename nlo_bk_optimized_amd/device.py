@@ -372,6 +372,89 @@ class Evolver:
             pass
 
 
+class BatchEvolver:
+    """Many evolutions of uneven grid sizes stepped together (C ABI ``nlo_bk_batch_*``).
+
+    ``r`` and ``edges`` concatenate the runs' grids ([sum R], [sum R, 4, B + 1]);
+    ``sizes`` lists R per run.  ``spt`` / ``grid_blocks`` fix the K2 + Kf launch
+    shape; left at 0 the first step times the candidates and keeps the fastest.
+    """
+
+    def __init__(self, r: torch.Tensor, edges: torch.Tensor, sizes, table: _lib.SobolTable, config, *,
+                 n_samples: int, precision: str, rk_method: str, step: float, nlo: bool = True,
+                 first_index: int = 0, spt: int = 0, grid_blocks: int = 0) -> None:
+        require_cuda(r, torch.float64, "r")
+        require_cuda(edges, torch.float64, "edges")
+        sizes = [int(v) for v in sizes]
+        if sum(sizes) != r.numel() or edges.shape[0] != r.numel() or edges.dim() != 3 or edges.shape[1] != 4:
+            raise ValueError("r [sum R] and edges [sum R, 4, bins + 1] must concatenate the runs")
+        if rk_method not in ("euler", "rk2", "kutta3", "rk4"):
+            raise ValueError("the batched driver takes fixed-step methods only")
+        self.r, self.edges, self.sizes = r, edges, sizes
+        d = _lib.BatchDesc()
+        d.runs = len(sizes)
+        d.bins = edges.shape[-1] - 1
+        d.n_samples = n_samples
+        d.first_index = first_index
+        d.precision = PRECISION_CODES[precision]
+        d.rk_method = RK_CODES[rk_method]
+        d.step = step
+        d.force_positive = int(bool(config.FORCE_POSITIVE_N))
+        d.nr = config.K1_RADIAL_ORDER
+        d.nt = config.K1_ANGULAR_ORDER
+        d.nlo = int(nlo)
+        d.spt = int(spt)
+        d.grid_blocks = int(grid_blocks)
+        d.physics = physics(config)
+        d.sobol = table
+        self.desc = d
+        self._sizes = (ctypes.c_int32 * len(sizes))(*sizes)
+        nbytes = lib().nlo_bk_batch_workspace(ctypes.byref(d), self._sizes)
+        if nbytes == 0:
+            raise ValueError("invalid batch description")
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device=r.device)
+        handle = ctypes.c_void_p()
+        check(lib().nlo_bk_batch_create(ctypes.byref(d), self._sizes, ptr(r), ptr(edges), ptr(self.workspace),
+                                        nbytes, stream_handle(r.device), ctypes.byref(handle)),
+              "nlo_bk_batch_create")
+        self.handle = handle
+        base = self.workspace.data_ptr()
+
+        def view(address: int, count: int, dtype: torch.dtype) -> torch.Tensor:
+            offset = address - base
+            size = torch.empty((), dtype=dtype).element_size()
+            if offset < 0 or offset % size or offset + count * size > nbytes:
+                raise RuntimeError("batch buffer outside its workspace")
+            return self.workspace[offset : offset + count * size].view(dtype)
+
+        self.state = view(lib().nlo_bk_batch_state(handle), r.numel(), torch.float64)
+        self.flag = view(lib().nlo_bk_batch_nonfinite_flag(handle), 1, torch.int32)
+        self.active_evaluations = view(lib().nlo_bk_batch_active_evaluations(handle), 1, torch.int64)
+
+    def step(self) -> None:
+        check(lib().nlo_bk_batch_step(self.handle, stream_handle(self.r.device)), "nlo_bk_batch_step")
+
+    def tuning(self) -> tuple[int, int, float]:
+        """(samples per lane, persistent grid blocks, ms per K2 + Kf launch) in use."""
+
+        spt, grid, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        check(lib().nlo_bk_batch_tuning(self.handle, ctypes.byref(spt), ctypes.byref(grid), ctypes.byref(ms)),
+              "nlo_bk_batch_tuning")
+        return spt.value, grid.value, ms.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            torch.cuda.current_stream(self.r.device).synchronize()
+            lib().nlo_bk_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self) -> None:
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 # ------------------------------------------------------------------- DIS
 
 
@@ -527,6 +610,7 @@ __all__ = [
     "dis_vegas",
     "PRECISION_CODES",
     "RK_CODES",
+    "BatchEvolver",
     "bk_k1",
     "bk_nlo",
     "dtype_epsilon",

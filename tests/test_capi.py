@@ -94,6 +94,20 @@ def test_invalid_arguments_return_status_and_message():
     ph = _lib.Physics(nc=3.0, nf=3.0, lambda_qcd=0.241, c2=1.0, ksub=0.65, minr=1e-6, symmetrize=1)
     assert L.nlo_bk_k1_history(8, 1, 1, 1, ctypes.byref(ph), 8, 24, 1, 1, 1, 0, 1, None) == 1
     assert b"capacity" in L.nlo_last_error()
+    # batched driver: every run's grid is checked before anything is allocated
+    d = _lib.BatchDesc(runs=2, bins=50, n_samples=4096, precision=1, rk_method=1, step=0.1, nr=8, nt=24,
+                       nlo=1)
+    d.physics = ph
+    d.sobol.dim = 4
+    sizes = (ctypes.c_int32 * 2)(64, 2)
+    assert L.nlo_bk_batch_workspace(ctypes.byref(d), sizes) == 0
+    handle = ctypes.c_void_p()
+    assert L.nlo_bk_batch_create(ctypes.byref(d), sizes, 1, 1, 1, 1 << 30, None, ctypes.byref(handle)) == 1
+    assert b"run 1" in L.nlo_last_error()
+    d.spt = 5
+    sizes[1] = 32
+    assert L.nlo_bk_batch_create(ctypes.byref(d), sizes, 1, 1, 1, 1 << 30, None, ctypes.byref(handle)) == 1
+    assert b"samples per lane" in L.nlo_last_error()
     with pytest.raises(RuntimeError, match="status 1"):
         _lib.check(L.nlo_vegas_adapt(0, 4, 50, None, None, None, 1e-16, None, None), "adapt")
 
@@ -115,6 +129,7 @@ STRUCTS = {
     "nlo_bk_physics": _lib.Physics,
     "nlo_sobol_table": _lib.SobolTable,
     "nlo_bk_evolver_desc": _lib.EvolverDesc,
+    "nlo_bk_batch_desc": _lib.BatchDesc,
     "nlo_dis_dipole": _lib.DisDipole,
     "nlo_dis_desc": _lib.DisDesc,
 }
