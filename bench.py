@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -244,9 +245,9 @@ def main() -> int:
         dipole = BKDipole(table, device=device)
         dcfg = DISConfig(order=DISOrder.NLO, quarks=(Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)),
                          maxeval=1 << 22, epsrel=1e-12)
-        grid = [(q, x) for q in np.geomspace(1.0, 100.0, 32) for x in np.geomspace(1e-5, 1e-2, 16)]
-        stride = max(1, len(grid) // args.dis_points)
-        points = grid[::stride][: args.dis_points]
+        qs_all, xs_all = np.geomspace(1.0, 100.0, 32), np.geomspace(1e-5, 1e-2, 16)
+        step = max(1, int(round(math.sqrt(512 / max(1, args.dis_points)))))
+        points = [(q, x) for q in qs_all[::step] for x in xs_all[::step]]
         res = {}
         for dtype, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
             torch.cuda.synchronize()
@@ -262,6 +263,35 @@ def main() -> int:
             log(f"DIS batch {name}: {len(points)} points in {res[name][1]:.2f} s")
         v32, v64 = res["fp32"][0], res["fp64"][0]
         f2_32, f2_64 = v32[:, 0] + v32[:, 1], v64[:, 0] + v64[:, 1]
+        # Integrand precision alone: one VEGAS pass of every VEGAS term (dipole
+        # and qqbar-g, T and L, both quarks) through the same frozen (uniform)
+        # grid at the same 2^22 Sobol points on the FP32 and the FP64 path.  (The adaptive runs above refine
+        # their grids with the dtype's epsilon in the smoothing, as the
+        # reference does, so they differ by about one Monte Carlo error.)
+        from nlo_bk_optimized_amd import _lib
+
+        frozen, worst = [], None
+        for q2, xb in points:
+            for transverse in (True, False):
+                kinds = (("T1", "T2") if transverse else ("ab", "cd")) + ("I1", "I2_nested", "I3_nested")
+                for quark in dcfg.quarks:
+                    desc = dev.dis_desc(q2, xb, transverse, (quark,), dipole, dcfg)
+                    for kind in kinds:
+                        dim = _lib.DIS_DIMENSIONS[kind]
+                        ed = torch.linspace(0, 1, 33, dtype=torch.float64, device=device).expand(dim, -1).contiguous()
+                        tb = dev.sobol_table(dim, 0)
+                        s32, _, _ = dev.dis_vegas(desc, kind, ed, tb, 1 << 22, precision="fp32")
+                        s64, _, _ = dev.dis_vegas(desc, kind, ed, tb, 1 << 22, precision="fp64")
+                        a32, a64 = s32.cpu().numpy(), s64.cpu().numpy()
+                        nn = float(1 << 22)
+                        stderr = math.sqrt(max(a64[1] - a64[0] ** 2 / nn, 0.0) / (nn - 1))
+                        row = (abs(a32[0] - a64[0]) / max(abs(a64[0]), 1e-300),
+                               abs(a32[0] - a64[0]) / max(stderr, 1e-300))
+                        frozen.append(row)
+                        if worst is None or row[1] > worst[0][1]:
+                            worst = (row, f"Q2={q2:.3g} x={xb:.3g} {'T' if transverse else 'L'} "
+                                          f"m={quark.mass} {kind}")
+        frozen = np.array(frozen)
         q2a = np.array([p[0] for p in points])
         xa = np.array([p[1] for p in points])
         yv = q2a / (xa * 318.0**2)
@@ -281,9 +311,17 @@ def main() -> int:
                                                   if ok.any() else None),
             "dis_sigma_r_points": int(ok.sum()),
             "dis_median_rel_stderr_F2": float(np.median(np.hypot(v64[:, 2], v64[:, 3]) / np.abs(f2_64))),
+            "dis_max_dF2_in_stderr": float(np.max(np.abs(f2_32 - f2_64) / np.hypot(v64[:, 2], v64[:, 3]))),
+            "dis_frozen_grid_max_rel_diff_fp32_vs_fp64": float(frozen[:, 0].max()),
+            "dis_frozen_grid_median_rel_diff_fp32_vs_fp64": float(np.median(frozen[:, 0])),
+            "dis_frozen_grid_max_diff_in_stderr": float(frozen[:, 1].max()),
+            "dis_frozen_grid_worst_case": worst[1],
         })
-        log(f"DIS batch: max rel dF2 {extra['dis_max_rel_dF2_fp32_vs_fp64']:.2e}, "
-            f"max rel dsigma_r {extra['dis_max_rel_dsigma_r_fp32_vs_fp64']}")
+        log(f"DIS batch: max rel dF2 {extra['dis_max_rel_dF2_fp32_vs_fp64']:.2e} "
+            f"({extra['dis_max_dF2_in_stderr']:.2f} MC errors), max rel dsigma_r "
+            f"{extra['dis_max_rel_dsigma_r_fp32_vs_fp64']}; frozen grid, same points: max rel "
+            f"{frozen[:, 0].max():.2e}, median {np.median(frozen[:, 0]):.2e}, max {frozen[:, 1].max():.3f} errors "
+            f"({worst[1]})")
 
     if args.batch_runs > 0:
         # Batched uneven evolutions: runs with Qs0^2 ~ U[0.1, 0.5], gamma ~

@@ -391,12 +391,13 @@ int nlo_dis_bk_initialize(const nlo_dis_dipole* dipole, double Y, double* d_coef
  * LO sums every quark of the descriptor; the others use quark 0. */
 #define NLO_DIS_LO 0             /* [2] S/dis/observables.py:345-352, S/dis/lo.py:12-51 */
 #define NLO_DIS_DIP_T0 1         /* [2] integrand_dip_massive "T0" (S/dis/observables.py:107-172) */
-#define NLO_DIS_DIP_T1 2         /* [3] "T1" */
+#define NLO_DIS_DIP_T1 2         /* [3] "T1"; FP64 also on the FP32 path (K1/K0 differences) */
 #define NLO_DIS_DIP_T2 3         /* [4] "T2" */
 #define NLO_DIS_DIP_L_CONST 4    /* [2] "Omega_L_const" */
-#define NLO_DIS_DIP_L_AB 5       /* [3] "ab" */
+#define NLO_DIS_DIP_L_AB 5       /* [3] "ab"; FP64 also on the FP32 path (K0 differences) */
 #define NLO_DIS_DIP_L_CD 6       /* [4] "cd" */
-#define NLO_DIS_QG_I1 7          /* [5] integrand_qgunsub_massive "I1" (:175-250) */
+#define NLO_DIS_QG_I1 7          /* [5] integrand_qgunsub_massive "I1" (:175-250); FP64 also on the
+                                    FP32 path: its UV-subtracted sum cancels to O(z2) as z2 -> 0 */
 #define NLO_DIS_QG_I2 8          /* [6] "I2" */
 #define NLO_DIS_QG_I3 9          /* [7] "I3" */
 #define NLO_DIS_QG_I2_NESTED 10  /* [5] integrand_qg_nested_massive "I2" (:253-331) */

@@ -675,6 +675,26 @@ int make_args(const nlo_dis_desc* desc, DisArgs& a) {
             break;                    \
     }
 
+// The arithmetic a contribution uses on the FP32 path.  FP64 where the
+// integrand is a small difference of O(1) terms:
+//  * qqbar-g I1 (uvsub + tripole I1) cancels to O(z2) in the soft-gluon region
+//    while its Jacobian carries 1 / z2 (measured in FP32: up to 9e-3 relative
+//    per 2^22-sample pass through the same grid, 35 standard errors);
+//  * the longitudinal "ab" term, 2 K0(b1) - K0(b2) - K0(b3) with b2, b3 -> b1
+//    for light quarks, and the transverse "T1" term, V1 / VMS1 differences of
+//    K1 and K0 at nearby arguments (the reference evaluates both in FP64
+//    inside its FP32 kernel, K/dis/lo_dipole/kernel.cu:205-227, 278-331).
+int path_precision(int contribution, int precision) {
+    switch (contribution) {
+        case NLO_DIS_QG_I1:
+        case NLO_DIS_DIP_L_AB:
+        case NLO_DIS_DIP_T1:
+            return NLO_PREC_FP64;
+        default:
+            return precision;
+    }
+}
+
 struct VegasPlan {
     int spt, blocks, groups, per_group;
     size_t part, hist, group, workspace, lds;
@@ -761,6 +781,7 @@ int nlo_dis_integrand(const nlo_dis_desc* desc, int32_t contribution, int64_t n,
     clear_error();
     NLO_REQUIRE(contribution >= 0 && contribution < NLO_DIS_KINDS, "unknown contribution %d", contribution);
     NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    precision = path_precision(contribution, precision);
     NLO_REQUIRE(n >= 0, "n must be >= 0");
     DisArgs a;
     int rc = make_args(desc, a);
@@ -792,6 +813,7 @@ int nlo_dis_vegas(const nlo_dis_desc* desc, int32_t contribution, int32_t bins, 
     clear_error();
     NLO_REQUIRE(contribution >= 0 && contribution < NLO_DIS_KINDS, "unknown contribution %d", contribution);
     NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    precision = path_precision(contribution, precision);
     const int D = kDims[contribution];
     NLO_REQUIRE(bins >= 2 && bins <= 255, "bins must be 2..255");
     NLO_REQUIRE(n >= 1, "n must be >= 1");

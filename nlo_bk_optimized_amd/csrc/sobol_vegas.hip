@@ -20,10 +20,10 @@ namespace {
 // gray(n) = (gray(hi) << 6) ^ (lane ^ (lane >> 1) ^ ((hi & 1) << 5)), so the
 // state splits into a wave-uniform part (direction bits 6..29, scalar ALU)
 // and a per-lane part (bits 0..5) fixed for the whole launch.
+template <int DIM>
 __global__ void __launch_bounds__(256) sobol_draw_kernel(nlo_sobol_table t, int64_t first,
-                                                          int64_t n, int32_t kind, void* out) {
+                                                          int64_t n, int32_t kind, int vec, void* out) {
     const int lane = threadIdx.x & 63;
-    const int dim = t.dim;
     const int64_t wave_global = (int64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
     const int64_t hi_begin = first >> 6;
@@ -32,25 +32,54 @@ __global__ void __launch_bounds__(256) sobol_draw_kernel(nlo_sobol_table t, int6
         const uint64_t idx = ((uint64_t)hi << 6) | (uint64_t)lane;
         const int64_t row = (int64_t)idx - first;
         const bool live = row >= 0 && row < n;
-        const uint64_t ghi = (uint64_t)hi ^ ((uint64_t)hi >> 1);
         const uint32_t glo = (uint32_t)(lane ^ (lane >> 1)) ^ ((uint32_t)(hi & 1) << 5);
-        for (int d = 0; d < dim; ++d) {
-            uint32_t q = t.shift[d];
-            uint64_t g = ghi;
-            for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
-                if (g & 1u) q ^= t.direction[d][b];
+        uint32_t q[DIM];
+        // wave-uniform high part: direction bits 6..29 of gray(hi)
 #pragma unroll
-            for (int b = 0; b < 6; ++b)
-                if ((glo >> b) & 1u) q ^= t.direction[d][b];
-            if (!live) continue;
-            const size_t o = (size_t)row * dim + d;
+        for (int d = 0; d < DIM; ++d) q[d] = t.shift[d];
+        uint64_t g = (uint64_t)hi ^ ((uint64_t)hi >> 1);
+        for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
+            if (g & 1u) {
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) q[d] ^= t.direction[d][b];
+            }
+#pragma unroll
+        for (int b = 0; b < 6; ++b) {
+            const bool on = (glo >> b) & 1u;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) q[d] ^= on ? t.direction[d][b] : 0u;
+        }
+        if (!live) continue;
+        const size_t o = (size_t)row * DIM;
+        if constexpr (DIM == 4) {
+            if (vec) {
+                // 16-byte aligned output: one 16- or 32-byte store per lane, so
+                // the wave writes one contiguous span
+                if (kind == NLO_SOBOL_U32) {
+                    reinterpret_cast<uint4*>(static_cast<uint32_t*>(out) + o)[0] = make_uint4(q[0], q[1], q[2], q[3]);
+                } else if (kind == NLO_SOBOL_F32) {
+                    reinterpret_cast<float4*>(static_cast<float*>(out) + o)[0] =
+                        make_float4((float)q[0] * 0x1p-30f, (float)q[1] * 0x1p-30f, (float)q[2] * 0x1p-30f,
+                                    (float)q[3] * 0x1p-30f);
+                } else {
+                    double v[4];
+                    for (int d = 0; d < 4; ++d) v[d] = (double)(idx == 0 ? t.first[d] : q[d]) * 0x1p-30;
+                    double2* po = reinterpret_cast<double2*>(static_cast<double*>(out) + o);
+                    po[0] = make_double2(v[0], v[1]);
+                    po[1] = make_double2(v[2], v[3]);
+                }
+                continue;
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
             if (kind == NLO_SOBOL_U32) {
-                static_cast<uint32_t*>(out)[o] = q;
+                static_cast<uint32_t*>(out)[o + d] = q[d];
             } else if (kind == NLO_SOBOL_F32) {
-                static_cast<float*>(out)[o] = (float)q * 0x1p-30f;
+                static_cast<float*>(out)[o + d] = (float)q[d] * 0x1p-30f;
             } else {
-                const uint32_t qq = idx == 0 ? t.first[d] : q;
-                static_cast<double*>(out)[o] = (double)qq * 0x1p-30;
+                const uint32_t qq = idx == 0 ? t.first[d] : q[d];
+                static_cast<double*>(out)[o + d] = (double)qq * 0x1p-30;
             }
         }
     }
@@ -68,9 +97,39 @@ __global__ void __launch_bounds__(256) vegas_map_kernel(int dim, int bins, int64
                                                          const double* __restrict__ width,
                                                          double* __restrict__ x,
                                                          double* __restrict__ jac,
-                                                         uint8_t* __restrict__ bin) {
+                                                         uint8_t* __restrict__ bin, int vec) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
+        if (vec) {
+            // dim = 4 with 16-byte aligned u and x and 4-byte aligned bins:
+            // vector loads and stores, one contiguous span per wave; the same
+            // arithmetic in the same order as below
+            const double2* u2 = reinterpret_cast<const double2*>(u + i * 4);
+            const double2 ua = u2[0], ub = u2[1];
+            const double uv[4] = {ua.x, ua.y, ub.x, ub.y};
+            double xv[4];
+            uint32_t packed = 0u;
+            double j = 1.0, vol = 1.0;
+            for (int d = 0; d < 4; ++d) {
+                const double scaled = uv[d] * (double)bins;
+                int b = (int)floor(scaled);
+                b = b < 0 ? 0 : (b > bins - 1 ? bins - 1 : b);
+                const double frac = scaled - (double)b;
+                const double left = edges[d * (bins + 1) + b];
+                const double right = edges[d * (bins + 1) + b + 1];
+                const double w = right - left;
+                xv[d] = lo[d] + width[d] * (left + w * frac);
+                j = j * ((double)bins * w);
+                vol = vol * width[d];
+                packed |= (uint32_t)b << (8 * d);
+            }
+            double2* x2 = reinterpret_cast<double2*>(x + i * 4);
+            x2[0] = make_double2(xv[0], xv[1]);
+            x2[1] = make_double2(xv[2], xv[3]);
+            if (bin) reinterpret_cast<uint32_t*>(bin)[i] = packed;
+            jac[i] = vol * j;
+            continue;
+        }
         double j = 1.0, vol = 1.0;
         for (int d = 0; d < dim; ++d) {
             const double scaled = u[i * dim + d] * (double)bins;
@@ -96,16 +155,21 @@ constexpr int kHistChunk = 4096;  // samples per block: 2^12, sums stay below 2^
 // then adds every |w| as a 64-bit fixed-point integer scaled so that the
 // chunk total cannot exceed 2^63: integer addition is exact and
 // order-independent, so LDS atomics give the same bits every run.
+// Per block of kHistChunk samples: |w| scaled to 64-bit fixed point by the
+// block maximum, then integer sums per (dim, bin) in LDS (exact and
+// order-free).  With `waves` > 1 every wave fills its own copy (fewer atomic
+// collisions on the same bin) and the copies are added in wave order.
 __global__ void __launch_bounds__(kHistThreads) vegas_hist_block_kernel(
-    int dim, int bins, int64_t n, const uint8_t* __restrict__ bin, const double* __restrict__ absw,
+    int dim, int bins, int64_t n, int waves, const uint8_t* __restrict__ bin, const double* __restrict__ absw,
     double* __restrict__ part_hist, double* __restrict__ part_count) {
     extern __shared__ unsigned long long smem[];
-    unsigned long long* h = smem;                                      // dim * bins
-    unsigned int* c = reinterpret_cast<unsigned int*>(smem + dim * bins);  // dim * bins
+    const int cells = dim * bins;
+    unsigned long long* h = smem;                                               // waves x cells
+    unsigned int* c = reinterpret_cast<unsigned int*>(smem + (size_t)waves * cells);  // waves x cells
     __shared__ double wave_max_v[kHistThreads / 64];
     const int64_t begin = (int64_t)blockIdx.x * kHistChunk;
     const int64_t end = begin + kHistChunk < n ? begin + kHistChunk : n;
-    for (int k = threadIdx.x; k < dim * bins; k += blockDim.x) {
+    for (int k = threadIdx.x; k < waves * cells; k += blockDim.x) {
         h[k] = 0ull;
         c[k] = 0u;
     }
@@ -118,31 +182,61 @@ __global__ void __launch_bounds__(kHistThreads) vegas_hist_block_kernel(
     for (int k = 0; k < kHistThreads / 64; ++k) m = fmax(m, wave_max_v[k]);
     const int e = m > 0.0 ? ilogb(m) : 0;
     const int s = 50 - e;  // m * 2^s < 2^51; 4096 terms < 2^63
+    const int copy = waves > 1 ? (int)(threadIdx.x >> 6) : 0;
+    unsigned long long* hw = h + (size_t)copy * cells;
+    unsigned int* cw = c + (size_t)copy * cells;
     for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) {
         const unsigned long long v = (unsigned long long)rint(ldexp(fabs(absw[i]), s));
         for (int d = 0; d < dim; ++d) {
             const int b = min((int)bin[i * dim + d], bins - 1);
-            atomicAdd(&h[d * bins + b], v);
-            atomicAdd(&c[d * bins + b], 1u);
+            atomicAdd(&hw[d * bins + b], v);
+            atomicAdd(&cw[d * bins + b], 1u);
         }
     }
     __syncthreads();
-    for (int k = threadIdx.x; k < dim * bins; k += blockDim.x) {
-        part_hist[(size_t)blockIdx.x * dim * bins + k] = ldexp((double)h[k], -s);
-        part_count[(size_t)blockIdx.x * dim * bins + k] = (double)c[k];
+    for (int k = threadIdx.x; k < cells; k += blockDim.x) {
+        unsigned long long th = 0ull;
+        unsigned int tc = 0u;
+        for (int w = 0; w < waves; ++w) {
+            th += h[(size_t)w * cells + k];
+            tc += c[(size_t)w * cells + k];
+        }
+        part_hist[(size_t)blockIdx.x * cells + k] = ldexp((double)th, -s);
+        part_count[(size_t)blockIdx.x * cells + k] = (double)tc;
     }
 }
 
-// Fixed-order sum of the per-block partials: one thread per (dim, bin).
-__global__ void vegas_hist_reduce_kernel(int values, int blocks, const double* __restrict__ ph,
-                                         const double* __restrict__ pc, double* __restrict__ hist,
+// Fixed-order sums of the per-block partials in two levels: group g of
+// consecutive blocks per (dim, bin) (lanes on consecutive cells, coalesced),
+// then the groups in order.
+constexpr int kHistGroups = 64;
+
+__global__ void __launch_bounds__(64) vegas_hist_group_kernel(int values, int blocks, int per_group,
+                                                             const double* __restrict__ ph,
+                                                             const double* __restrict__ pc,
+                                                             double* __restrict__ gh, double* __restrict__ gc) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int g = blockIdx.y;
+    if (k >= values) return;
+    const int b0 = g * per_group, b1 = min(b0 + per_group, blocks);
+    double sh = 0.0, sc = 0.0;
+    for (int b = b0; b < b1; ++b) {
+        sh += ph[(size_t)b * values + k];
+        sc += pc[(size_t)b * values + k];
+    }
+    gh[(size_t)g * values + k] = sh;
+    gc[(size_t)g * values + k] = sc;
+}
+
+__global__ void vegas_hist_reduce_kernel(int values, int groups, const double* __restrict__ gh,
+                                         const double* __restrict__ gc, double* __restrict__ hist,
                                          double* __restrict__ count) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= values) return;
     double sh = 0.0, sc = 0.0;
-    for (int b = 0; b < blocks; ++b) {
-        sh += ph[(size_t)b * values + k];
-        sc += pc[(size_t)b * values + k];
+    for (int g = 0; g < groups; ++g) {
+        sh += gh[(size_t)g * values + k];
+        sc += gc[(size_t)g * values + k];
     }
     hist[k] = sh;
     count[k] = sc;
@@ -305,8 +399,25 @@ int nlo_sobol_draw(const nlo_sobol_table* table, int64_t first_index, int64_t n,
     const int64_t waves_per_block = 4;
     int64_t blocks = (groups + waves_per_block - 1) / waves_per_block;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(nlo::sobol_draw_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, *table,
-                       first_index, n, out_kind, d_out);
+    const int vec = table->dim == 4 && (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
+#define NLO_SOBOL_DIM(D)                                                                                \
+    case D:                                                                                             \
+        hipLaunchKernelGGL(nlo::sobol_draw_kernel<D>, dim3((unsigned)blocks), dim3(256), 0, stream, *table, \
+                           first_index, n, out_kind, vec, d_out);                                        \
+        break;
+    switch (table->dim) {
+        NLO_SOBOL_DIM(1)
+        NLO_SOBOL_DIM(2)
+        NLO_SOBOL_DIM(3)
+        NLO_SOBOL_DIM(4)
+        NLO_SOBOL_DIM(5)
+        NLO_SOBOL_DIM(6)
+        NLO_SOBOL_DIM(7)
+        NLO_SOBOL_DIM(8)
+        default:
+            break;
+    }
+#undef NLO_SOBOL_DIM
     NLO_LAUNCH_CHECK();
     nlo::clear_error();
     return NLO_OK;
@@ -323,17 +434,39 @@ int nlo_vegas_map(int32_t dim, int32_t bins, int64_t n, const double* d_u, const
     if (n == 0) return NLO_OK;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 65536) blocks = 65536;
+    const int vec = dim == 4 && (reinterpret_cast<uintptr_t>(d_u) & 15) == 0 &&
+                    (reinterpret_cast<uintptr_t>(d_x) & 15) == 0 && (reinterpret_cast<uintptr_t>(d_bin) & 3) == 0;
     hipLaunchKernelGGL(nlo::vegas_map_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, dim, bins,
-                       n, d_u, d_edges, d_lo, d_width, d_x, d_jac, d_bin);
+                       n, d_u, d_edges, d_lo, d_width, d_x, d_jac, d_bin, vec);
     NLO_LAUNCH_CHECK();
     nlo::clear_error();
     return NLO_OK;
 }
 
+namespace {
+
+struct HistPlan {
+    int blocks, per_group, groups;
+    size_t part, group, total;
+};
+
+HistPlan hist_plan(int dim, int bins, int64_t n) {
+    HistPlan p{};
+    p.blocks = (int)((n + nlo::kHistChunk - 1) / nlo::kHistChunk);
+    if (p.blocks < 1) p.blocks = 1;
+    p.per_group = (p.blocks + nlo::kHistGroups - 1) / nlo::kHistGroups;
+    p.groups = (p.blocks + p.per_group - 1) / p.per_group;
+    p.part = (size_t)p.blocks * dim * bins * 2 * sizeof(double);
+    p.group = (size_t)p.groups * dim * bins * 2 * sizeof(double);
+    p.total = p.part + p.group;
+    return p;
+}
+
+}  // namespace
+
 size_t nlo_vegas_histogram_workspace(int32_t dim, int32_t bins, int64_t n) {
     if (dim < 1 || bins < 1 || n < 0) return 0;
-    const int64_t blocks = (n + nlo::kHistChunk - 1) / nlo::kHistChunk;
-    return (size_t)(blocks > 0 ? blocks : 1) * dim * bins * 2 * sizeof(double);
+    return hist_plan(dim, bins, n).total;
 }
 
 int nlo_vegas_histogram(int32_t dim, int32_t bins, int64_t n, const uint8_t* d_bin,
@@ -348,16 +481,25 @@ int nlo_vegas_histogram(int32_t dim, int32_t bins, int64_t n, const uint8_t* d_b
         nlo::set_error("nlo_vegas_histogram: workspace %zu bytes < %zu", workspace_bytes, need);
         return NLO_ERR_WORKSPACE;
     }
-    const int blocks = (int)((n + nlo::kHistChunk - 1) / nlo::kHistChunk);
-    double* ph = static_cast<double*>(d_workspace);
-    double* pc = ph + (size_t)blocks * dim * bins;
-    const size_t smem = (size_t)dim * bins * (sizeof(unsigned long long) + sizeof(unsigned int)) + 8;
-    hipLaunchKernelGGL(nlo::vegas_hist_block_kernel, dim3(blocks), dim3(nlo::kHistThreads), smem, stream,
-                       dim, bins, n, d_bin, d_absw, ph, pc);
-    NLO_LAUNCH_CHECK();
+    const HistPlan pl = hist_plan(dim, bins, n);
+    const int blocks = pl.blocks;
     const int values = dim * bins;
+    double* ph = static_cast<double*>(d_workspace);
+    double* pc = ph + (size_t)blocks * values;
+    double* gh = pc + (size_t)blocks * values;
+    double* gc = gh + (size_t)pl.groups * values;
+    // one sub-histogram per wave while they fit in 48 KiB of LDS
+    const size_t cell_bytes = sizeof(unsigned long long) + sizeof(unsigned int);
+    const int waves = (size_t)(nlo::kHistThreads / 64) * values * cell_bytes <= 48 * 1024 ? nlo::kHistThreads / 64 : 1;
+    const size_t smem = (size_t)waves * values * cell_bytes + 8;
+    hipLaunchKernelGGL(nlo::vegas_hist_block_kernel, dim3(blocks), dim3(nlo::kHistThreads), smem, stream,
+                       dim, bins, n, waves, d_bin, d_absw, ph, pc);
+    NLO_LAUNCH_CHECK();
+    hipLaunchKernelGGL(nlo::vegas_hist_group_kernel, dim3((values + 63) / 64, pl.groups), dim3(64), 0, stream,
+                       values, blocks, pl.per_group, ph, pc, gh, gc);
+    NLO_LAUNCH_CHECK();
     hipLaunchKernelGGL(nlo::vegas_hist_reduce_kernel, dim3((values + 255) / 256), dim3(256), 0, stream,
-                       values, blocks, ph, pc, d_hist, d_count);
+                       values, pl.groups, gh, gc, d_hist, d_count);
     NLO_LAUNCH_CHECK();
     nlo::clear_error();
     return NLO_OK;

@@ -53,14 +53,15 @@ def test_sobol_bit_exact_with_sobolengine(dev, dim, seed, first, n):
 # ------------------------------------------------------------------ VEGAS
 
 
-def test_vegas_map_bins_bit_exact(dev):
-    dim, bins, n = 5, 50, 20000
+@pytest.mark.parametrize("dim", [4, 5])  # 4: the vector load / store path
+def test_vegas_map_bins_bit_exact(dev, dim):
+    bins, n = 50, 20000
     table = dev.sobol_table(dim, 11)
     u = dev.sobol_draw(table, n, kind="f64")
     edges = random_edges(1, bins, seed=3)[0, :4]
-    edges = np.concatenate([edges, edges[:1]], axis=0)
-    lo = np.array([-3.0, 0.5, 0.0, 1.0, -1.0])
-    width = np.array([6.0, 2.0, 6.283185307179586, 0.5, 4.0])
+    edges = np.concatenate([edges, edges[:1]], axis=0)[:dim]
+    lo = np.array([-3.0, 0.5, 0.0, 1.0, -1.0])[:dim]
+    width = np.array([6.0, 2.0, 6.283185307179586, 0.5, 4.0])[:dim]
     x, jac, b = dev.vegas_map(u, cuda(edges), cuda(lo), cuda(width))
     q = np.round(u.cpu().numpy() * 2.0**30).astype(np.uint32)
     ox, ojac, ob = O.vegas_map(q, edges, lo, width)
