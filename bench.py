@@ -69,6 +69,8 @@ def parse() -> argparse.Namespace:
                    help="Sobol points per parent in the CPU baseline sample")
     p.add_argument("--no-dis", action="store_true", help="skip the DIS VEGAS-pass measurements")
     p.add_argument("--no-hbm", action="store_true", help="skip the Sobol / VEGAS map / histogram bandwidth runs")
+    p.add_argument("--large-maxy", type=float, default=0.0,
+                   help="run the 1024-node, 2^24-point RK4 evolution to this Y in FP32 and FP64 (0 skips it)")
     p.add_argument("--batch-runs", type=int, default=64,
                    help="evolutions in the batched uneven-grid run (0 skips it)")
     p.add_argument("--dis-points", type=int, default=32,
@@ -322,6 +324,46 @@ def main() -> int:
             f"{extra['dis_max_rel_dsigma_r_fp32_vs_fp64']}; frozen grid, same points: max rel "
             f"{frozen[:, 0].max():.2e}, median {np.median(frozen[:, 0]):.2e}, max {frozen[:, 1].max():.3f} errors "
             f"({worst[1]})")
+
+    if args.large_maxy > 0:
+        # The large run: 1024 r in [1e-7, 1e2], 2^24 Sobol points per parent per
+        # RHS, classical RK4 with dY = 0.05, FP32 integrand with FP64 sums
+        # against the FP64 evaluation of the same formula (the GPU oracle of
+        # the FP32 path), max |dN| at every unit of Y up to --large-maxy.
+        nl = 1 << 24
+        large = dict(RPOINTS=1023, MINR=1e-7, MAXR=1e2, MCINTPOINTS=nl, MC_BINS=50, VEGAS_ADAPT_ITERATIONS=5,
+                     DE_SOLVER_STEP=0.05, RK_METHOD="rk4")
+        steps_l = int(round(args.large_maxy / 0.05))
+        per_unit = int(round(1 / 0.05))
+        states, times = {}, {}
+        for prec in ("fp32", "fp64_stable"):
+            cfg = BKConfig(PRECISION=prec, **large)
+            setup_l = evo.prepare(ic, cfg, device=device, seed=0)
+            ev = dev.Evolver(setup_l.r, setup_l.edges, setup_l.table, cfg, n_samples=nl, precision=prec,
+                             rk_method="rk4", step=0.05)
+            ev.state.copy_(setup_l.N0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            rows = []
+            for k in range(1, steps_l + 1):
+                ev.step()
+                if k % per_unit == 0:
+                    rows.append(ev.state.clone())
+                    torch.cuda.synchronize()
+                    log(f"large {prec}: Y = {k * 0.05:.0f} after {time.perf_counter() - t0:.1f} s")
+            torch.cuda.synchronize()
+            times[prec] = time.perf_counter() - t0
+            states[prec] = rows
+            ev.close()
+        dn = [float((a - b).abs().max()) for a, b in zip(states["fp32"], states["fp64_stable"])]
+        extra.update({
+            "large_maxy": args.large_maxy,
+            "large_fp32_s": times["fp32"],
+            "large_fp64_stable_s": times["fp64_stable"],
+            "large_speedup_fp32_vs_fp64_stable": times["fp64_stable"] / times["fp32"],
+            "large_max_abs_dN_by_unit_Y": dn,
+        })
+        log(f"large: fp32 {times['fp32']:.1f} s, fp64 {times['fp64_stable']:.1f} s, max |dN| per unit Y {dn}")
 
     if args.batch_runs > 0:
         # Batched uneven evolutions: runs with Qs0^2 ~ U[0.1, 0.5], gamma ~

@@ -35,7 +35,7 @@ extern "C" {
 #define NLO_ERR_WORKSPACE 3   /* workspace too small */
 #define NLO_ERR_UNSUPPORTED 4 /* configuration outside the implemented set */
 
-#define NLO_API_VERSION 2
+#define NLO_API_VERSION 3
 
 /* ------------------------------------------------------------------ misc */
 
@@ -200,6 +200,25 @@ int nlo_bk_k1_history(int32_t R, const double* d_r, const double* d_N, const dou
                       const double* d_hist_y, const double* d_hist_N, const int32_t* d_rows,
                       int32_t capacity, double* d_k1, hipStream_t stream);
 
+/* K1 for every parent by the nested adaptive Gauss-Kronrod 21 rule of
+ * rapidity_derivative_lo (S/bk/derivatives.py:43-110, with
+ * adaptive_gauss_kronrod_1d and _gauss_kronrod_intervals of
+ * S/numerics/integration.py:43-92, :624-746): in ln z over
+ * [ln r_0, ln r_{R-1}] with at most rint intervals (RINTPOINTS) of 2 z^2
+ * times the theta integral over [0, pi] with at most tint intervals
+ * (THETAINTPOINTS), both to epsrel (INTACCURACY); rint, tint <= 100.  The
+ * reference takes this rule wherever its fixed rule is not validated
+ * (S/bk/evolution.py:491-533, S/bk/derivatives.py:145-166).  The history
+ * pointers are all NULL, or all set for the kinematical constraint as in
+ * nlo_bk_k1_history.  d_err (R, may be NULL): the summed outer error
+ * estimate; d_evals (R, may be NULL): integrand evaluations.  Inactive
+ * parents get 0. */
+int nlo_bk_k1_adaptive(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+                       const nlo_bk_physics* physics, int32_t rint, int32_t tint, double epsrel,
+                       const double* d_hist_y, const double* d_hist_N, const int32_t* d_rows,
+                       int32_t capacity, double* d_k1, double* d_err, int64_t* d_evals,
+                       hipStream_t stream);
+
 /* K2 + Kf VEGAS sums for every active parent over Sobol points
  * [first_index, first_index + n) mapped through the parent's grid
  * (d_edges: R x 4 x (bins + 1); row 0 unused) on the box
@@ -230,6 +249,9 @@ int nlo_bk_rhs_finalize(int32_t R, const double* d_N, const double* d_k1, const 
 #define NLO_RK_ADAPTIVE2 4 /* midpoint RK2 with step doubling and step-size control
                               (_adaptive_rk2_to, S/bk/evolution.py:239-326)      */
 
+#define NLO_K1_FIXED 0
+#define NLO_K1_ADAPTIVE 1
+
 typedef struct nlo_bk_evolver_desc {
     int32_t R;            /* r nodes */
     int32_t bins;         /* VEGAS bins per dimension */
@@ -253,6 +275,13 @@ typedef struct nlo_bk_evolver_desc {
     int32_t history_capacity;
     double* d_history_y;  /* capacity doubles */
     double* d_history_N;  /* capacity x R doubles */
+    /* K1 rule: NLO_K1_FIXED (the nr x nt Gauss-Legendre rule per log-r panel)
+     * or NLO_K1_ADAPTIVE (nlo_bk_k1_adaptive with k1_rint, k1_tint,
+     * k1_epsrel) */
+    int32_t k1_rule;
+    int32_t k1_rint, k1_tint;
+    int32_t k1_reserved;
+    double k1_epsrel;
 } nlo_bk_evolver_desc;
 
 typedef struct nlo_bk_evolver nlo_bk_evolver;

@@ -18,6 +18,8 @@ NLO_OK = 0
 NLO_SOBOL_U32, NLO_SOBOL_F32, NLO_SOBOL_F64 = 0, 1, 2
 NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
 NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4, NLO_RK_ADAPTIVE2 = 0, 1, 2, 3, 4
+NLO_K1_FIXED, NLO_K1_ADAPTIVE = 0, 1
+MAX_GK_INTERVALS = 100
 # physics choice codes of nlo_bk_physics (zero = production)
 RC_LO_CODES = {"BALITSKY_LO": 0, "FIXED_LO": 1, "PARENT_LO": 2, "SMALLEST_LO": 3, "FAC_LO": 4, "BEUF_LO": 5}
 RC_NLO_CODES = {"SMALLEST_NLO": 0, "FIXED_NLO": 1, "PARENT_NLO": 2}
@@ -92,6 +94,11 @@ class EvolverDesc(ctypes.Structure):
         ("history_capacity", ctypes.c_int32),
         ("d_history_y", ctypes.c_void_p),
         ("d_history_N", ctypes.c_void_p),
+        ("k1_rule", ctypes.c_int32),
+        ("k1_rint", ctypes.c_int32),
+        ("k1_tint", ctypes.c_int32),
+        ("k1_reserved", ctypes.c_int32),
+        ("k1_epsrel", ctypes.c_double),
     ]
 
 
@@ -184,6 +191,11 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_bk_k1_history": (
         ctypes.c_int,
         [_i32, _vp, _vp, _vp, ctypes.POINTER(Physics), _i32, _i32, _vp, _vp, _vp, _i32, _vp, _vp],
+    ),
+    "nlo_bk_k1_adaptive": (
+        ctypes.c_int,
+        [_i32, _vp, _vp, _vp, ctypes.POINTER(Physics), _i32, _i32, _f64, _vp, _vp, _vp, _i32, _vp, _vp,
+         _vp, _vp],
     ),
     "nlo_bk_nlo_workspace": (_size, [_i32, _i32, _i64, _i32]),
     "nlo_bk_nlo": (

@@ -213,7 +213,7 @@ __device__ __forceinline__ T loglog_from_log(const LdsSpline<T>& s, T log_d, T u
 // log_scale0 - 2 ln d.  The denominator is returned too, so that ratios of
 // couplings need no division: alpha_a / alpha_b = alpha_a * den_b.
 __device__ __forceinline__ double alpha_s64_den(double log_d, const BkConsts& c) {
-    const double right = (c.log_scale0 - 2.0 * log_d) / 0.2;
+    const double right = (c.log_scale0 - 2.0 * log_d) * 5.0;  // / 0.2 within one rounding
     const double hi = fmax(c.log_mu0, right);
     return c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right)))));
 }
@@ -241,16 +241,18 @@ __device__ __forceinline__ double k1_theta_term(const K1Parent& P, const K1Z& Z,
     const double n_x = fmin(loglog_from_log(sN, lnX, 0.0, 1.0), 1.0);
     const double dipole = n_x + Z.n_y - P.n_r - n_x * Z.n_y;
     const double den_x = alpha_s64_den(lnX, c);
-    const double a_x = 1.0 / den_x;
+    // one division for both reciprocals: q = 1 / (den_x X^2)
+    const double q = 1.0 / (den_x * xsq);
+    const double a_x = q * xsq;
+    const double inv_x2 = q * den_x;
     const double Y = z;
     const double a_min = fmin(X, Y) <= r ? (X <= Y ? a_x : Z.a_y) : P.a_r;
-    const double inv_x2 = 1.0 / xsq;
     const double rxy2 = r * r * inv_x2 * Z.inv_y2;  // r^2 / (X^2 Y^2)
     // Kernel_lo: alpha_r (r^2/(X^2 Y^2) + (alpha_Y/alpha_X - 1)/Y^2 + (alpha_X/alpha_Y - 1)/X^2)
     double lo = P.lo_coef * (rxy2 + (Z.a_y * den_x - 1.0) * Z.inv_y2 + (a_x * Z.den_y - 1.0) * inv_x2);
     lo = isfinite(lo) ? lo : 0.0;
     const double xl = 4.0 * (lnX - P.log_r) * Z.ly;
-    const double abar = a_min * c.nc / kPi;
+    const double abar = a_min * (c.nc / kPi);  // loop-invariant quotient
     const double as_x = sqrt(abar * fabs(xl));
     double dlog = 1.0;
     // J1(2a)/a or I1(2a)/a (k1.py:108-132)
@@ -670,7 +672,7 @@ __device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x)
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 // sin y for y in [-pi, pi]: fold |y| > pi/2 with pi - |y| computed exactly
 // (Sterbenz) plus the low part of pi, then the Taylor polynomial to y^11
@@ -771,7 +773,8 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
     const float x2sq = fmaf(4.0f * r * z2, s2 * s2, (r - z2) * (r - z2));
     const float zsq = fmaf(4.0f * z * z2, s3 * s3, (z - z2) * (z - z2));
     if (!(xsq >= 1e-37f) || !(x2sq >= 1e-37f) || !(zsq >= 1e-37f)) return out;
-    const float X = fsqrt(xsq), X2 = fsqrt(x2sq), Z = fsqrt(zsq);
+    // the distances enter only as reciprocals: 1 / X = rsq(X^2)
+    const float iX = frsq(xsq), iX2 = frsq(x2sq), iZ = frsq(zsq);
     // ln(X / Y): (X^2 - z^2) / z^2 = rho (rho - 2 cos t), cos t = 1 - 2 s^2
     const float rho1 = r * frcp(z), rho2 = r * frcp(z2);
     const float u1 = rho1 * (rho1 - 2.0f + 4.0f * s1 * s1);
@@ -794,24 +797,28 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
     const float aL = fabsf(L);
     float g;
     if (aL < 1.0f) {
+        // L coth L - 1 = sum_n 2^2n B_2n L^2n / (2n)!: seven terms reach the
+        // rounding of y = L^2 on |L| < 1 (the terms fall by ~1/pi^2)
         const float y = L * L;
-        const float p = fmaf(y, fmaf(y, fmaf(y, fmaf(y, 1.0f / 518918400.0f, 1.0f / 3991680.0f),
-                                              1.0f / 45360.0f), 1.0f / 840.0f), 1.0f / 30.0f);
-        const float q = fmaf(y, fmaf(y, fmaf(y, fmaf(y, 1.0f / 39916800.0f, 1.0f / 362880.0f),
-                                              1.0f / 5040.0f), 1.0f / 120.0f), 1.0f / 6.0f);
-        g = y * fmaf(p, y, 1.0f / 3.0f) * frcp(fmaf(q, y, 1.0f));
+        float p = fmaf(y, 2.19259479e-7f, -2.16440428e-6f);
+        p = fmaf(y, p, 2.13777992e-5f);
+        p = fmaf(y, p, -2.11640212e-4f);
+        p = fmaf(y, p, 2.11640212e-3f);
+        p = fmaf(y, p, -2.22222222e-2f);
+        p = fmaf(y, p, 3.33333333e-1f);
+        g = y * p;
     } else {
         g = aL + (L >= 0.0f ? h : hm) - 1.0f;  // |L| + h(|L|) - 1
     }
-    const float iz = frcp(zsq);
-    const float A = z * z2 * iz;
-    const float rZ = r * frcp(Z);
-    const float B = rZ * z2 * frcp(X2), C = r * r * frcp(X * X2), D = rZ * z * frcp(X);
+    const float A = z * z2 * (iZ * iZ);
+    const float rZ = r * iZ;
+    const float B = rZ * z2 * iX2, C = r * r * (iX * iX2), D = rZ * z * iX;
     const float B2 = B * B, C2 = C * C, D2 = D * D;
     const float A2g = 2.0f * A * A * g;
     float k = A2g - 4.0f * h * B2 + h * C2 + 2.0f * L * D2;
     float ks = A2g - 4.0f * hm * D2 + hm * C2 - 2.0f * L * B2;
-    const float cut = 1e10f * fexp(2.0f * (lz + lz2));  // |K2| > 1e10 (S/bk/derivatives.py:589)
+    const float zz2 = z * z2;
+    const float cut = 1e10f * (zz2 * zz2);  // |K2| > 1e10 (S/bk/derivatives.py:589), J = z^2 z'^2
     if (fabsf(k) > cut && fabsf(dip) < 1e-10f) { k = 0.0f; dip = 0.0f; }
     if (fabsf(ks) > cut && fabsf(dips) < 1e-10f) { ks = 0.0f; dips = 0.0f; }
     float k2r = symmetrize ? (k * dip + ks * dips) * 0.5f : k * dip;
@@ -828,7 +835,9 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
     if (!ANYC || cpl.mode == 0) {  // smooth coupling at the smallest distance (production)
         const float right = 5.0f * (cpl.log_scale0 - 2.0f * log_min);
         const float hi = fmaxf(cpl.log_mu0, right);
-        as = frcp(cpl.b0 * (0.2f * (hi + flog(1.0f + fexp(-fabsf(cpl.log_mu0 - right))))));
+        // e^-|d| <= 1 is added to 1: the plain 2^x is accurate enough here
+        const float t = __builtin_amdgcn_exp2f(-fabsf(cpl.log_mu0 - right) * 1.44269502f);
+        as = frcp(cpl.b0 * (0.2f * (hi + flog(1.0f + t))));
     } else if (cpl.mode == 1) {
         as = cpl.as_const;
     } else {

@@ -205,7 +205,11 @@ int enqueue_rhs(nlo_bk_evolver* ev, const double* in, double* out, hipStream_t s
     int rc = launch_spline_build(R, ev->r, in, d.force_positive, ev->t64, ev->t32, s);
     if (rc != NLO_OK) return rc;
     K1History H{d.d_history_y, d.d_history_N, ev->r, ev->rows, R, d.history_capacity};
-    rc = launch_k1(R, ev->r, in, ev->t64, ev->c, d.nr, d.nt, ev->k1, s, d.kinematical ? &H : nullptr);
+    if (d.k1_rule == NLO_K1_ADAPTIVE)
+        rc = launch_k1_adaptive(R, ev->r, in, ev->t64, ev->c, d.k1_rint, d.k1_tint, d.k1_epsrel, ev->k1,
+                                nullptr, nullptr, s, d.kinematical ? &H : nullptr);
+    else
+        rc = launch_k1(R, ev->r, in, ev->t64, ev->c, d.nr, d.nt, ev->k1, s, d.kinematical ? &H : nullptr);
     if (rc != NLO_OK) return rc;
     if (d.nlo) {
         rc = launch_nlo(R, ev->r, in, ev->t64, ev->t32, ev->edges, d.bins, d.sobol, d.first_index,
@@ -391,6 +395,13 @@ int nlo_bk_evolver_create(const nlo_bk_evolver_desc* desc, const double* d_r, co
     NLO_REQUIRE(d.nr >= 1 && d.nr <= nlo::kMaxGaussR && d.nt >= 1 && d.nt <= nlo::kMaxGaussT,
                 "nlo_bk_evolver_create: GL orders invalid");
     NLO_REQUIRE(d.sobol.dim == 4, "nlo_bk_evolver_create: 4-dimensional Sobol table required");
+    NLO_REQUIRE(d.k1_rule == NLO_K1_FIXED || d.k1_rule == NLO_K1_ADAPTIVE,
+                "nlo_bk_evolver_create: unknown k1_rule %d", d.k1_rule);
+    if (d.k1_rule == NLO_K1_ADAPTIVE)
+        NLO_REQUIRE(d.k1_rint >= 1 && d.k1_rint <= nlo::kMaxGkIntervals && d.k1_tint >= 1 &&
+                        d.k1_tint <= nlo::kMaxGkIntervals && d.k1_epsrel > 0.0,
+                    "nlo_bk_evolver_create: adaptive K1 needs interval limits in [1, %d] and epsrel > 0",
+                    nlo::kMaxGkIntervals);
     NLO_REQUIRE(d_r && d_edges && d_workspace, "nlo_bk_evolver_create: null pointer");
     if (d.rk_method == NLO_RK_ADAPTIVE2)
         NLO_REQUIRE(d.abserr >= 0.0 && d.relerr >= 0.0 && d.abserr + d.relerr > 0.0 && d.eps > 0.0,

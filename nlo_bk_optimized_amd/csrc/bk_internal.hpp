@@ -9,6 +9,7 @@ namespace nlo {
 constexpr int kMaxR = 2048;       // r nodes (LDS: FP64 spline + Thomas scratch)
 constexpr int kMaxGaussR = 32;    // K1 radial GL order
 constexpr int kMaxGaussT = 64;    // K1 angular GL order
+constexpr int kMaxGkIntervals = 100;  // adaptive K1: intervals per list (RINTPOINTS, THETAINTPOINTS)
 
 struct NloPlan {
     int spt;              // samples per lane
@@ -30,6 +31,9 @@ int launch_spline_build(int R, const double* r, const double* N, int force_posit
                         float* t32, hipStream_t stream);
 int launch_k1(int R, const double* r, const double* N, const double* t64, const BkConsts& c, int nr,
               int nt, double* k1, hipStream_t stream, const K1History* hist = nullptr);
+int launch_k1_adaptive(int R, const double* r, const double* N, const double* t64, const BkConsts& c,
+                       int rint, int tint, double epsrel, double* k1, double* k1_err,
+                       long long* k1_evals, hipStream_t stream, const K1History* hist = nullptr);
 int launch_nlo(int R, const double* r, const double* N, const double* t64, const float* t32,
                const double* edges, int bins, const nlo_sobol_table& sobol, int64_t first, int64_t n,
                int precision, const BkConsts& c, double* sums, double* hist, double* count, void* ws,

@@ -292,6 +292,336 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
     k1_body<ANY>(R, r, N, t64, c, g, H, blockIdx.x, k1);
 }
 
+// ------------------------------------------------------ K1, adaptive rule
+
+// rapidity_derivative_lo (S/bk/derivatives.py:43-110): adaptive
+// Gauss-Kronrod 21 (adaptive_gauss_kronrod_1d, S/numerics/integration.py:
+// 43-92; the rule and its error of :624-746) in ln z over [ln r_0, ln r_{R-1}]
+// of 2 z^2 times the adaptive rule in theta over [0, pi].  One block per
+// parent: thread 0 keeps the outer interval list; every outer node is an
+// inner integral, taken by the next free wave, whose 64 lanes evaluate the
+// 21 (first interval) or 42 (two halves) theta nodes of each bisection step
+// while every lane runs the same list bookkeeping on LDS.  Interval lists
+// stay in order and sums run in list order, as in the reference.
+constexpr int kK1aWaves = 16;
+
+// QUADPACK's 21-point Kronrod extension of the 10-point Gauss rule: nodes
+// kGkX[0..9] (kGkX[1, 3, .., 9] are the Gauss nodes), centre node last.
+__device__ constexpr double kGkX[10] = {
+    0.995657163025808080735527280689003, 0.973906528517171720077964012084452,
+    0.930157491355708226001207180059508, 0.865063366688984510732096688423493,
+    0.780817726586416897063717578345042, 0.679409568299024406234327365114874,
+    0.562757134668604683339000099272694, 0.433395394129247190799265943165784,
+    0.294392862701460198131126603103866, 0.148874338981631210884826001129720};
+__device__ constexpr double kGkW[10] = {
+    0.011694638867371874278064396062192, 0.032558162307964727478818972459390,
+    0.054755896574351996031381300244580, 0.075039674810919952767043140916190,
+    0.093125454583697605535065465083366, 0.109387158802297641899210590325805,
+    0.123491976262065851077958109831074, 0.134709217311473325928054001771707,
+    0.142775938577060080797094273138717, 0.147739104901338491374841515972068};
+constexpr double kGkWCenter = 0.149445554002916905664936468389821;
+__device__ constexpr double kGW[5] = {
+    0.066671344308688137593568809893332, 0.149451349150580593145776339657697,
+    0.219086362515982043995534934228163, 0.269266719309996355091226921569469,
+    0.295524224714752870173892994651338};
+
+// Node k (0..20) of [a, b]: centre - h x_k, k < 10; centre, k = 10; centre + h x_{k-11}.
+__device__ __forceinline__ double gk_node(double a, double b, int k) {
+    const double center = (a + b) / 2.0, half = (b - a) / 2.0;
+    if (k < 10) return center - half * kGkX[k];
+    if (k == 10) return center;
+    return center + half * kGkX[k - 11];
+}
+
+// The estimate and error of _gauss_kronrod_intervals from the 21 values v
+// (node order of gk_node): min(1, (200 |K - G| / resasc)^1.5) resasc, at
+// least 50 eps resabs.
+__device__ __forceinline__ void gk_from_values(double a, double b, const double* v, double& est,
+                                               double& err) {
+    const double half = (b - a) / 2.0;
+    const double fc = v[10];
+    double kron = 0.0, gauss = 0.0;
+    for (int i = 0; i < 10; ++i) kron += (v[i] + v[11 + i]) * kGkW[i];
+    kron += kGkWCenter * fc;
+    for (int i = 0; i < 5; ++i) gauss += (v[2 * i + 1] + v[12 + 2 * i]) * kGW[i];
+    const double mean = kron / 2.0;
+    double sabs = 0.0, sasc = 0.0;
+    for (int i = 0; i < 10; ++i) {
+        sabs += (fabs(v[i]) + fabs(v[11 + i])) * kGkW[i];
+        sasc += (fabs(v[i] - mean) + fabs(v[11 + i] - mean)) * kGkW[i];
+    }
+    const double resabs = half * (sabs + kGkWCenter * fabs(fc));
+    const double resasc = half * (sasc + kGkWCenter * fabs(fc - mean));
+    double e = half * fabs(kron - gauss);
+    const double scale = pow(200.0 * e / (resasc > 0.0 ? resasc : 1.0), 1.5);
+    if (resasc > 0.0 && e > 0.0) e = resasc * fmin(scale, 1.0);
+    est = half * kron;
+    err = fmax(e, 50.0 * 2.220446049250313e-16 * resabs);
+}
+
+// Ordered interval list in LDS: lower, upper, estimate, error (cap entries each).
+struct GkList {
+    double* lo;
+    double* hi;
+    double* est;
+    double* err;
+};
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Summed estimate and error in list order; first interval of largest error.
+__device__ __forceinline__ void gk_scan(const GkList& L, int count, double& value, double& error,
+                                        int& split) {
+    value = 0.0;
+    error = 0.0;
+    split = 0;
+    for (int i = 0; i < count; ++i) {
+        value += L.est[i];
+        error += L.err[i];
+        if (L.err[i] > L.err[split]) split = i;
+    }
+}
+
+// Replace entry `split` of a list of `count` (< cap <= 128) entries by its
+// halves [lo, mid] and [mid, hi] (estimates e0, e1; errors r0, r1), moving the
+// entries after it one place up.  Every lane of one wave calls it: each lane
+// reads up to two of the entries to move, the fence orders all reads before
+// the writes, then lane 0 writes the two halves.
+__device__ __forceinline__ void gk_insert_wave(const GkList& L, int count, int split, double mid,
+                                               double e0, double r0, double e1, double r1, int lane) {
+    const double l = L.lo[split], u = L.hi[split];
+    double m[2][4];
+    int idx[2];
+    for (int k = 0; k < 2; ++k) {
+        idx[k] = split + 1 + lane + 64 * k;
+        if (idx[k] < count) {
+            m[k][0] = L.lo[idx[k]];
+            m[k][1] = L.hi[idx[k]];
+            m[k][2] = L.est[idx[k]];
+            m[k][3] = L.err[idx[k]];
+        }
+    }
+    wave_sync();
+    for (int k = 0; k < 2; ++k) {
+        if (idx[k] < count) {
+            L.lo[idx[k] + 1] = m[k][0];
+            L.hi[idx[k] + 1] = m[k][1];
+            L.est[idx[k] + 1] = m[k][2];
+            L.err[idx[k] + 1] = m[k][3];
+        }
+    }
+    if (lane == 0) {
+        L.lo[split] = l;
+        L.hi[split] = mid;
+        L.est[split] = e0;
+        L.err[split] = r0;
+        L.lo[split + 1] = mid;
+        L.hi[split + 1] = u;
+        L.est[split + 1] = e1;
+        L.err[split + 1] = r1;
+    }
+}
+
+// The same by one thread.
+__device__ __forceinline__ void gk_insert_serial(const GkList& L, int count, int split, double mid,
+                                                 double e0, double r0, double e1, double r1) {
+    const double u = L.hi[split];
+    for (int i = count; i > split + 1; --i) {
+        L.lo[i] = L.lo[i - 1];
+        L.hi[i] = L.hi[i - 1];
+        L.est[i] = L.est[i - 1];
+        L.err[i] = L.err[i - 1];
+    }
+    L.hi[split] = mid;
+    L.est[split] = e0;
+    L.err[split] = r0;
+    L.lo[split + 1] = mid;
+    L.hi[split + 1] = u;
+    L.est[split + 1] = e1;
+    L.err[split + 1] = r1;
+}
+
+// The theta integral at one daughter size z for parent P, by one wave.
+template <bool ANY>
+__device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const LdsSpline<double>& s,
+                                    const BkConsts& c, const K1History& H, int rows, double rapidity,
+                                    int tint, double epsrel, const GkList& L, double* v, int lane,
+                                    long long& evals) {
+    K1Z Z;
+    Z.log_z = log_z;
+    Z.z = exp(log_z);
+    Z.n_y = fmin(loglog_from_log(s, Z.log_z, 0.0, 1.0), 1.0);
+    if constexpr (!ANY) {
+        Z.den_y = alpha_s64_den(Z.log_z, c);
+        Z.a_y = 1.0 / Z.den_y;
+        Z.ly = Z.log_z - P.log_r;
+        Z.inv_y2 = 1.0 / (Z.z * Z.z);
+    }
+    auto f = [&](double theta) {
+        const double sn = sin(0.5 * theta);
+        if constexpr (ANY)
+            return k1_theta_term_any(P, Z, sn * sn, s, c, H, rows, rapidity);
+        else
+            return k1_theta_term(P, Z, sn * sn, s, c);
+    };
+    if (lane < 21) v[lane] = f(gk_node(0.0, kPi, lane));
+    wave_sync();
+    double e, r;
+    gk_from_values(0.0, kPi, v, e, r);
+    wave_sync();
+    if (lane == 0) {
+        L.lo[0] = 0.0;
+        L.hi[0] = kPi;
+        L.est[0] = e;
+        L.err[0] = r;
+    }
+    evals += 21;
+    int count = 1;
+    double value = e;
+    for (;;) {
+        wave_sync();
+        double error;
+        int split;
+        gk_scan(L, count, value, error, split);
+        if (count >= tint || error <= epsrel * fabs(value)) break;
+        const double a = L.lo[split], b = L.hi[split], mid = (a + b) / 2.0;
+        if (lane < 42) v[lane] = f(lane < 21 ? gk_node(a, mid, lane) : gk_node(mid, b, lane - 21));
+        wave_sync();
+        double e0, r0, e1, r1;
+        gk_from_values(a, mid, v, e0, r0);
+        gk_from_values(mid, b, v + 21, e1, r1);
+        gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
+        ++count;
+        evals += 42;
+    }
+    wave_sync();
+    return value;
+}
+
+// LDS of the adaptive K1 kernel (doubles): spline (R nodes + 4 (R - 1)
+// coefficients), J1 table, outer list (4 cap), outer nodes and values (2 x
+// 42), then per wave a list (4 cap) and 42 node values.
+__host__ __device__ inline size_t k1a_lds_doubles(int R, int cap) {
+    return (size_t)R + 4 * (size_t)(R - 1) + kJ1Pieces * (kJ1Deg + 1) + 4 * (size_t)cap + 84 +
+           (size_t)kK1aWaves * (4 * (size_t)cap + 42);
+}
+
+template <bool ANY>
+__global__ void __launch_bounds__(kK1aWaves * 64)
+    k1_adaptive_kernel(int R, const double* __restrict__ r, const double* __restrict__ N,
+                       const double* __restrict__ t64, BkConsts c, K1History H, int rint, int tint,
+                       int cap, double epsrel, double* __restrict__ k1, double* __restrict__ k1_err,
+                       long long* __restrict__ k1_evals) {
+    extern __shared__ double sh[];
+    __shared__ int task_next, n_tasks, o_done;
+    __shared__ unsigned long long ev_total;
+    const int p = blockIdx.x;
+    if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
+        if (threadIdx.x == 0) {
+            k1[p] = 0.0;
+            if (k1_err) k1_err[p] = 0.0;
+            if (k1_evals) k1_evals[p] = 0;
+        }
+        return;
+    }
+    double* j1tab = sh + R + 4 * (R - 1);
+    double* o_base = j1tab + kJ1Pieces * (kJ1Deg + 1);
+    const GkList O{o_base, o_base + cap, o_base + 2 * cap, o_base + 3 * cap};
+    double* o_node = o_base + 4 * cap;  // 42 ln z nodes
+    double* o_val = o_node + 42;        // 42 outer integrand values
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double* w_base = o_val + 42 + (size_t)wave * (4 * cap + 42);
+    const GkList W{w_base, w_base + cap, w_base + 2 * cap, w_base + 3 * cap};
+    double* w_val = w_base + 4 * cap;
+
+    for (int i = threadIdx.x; i < kJ1Pieces * (kJ1Deg + 1); i += blockDim.x) j1tab[i] = kJ1Cheb[i];
+    LdsSpline<double> s;
+    stage_spline<double>(t64, R, 0, sh, sh + R, s);
+    const double a0 = t64[2], b0 = t64[2 + R - 1];  // ln r_0, ln r_{R-1}
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 21; ++k) o_node[k] = gk_node(a0, b0, k);
+        n_tasks = 21;
+        task_next = 0;
+        o_done = 0;
+        ev_total = 0ull;
+    }
+    __syncthreads();
+    K1Parent P;
+    P.r = r[p];
+    P.log_r = log(P.r);
+    P.n_r = fmin(loglog64(s, t64[0], t64[1], P.r, 0.0, 1.0), 1.0);
+    P.a_r = ANY ? 0.0 : alpha_s64(P.r, c);
+    P.lo_coef = c.nc / (2.0 * kPi * kPi) * P.a_r;
+    P.fin_coef = P.a_r * c.nc / (4.0 * kPi) * (67.0 / 9.0 - kPi * kPi / 3.0 - 10.0 / 9.0 * c.nf / c.nc);
+    P.log_ksub = log(c.ksub);
+    P.resum_on = P.r > 1.01 * c.minr ? 1.0 : 0.0;
+    P.j1tab = j1tab;
+    const int rows = (ANY && H.rows) ? min(max(*H.rows, 1), H.capacity) : 0;
+    const double rapidity = rows > 0 ? H.y[rows - 1] : 0.0;
+
+    int count = 0;  // outer intervals (thread 0's copy is authoritative)
+    int split = 0;
+    double value = 0.0, error = 0.0;
+    long long evals = 0;
+    for (int step = 0; step < rint; ++step) {
+        // inner integrals at the pending outer nodes, one wave each
+        for (;;) {
+            int t = 0;
+            if (lane == 0) t = atomicAdd(&task_next, 1);
+            t = __shfl(t, 0, 64);
+            if (t >= n_tasks) break;
+            const double lz = o_node[t];
+            const double inner = k1_theta_adaptive<ANY>(P, lz, s, c, H, rows, rapidity, tint, epsrel, W,
+                                                        w_val, lane, evals);
+            if (lane == 0) o_val[t] = inner * exp(2.0 * lz) * 2.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (count == 0) {
+                double e, rr;
+                gk_from_values(a0, b0, o_val, e, rr);
+                O.lo[0] = a0;
+                O.hi[0] = b0;
+                O.est[0] = e;
+                O.err[0] = rr;
+                count = 1;
+            } else {
+                const double a = O.lo[split], b = O.hi[split], mid = (a + b) / 2.0;
+                double e0, r0, e1, r1;
+                gk_from_values(a, mid, o_val, e0, r0);
+                gk_from_values(mid, b, o_val + 21, e1, r1);
+                gk_insert_serial(O, count, split, mid, e0, r0, e1, r1);
+                ++count;
+            }
+            gk_scan(O, count, value, error, split);
+            if (count >= rint || error <= epsrel * fabs(value)) {
+                o_done = 1;
+            } else {
+                const double a = O.lo[split], b = O.hi[split], mid = (a + b) / 2.0;
+                for (int k = 0; k < 21; ++k) {
+                    o_node[k] = gk_node(a, mid, k);
+                    o_node[21 + k] = gk_node(mid, b, k);
+                }
+                n_tasks = 42;
+                task_next = 0;
+            }
+        }
+        __syncthreads();
+        if (o_done) break;
+    }
+    if (lane == 0) atomicAdd(&ev_total, (unsigned long long)evals);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        k1[p] = value;
+        if (k1_err) k1_err[p] = error;
+        if (k1_evals) k1_evals[p] = (long long)ev_total;
+    }
+}
+
 // ------------------------------------------------------------- K2 + Kf
 
 // LDS layout of the NLO kernel (bytes), shared by host sizing and device.
@@ -902,6 +1232,8 @@ int init_kernel_attributes() {
             (const void*)spline_build_kernel,
             (const void*)k1_kernel<false>,
             (const void*)k1_kernel<true>,
+            (const void*)k1_adaptive_kernel<false>,
+            (const void*)k1_adaptive_kernel<true>,
             (const void*)spline_build_batch_kernel,
             (const void*)k1_batch_kernel<false>,
             (const void*)k1_batch_kernel<true>,
@@ -962,6 +1294,31 @@ int launch_k1(int R, const double* r, const double* N, const double* t64, const 
     else
         hipLaunchKernelGGL(k1_kernel<true>, dim3(R), dim3(kK1Threads), smem, stream, R, r, N, t64, c, g,
                            H, k1);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_k1_adaptive(int R, const double* r, const double* N, const double* t64, const BkConsts& c,
+                       int rint, int tint, double epsrel, double* k1, double* k1_err,
+                       long long* k1_evals, hipStream_t stream, const K1History* hist) {
+    const int cap = rint > tint ? rint : tint;
+    const size_t smem = k1a_lds_doubles(R, cap) * sizeof(double);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    int dev = 0, max_lds = 0;
+    NLO_HIP_CHECK(hipGetDevice(&dev));
+    NLO_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    NLO_REQUIRE(smem + 64 <= (size_t)max_lds,
+                "adaptive K1: R=%d with %d intervals needs %zu bytes of LDS (device: %d)", R, cap, smem,
+                max_lds);
+    K1History H{};
+    if (hist) H = *hist;
+    if (c.production && !hist)
+        hipLaunchKernelGGL(k1_adaptive_kernel<false>, dim3(R), dim3(kK1aWaves * 64), smem, stream, R, r,
+                           N, t64, c, H, rint, tint, cap, epsrel, k1, k1_err, k1_evals);
+    else
+        hipLaunchKernelGGL(k1_adaptive_kernel<true>, dim3(R), dim3(kK1aWaves * 64), smem, stream, R, r,
+                           N, t64, c, H, rint, tint, cap, epsrel, k1, k1_err, k1_evals);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }
@@ -1266,6 +1623,30 @@ int nlo_bk_k1_history(int32_t R, const double* d_r, const double* d_N, const dou
     if (rc != NLO_OK) return rc;
     nlo::K1History H{d_hist_y, d_hist_N, d_r, d_rows, R, capacity};
     rc = nlo::launch_k1(R, d_r, d_N, d_table64, nlo::make_consts(*physics), nr, nt, d_k1, stream, &H);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+int nlo_bk_k1_adaptive(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+                       const nlo_bk_physics* physics, int32_t rint, int32_t tint, double epsrel,
+                       const double* d_hist_y, const double* d_hist_N, const int32_t* d_rows,
+                       int32_t capacity, double* d_k1, double* d_err, int64_t* d_evals,
+                       hipStream_t stream) {
+    NLO_REQUIRE(R >= 3 && R <= nlo::kMaxR, "nlo_bk_k1_adaptive: R=%d outside [3, %d]", R, nlo::kMaxR);
+    NLO_REQUIRE(rint >= 1 && rint <= nlo::kMaxGkIntervals && tint >= 1 && tint <= nlo::kMaxGkIntervals,
+                "nlo_bk_k1_adaptive: interval limits (%d, %d) outside [1, %d]", rint, tint,
+                nlo::kMaxGkIntervals);
+    NLO_REQUIRE(epsrel > 0.0, "nlo_bk_k1_adaptive: epsrel must be positive");
+    NLO_REQUIRE(d_r && d_N && d_table64 && d_k1, "nlo_bk_k1_adaptive: null pointer");
+    const bool kin = d_hist_y || d_hist_N || d_rows;
+    NLO_REQUIRE(!kin || (d_hist_y && d_hist_N && d_rows && capacity >= 1),
+                "nlo_bk_k1_adaptive: the history needs y, N, rows and a positive capacity");
+    int rc = nlo::validate_physics(physics);
+    if (rc != NLO_OK) return rc;
+    nlo::K1History H{d_hist_y, d_hist_N, d_r, d_rows, R, capacity};
+    rc = nlo::launch_k1_adaptive(R, d_r, d_N, d_table64, nlo::make_consts(*physics), rint, tint, epsrel,
+                                 d_k1, d_err, reinterpret_cast<long long*>(d_evals), stream,
+                                 kin ? &H : nullptr);
     if (rc == NLO_OK) nlo::clear_error();
     return rc;
 }

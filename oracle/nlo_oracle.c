@@ -480,6 +480,161 @@ void orc_k1_fixed(int parents, const double* parent_r, const orc_spline* sN, int
     orc_k1_fixed_history(parents, parent_r, sN, nr, gr_x, gr_w, nt, gt_x, gt_w, p, NULL, out);
 }
 
+/* ------------------------------------------------ adaptive Gauss-Kronrod */
+
+/* QUADPACK's 21-point Kronrod extension of the 10-point Gauss rule: nodes
+ * xgk[0..9] (xgk[1, 3, .., 9] are the Gauss nodes), the centre is node 10. */
+static const double kXgk[10] = {
+    0.995657163025808080735527280689003, 0.973906528517171720077964012084452,
+    0.930157491355708226001207180059508, 0.865063366688984510732096688423493,
+    0.780817726586416897063717578345042, 0.679409568299024406234327365114874,
+    0.562757134668604683339000099272694, 0.433395394129247190799265943165784,
+    0.294392862701460198131126603103866, 0.148874338981631210884826001129720};
+static const double kWgk[10] = {
+    0.011694638867371874278064396062192, 0.032558162307964727478818972459390,
+    0.054755896574351996031381300244580, 0.075039674810919952767043140916190,
+    0.093125454583697605535065465083366, 0.109387158802297641899210590325805,
+    0.123491976262065851077958109831074, 0.134709217311473325928054001771707,
+    0.142775938577060080797094273138717, 0.147739104901338491374841515972068};
+static const double kWgkCenter = 0.149445554002916905664936468389821;
+static const double kWg[5] = {
+    0.066671344308688137593568809893332, 0.149451349150580593145776339657697,
+    0.219086362515982043995534934228163, 0.269266719309996355091226921569469,
+    0.295524224714752870173892994651338};
+
+void orc_gk21(orc_fn f, void* ctx, double a, double b, double* estimate, double* error) {
+    const double center = (a + b) / 2.0, half = (b - a) / 2.0;
+    double neg[10], pos[10];
+    for (int i = 0; i < 10; ++i) {
+        neg[i] = f(center - half * kXgk[i], ctx);
+    }
+    const double fc = f(center, ctx);
+    for (int i = 0; i < 10; ++i) {
+        pos[i] = f(center + half * kXgk[i], ctx);
+    }
+    double kron = 0.0, gauss = 0.0;
+    for (int i = 0; i < 10; ++i) kron += (neg[i] + pos[i]) * kWgk[i];
+    kron += kWgkCenter * fc;
+    for (int i = 0; i < 5; ++i) gauss += (neg[2 * i + 1] + pos[2 * i + 1]) * kWg[i];
+    const double mean = kron / 2.0;
+    double sabs = 0.0, sasc = 0.0;
+    for (int i = 0; i < 10; ++i) {
+        sabs += (fabs(neg[i]) + fabs(pos[i])) * kWgk[i];
+        sasc += (fabs(neg[i] - mean) + fabs(pos[i] - mean)) * kWgk[i];
+    }
+    const double resabs = half * (sabs + kWgkCenter * fabs(fc));
+    const double resasc = half * (sasc + kWgkCenter * fabs(fc - mean));
+    double e = half * fabs(kron - gauss);
+    const double scale = pow(200.0 * e / (resasc > 0.0 ? resasc : 1.0), 1.5);
+    if (resasc > 0.0 && e > 0.0) e = resasc * (scale < 1.0 ? scale : 1.0);
+    const double floor_e = 50.0 * 2.220446049250313e-16 * resabs;
+    *estimate = half * kron;
+    *error = e > floor_e ? e : floor_e;
+}
+
+double orc_adaptive_gk21(orc_fn f, void* ctx, double a, double b, double epsrel, int max_intervals,
+                         double* error, int* converged, long* evaluations) {
+    double* lo = (double*)malloc(sizeof(double) * 4 * (size_t)(max_intervals + 1));
+    double* hi = lo + max_intervals + 1;
+    double* est = hi + max_intervals + 1;
+    double* err = est + max_intervals + 1;
+    int count = 1;
+    long nodes = 21;
+    lo[0] = a;
+    hi[0] = b;
+    orc_gk21(f, ctx, a, b, &est[0], &err[0]);
+    double value = 0.0, total = 0.0;
+    int done = 0;
+    while (count < max_intervals) {
+        value = 0.0;
+        total = 0.0;
+        for (int i = 0; i < count; ++i) {
+            value += est[i];
+            total += err[i];
+        }
+        if (total <= epsrel * fabs(value)) {
+            done = 1;
+            break;
+        }
+        int split = 0;
+        for (int i = 1; i < count; ++i)
+            if (err[i] > err[split]) split = i;
+        const double l = lo[split], u = hi[split], mid = (l + u) / 2.0;
+        double e0, r0, e1, r1;
+        orc_gk21(f, ctx, l, mid, &e0, &r0);
+        orc_gk21(f, ctx, mid, u, &e1, &r1);
+        for (int i = count; i > split + 1; --i) {
+            lo[i] = lo[i - 1];
+            hi[i] = hi[i - 1];
+            est[i] = est[i - 1];
+            err[i] = err[i - 1];
+        }
+        hi[split] = mid;
+        est[split] = e0;
+        err[split] = r0;
+        lo[split + 1] = mid;
+        hi[split + 1] = u;
+        est[split + 1] = e1;
+        err[split + 1] = r1;
+        ++count;
+        nodes += 42;
+    }
+    if (!done) {
+        value = 0.0;
+        total = 0.0;
+        for (int i = 0; i < count; ++i) {
+            value += est[i];
+            total += err[i];
+        }
+        done = total <= epsrel * fabs(value);
+    }
+    free(lo);
+    if (error) *error = total;
+    if (converged) *converged = done;
+    if (evaluations) *evaluations = nodes;
+    return value;
+}
+
+typedef struct {
+    double r, z;
+    const orc_spline* sN;
+    const orc_bk_params* p;
+    const orc_history* h;
+    int tint;
+    double epsrel;
+    long evals;
+} k1_ctx;
+
+static double k1_theta_fn(double theta, void* ctx) {
+    const k1_ctx* c = (const k1_ctx*)ctx;
+    return c->h ? orc_k1_integrand_history(c->r, c->z, theta, c->sN, c->p, c->h)
+                : orc_k1_integrand(c->r, c->z, theta, c->sN, c->p);
+}
+
+/* integrate_theta (S/bk/derivatives.py:61-101): 2 z^2 x the theta integral. */
+static double k1_logz_fn(double log_z, void* ctx) {
+    k1_ctx* c = (k1_ctx*)ctx;
+    c->z = exp(log_z);
+    long n = 0;
+    const double v = orc_adaptive_gk21(k1_theta_fn, c, 0.0, M_PI, c->epsrel, c->tint, NULL, NULL, &n);
+    c->evals += n;
+    return v * exp(2.0 * log_z) * 2.0;
+}
+
+void orc_k1_adaptive(int parents, const double* parent_r, const orc_spline* sN, int rint, int tint,
+                     double epsrel, const orc_bk_params* p, const orc_history* h, double* out,
+                     double* err, long* evals) {
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int pi = 0; pi < parents; ++pi) {
+        k1_ctx c = {parent_r[pi], 0.0, sN, p, h, tint, epsrel, 0};
+        double e = 0.0;
+        out[pi] = orc_adaptive_gk21(k1_logz_fn, &c, sN->x[0], sN->x[sN->n - 1], epsrel, rint, &e,
+                                    NULL, NULL);
+        if (err) err[pi] = e;
+        if (evals) evals[pi] = c.evals;
+    }
+}
+
 /* Same rule over |integrand| magnitudes: the rounding scale of K1. */
 void orc_k1_fixed_magnitude(int parents, const double* parent_r, const orc_spline* sN, int nr,
                             const double* gr_x, const double* gr_w, int nt, const double* gt_x,

@@ -157,6 +157,29 @@ double orc_history_N(double r, double rapidity, const orc_spline* sN, const orc_
 double orc_k1_integrand_history(double r, double z, double theta, const orc_spline* sN,
                                 const orc_bk_params* p, const orc_history* h);
 
+/* Gauss-Kronrod 21 on [a, b] as _gauss_kronrod_intervals
+ * (S/numerics/integration.py:624-746) forms it: the Kronrod estimate and the
+ * error min(1, (200 |K - G| / resasc)^1.5) resasc, at least 50 eps resabs. */
+typedef double (*orc_fn)(double x, void* ctx);
+void orc_gk21(orc_fn f, void* ctx, double a, double b, double* estimate, double* error);
+
+/* adaptive_gauss_kronrod_1d (S/numerics/integration.py:43-92) with rule 21
+ * and epsabs 0: bisect the interval of largest error (the first of equal
+ * ones) in an ordered list until the summed error <= epsrel |summed
+ * estimate| or max_intervals intervals.  error, converged, evaluations may be
+ * NULL. */
+double orc_adaptive_gk21(orc_fn f, void* ctx, double a, double b, double epsrel, int max_intervals,
+                         double* error, int* converged, long* evaluations);
+
+/* rapidity_derivative_lo (S/bk/derivatives.py:43-110): the adaptive rule in
+ * log z over [ln r_0, ln r_{n-1}] (rint intervals) of 2 z^2 times the
+ * adaptive rule in theta over [0, pi] (tint intervals), both to epsrel.  h
+ * (may be NULL) switches the kinematical constraint on.  err and evals (may
+ * be NULL) receive the outer error estimate and the integrand evaluations. */
+void orc_k1_adaptive(int parents, const double* parent_r, const orc_spline* sN, int rint, int tint,
+                     double epsrel, const orc_bk_params* p, const orc_history* h, double* out,
+                     double* err, long* evals);
+
 void orc_k1_fixed_magnitude(int parents, const double* parent_r, const orc_spline* sN,
                             int nr, const double* gr_x, const double* gr_w,
                             int nt, const double* gt_x, const double* gt_w,

@@ -35,6 +35,7 @@ _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "libnlo_oracle.so"
 
 _dp = ctypes.POINTER(ctypes.c_double)
+_GK_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_double, ctypes.c_void_p)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i32p = ctypes.POINTER(ctypes.c_int32)
 
@@ -184,6 +185,15 @@ def lib() -> ctypes.CDLL:
             ctypes.POINTER(_Spline),
             ctypes.POINTER(BKParams),
             ctypes.POINTER(_History),
+        ]
+        L.orc_adaptive_gk21.restype = ctypes.c_double
+        L.orc_adaptive_gk21.argtypes = [_GK_FN, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, _dp,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long)]
+        L.orc_k1_adaptive.argtypes = [
+            ctypes.c_int, _dp, ctypes.POINTER(_Spline), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+            ctypes.POINTER(BKParams), ctypes.POINTER(_History), _dp, _dp,
+            ctypes.POINTER(ctypes.c_long),
         ]
         L.orc_sobol_u32.argtypes = [
             ctypes.c_int,
@@ -498,6 +508,34 @@ def history_N(r, rapidity, sN: Spline, history: History) -> np.ndarray:
     st, hs = sN.struct(), history.struct()
     return np.array([lib().orc_history_N(float(a), float(b), ctypes.byref(st), ctypes.byref(hs))
                      for a, b in zip(np.ravel(r), np.ravel(rapidity))])
+
+
+def adaptive_gk21(f, a: float, b: float, epsrel: float = 1e-3, max_intervals: int = 85):
+    """adaptive_gauss_kronrod_1d (S/numerics/integration.py:43-92), rule 21, of a
+    Python callable: (value, error, converged, evaluations)."""
+
+    cb = _GK_FN(lambda x, _ctx: float(f(x)))
+    err, conv, ev = ctypes.c_double(), ctypes.c_int(), ctypes.c_long()
+    v = lib().orc_adaptive_gk21(cb, None, a, b, epsrel, max_intervals, ctypes.byref(err),
+                                ctypes.byref(conv), ctypes.byref(ev))
+    return v, err.value, bool(conv.value), ev.value
+
+
+def k1_adaptive(parent_r: np.ndarray, sN: Spline, rint: int = 85, tint: int = 85, epsrel: float = 1e-3,
+                p: BKParams | None = None, history: History | None = None):
+    """K1 by the nested adaptive rule of rapidity_derivative_lo
+    (S/bk/derivatives.py:43-110): (K1, outer error estimate, integrand evaluations)."""
+
+    p = p or params()
+    parent_r = np.ascontiguousarray(parent_r, dtype=np.float64)
+    out, err = np.empty(parent_r.size), np.empty(parent_r.size)
+    ev = (ctypes.c_long * parent_r.size)()
+    st = sN.struct()
+    hs = history.struct() if history is not None else None
+    lib().orc_k1_adaptive(parent_r.size, _d(parent_r), ctypes.byref(st), rint, tint, epsrel,
+                          ctypes.byref(p), ctypes.byref(hs) if hs is not None else None, _d(out),
+                          _d(err), ev)
+    return out, err, np.array(ev[:], dtype=np.int64)
 
 
 def k1_integrand_history(r, z, theta, sN: Spline, history: History, p: BKParams | None = None):

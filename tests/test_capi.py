@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_lib.LIBRARY))
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
-    assert _lib.lib().nlo_api_version() == 2
+    assert _lib.lib().nlo_api_version() == 3
 
 
 def _gauss_legendre_mp(n: int):
