@@ -351,6 +351,10 @@ typedef struct nlo_bk_batch_desc {
     int32_t grid_blocks;    /* 0 = tune */
     nlo_bk_physics physics;
     nlo_sobol_table sobol;
+    int32_t k1_rule;        /* NLO_K1_FIXED or NLO_K1_ADAPTIVE (as in nlo_bk_evolver_desc) */
+    int32_t k1_rint, k1_tint;
+    int32_t k1_reserved;
+    double k1_epsrel;
 } nlo_bk_batch_desc;
 
 typedef struct nlo_bk_batch nlo_bk_batch;

@@ -121,6 +121,11 @@ class BatchDesc(ctypes.Structure):
         ("grid_blocks", ctypes.c_int32),
         ("physics", Physics),
         ("sobol", SobolTable),
+        ("k1_rule", ctypes.c_int32),
+        ("k1_rint", ctypes.c_int32),
+        ("k1_tint", ctypes.c_int32),
+        ("k1_reserved", ctypes.c_int32),
+        ("k1_epsrel", ctypes.c_double),
     ]
 
 

@@ -70,6 +70,8 @@ class BKOrder(Enum):
 
 PRECISIONS = ("fp32", "fp64", "fp64_stable")
 RK_METHODS = ("auto", "euler", "rk2", "kutta3", "rk4", "adaptive_rk2")
+K1_RULES = ("auto", "fixed", "adaptive")
+MAX_K1_INTERVALS = 100  # device limit of the adaptive K1 interval lists
 
 
 @dataclass(frozen=True, slots=True)
@@ -127,6 +129,14 @@ class BKConfig:
     # K1 Gauss-Legendre orders per log-r panel (ln z) and in v (theta = pi v^2).
     K1_RADIAL_ORDER: int = 8
     K1_ANGULAR_ORDER: int = 24
+    # K1 quadrature: "fixed" (the Gauss-Legendre rule above), "adaptive" (the
+    # nested adaptive Gauss-Kronrod 21 rule of rapidity_derivative_lo,
+    # S/bk/derivatives.py:43-110, to INTACCURACY with at most RINTPOINTS /
+    # THETAINTPOINTS intervals) or "auto": the reference's choice
+    # (S/bk/evolution.py:160-177, :491-533): the fixed rule for float32 where
+    # it is validated (production physics, RPOINTS = 100, RINTPOINTS =
+    # THETAINTPOINTS = 85, INTACCURACY = 0.001), the adaptive rule otherwise.
+    K1_RULE: str = "auto"
 
     def __post_init__(self) -> None:
         if self.NF not in {0, 3, 5}:
@@ -163,6 +173,8 @@ class BKConfig:
             raise ValueError(f"RK_METHOD must be one of {RK_METHODS}")
         if not 1 <= self.K1_RADIAL_ORDER <= 32 or not 1 <= self.K1_ANGULAR_ORDER <= 64:
             raise ValueError("K1 Gauss-Legendre orders must lie in [1, 32] x [1, 64]")
+        if self.K1_RULE not in K1_RULES:
+            raise ValueError(f"K1_RULE must be one of {K1_RULES}")
 
 
 def require_gpu_supported(config: BKConfig) -> None:
@@ -177,6 +189,23 @@ def require_gpu_supported(config: BKConfig) -> None:
         )
 
 
+def fixed_k1_validated(config: BKConfig) -> bool:
+    """_fixed_k1_supported (S/bk/derivatives.py:145-166): the configuration
+    the reference validated its fixed K1 rule on."""
+
+    return (
+        not config.KINEMATICAL_CONSTRAINT
+        and config.RC_LO is RunningCouplingLO.BALITSKY_LO
+        and config.RESUM_RC is ResummationCoupling.RESUM_RC_SMALLEST
+        and config.Order is BKOrder.NLO_RESUM_DLOG_SLOG
+        and config.NF <= 3
+        and config.RPOINTS == 100
+        and config.RINTPOINTS == 85
+        and config.THETAINTPOINTS == 85
+        and config.INTACCURACY == 0.001
+    )
+
+
 def resolve_rk_method(config: BKConfig, dtype) -> str:
     """The stepping method RK_METHOD="auto" stands for (S/bk/evolution.py:108-146,
     :180-187): the reference takes its fixed RK2(3) step only for float32 with
@@ -189,28 +218,42 @@ def resolve_rk_method(config: BKConfig, dtype) -> str:
         return "euler"
     if config.RK_METHOD != "auto":
         return config.RK_METHOD
-    fixed_k1 = (
-        not config.KINEMATICAL_CONSTRAINT
-        and config.RC_LO is RunningCouplingLO.BALITSKY_LO
-        and config.RESUM_RC is ResummationCoupling.RESUM_RC_SMALLEST
-        and config.Order is BKOrder.NLO_RESUM_DLOG_SLOG
-        and config.NF <= 3
-        and config.RPOINTS == 100
-        and config.RINTPOINTS == 85
-        and config.THETAINTPOINTS == 85
-        and config.INTACCURACY == 0.001
-    )
-    if dtype is torch.float32 and config.INTMETHOD_NLO is IntegrationMethod.VEGAS and fixed_k1:
+    if dtype is torch.float32 and config.INTMETHOD_NLO is IntegrationMethod.VEGAS and fixed_k1_validated(config):
         return "kutta3"
     return "adaptive_rk2"
+
+
+def resolve_k1_rule(config: BKConfig, dtype) -> str:
+    """The K1 rule K1_RULE="auto" stands for: on the GPU with float32 the
+    reference switches K1_FIXED on (S/bk/evolution.py:160-177) and takes the
+    fixed rule where it is validated (:491-533); everything else goes through
+    rapidity_derivative_lo's adaptive rule.  The adaptive rule's interval
+    limits must fit the device lists."""
+
+    import torch
+
+    rule = config.K1_RULE
+    if rule == "auto":
+        rule = "fixed" if dtype is torch.float32 and fixed_k1_validated(config) else "adaptive"
+    if rule == "adaptive" and not (
+        1 <= config.RINTPOINTS <= MAX_K1_INTERVALS and 1 <= config.THETAINTPOINTS <= MAX_K1_INTERVALS
+    ):
+        raise NotImplementedError(
+            f"the adaptive K1 rule keeps at most {MAX_K1_INTERVALS} intervals per list on the device; "
+            f"got RINTPOINTS={config.RINTPOINTS}, THETAINTPOINTS={config.THETAINTPOINTS}"
+        )
+    return rule
 
 
 __all__ = [
     "BKConfig",
     "BKOrder",
     "IntegrationMethod",
+    "K1_RULES",
     "PRECISIONS",
     "RK_METHODS",
+    "fixed_k1_validated",
+    "resolve_k1_rule",
     "resolve_rk_method",
     "ResummationCoupling",
     "RunningCouplingLO",

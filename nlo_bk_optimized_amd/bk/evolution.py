@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import torch
 
 from nlo_bk_optimized_amd import device as dev
-from nlo_bk_optimized_amd.bk.config import BKConfig, require_gpu_supported, resolve_rk_method
+from nlo_bk_optimized_amd.bk.config import BKConfig, require_gpu_supported, resolve_k1_rule, resolve_rk_method
 from nlo_bk_optimized_amd.bk.initial_conditions import MV, ICDataFile
 from nlo_bk_optimized_amd.dipole.table import DipoleTable
 
@@ -126,12 +126,15 @@ def prepare(
 
 
 def rhs(setup: BKSetup, N: torch.Tensor, config: BKConfig,
-        history: tuple[torch.Tensor, torch.Tensor] | None = None) -> tuple[torch.Tensor, torch.Tensor]:
+        history: tuple[torch.Tensor, torch.Tensor] | None = None,
+        k1_rule: str = "fixed") -> tuple[torch.Tensor, torch.Tensor]:
     """K1 and K2 + Kf at amplitude N (S/bk/evolution.py:354-488); with the
-    kinematical constraint ``history`` = (rapidities, amplitudes), N its last row."""
+    kinematical constraint ``history`` = (rapidities, amplitudes), N its last row.
+    ``k1_rule``: "fixed" or "adaptive" (resolve_k1_rule gives the reference's choice)."""
 
     t64, t32 = dev.spline_build(setup.r, N, config.FORCE_POSITIVE_N)
-    k1 = dev.bk_k1(setup.r, N, t64, config, history=history if config.KINEMATICAL_CONSTRAINT else None)
+    k1 = dev.bk_k1(setup.r, N, t64, config, history=history if config.KINEMATICAL_CONSTRAINT else None,
+                   rule=k1_rule)
     if config.Order.has_nlo_kernels:
         sums, _, _ = dev.bk_nlo(
             setup.r, N, t64, t32, setup.edges, setup.table, setup.n_samples, config,
@@ -161,6 +164,7 @@ def solve_bk(
     d = _resolve_device(device)
     setup = prepare(initial_condition, config, device=d, seed=seed)
     rk = resolve_rk_method(config, dtype)
+    k1_rule = resolve_k1_rule(config, dtype)
     tolerance = 1e-12 * max(1.0, abs(maxy), abs(config.DE_SOLVER_STEP))
     # output rapidities of fixed steps, as S/bk/evolution.py:108-159 produces them
     steps, y = 0, 0.0
@@ -179,6 +183,7 @@ def solve_bk(
         n_samples=setup.n_samples, precision=config.PRECISION, rk_method=rk,
         step=config.DE_SOLVER_STEP, nlo=config.Order.has_nlo_kernels,
         eps=float(torch.finfo(dtype).eps), history=(history_y, history) if kinematical else None,
+        k1_rule=k1_rule,
     )
     ys = [0.0]
     try:
@@ -217,7 +222,8 @@ def solve_bk(
 _SHARED_FIELDS = ("NC", "NF", "LambdaQCD", "MCINTPOINTS", "DE_SOLVER_STEP", "FIXED_AS", "RC_LO", "RC_NLO",
                   "RESUM_RC", "INTMETHOD_NLO", "FORCE_POSITIVE_N", "SYMMETRIZE_Z_Z2_INTEGRATION", "Order", "KSUB",
                   "KINEMATICAL_CONSTRAINT", "EULER_METHOD", "C2", "MC_BINS", "PRECISION", "RK_METHOD",
-                  "K1_RADIAL_ORDER", "K1_ANGULAR_ORDER")
+                  "K1_RADIAL_ORDER", "K1_ANGULAR_ORDER", "K1_RULE", "RINTPOINTS", "THETAINTPOINTS",
+                  "INTACCURACY")
 
 
 def solve_bk_batch(
@@ -256,6 +262,11 @@ def solve_bk_batch(
     rk = "euler" if c0.EULER_METHOD else (c0.RK_METHOD if c0.RK_METHOD != "auto" else "kutta3")
     if rk == "adaptive_rk2":
         raise NotImplementedError("the batched driver takes fixed-step methods only")
+    rules = {resolve_k1_rule(c, dtype) for c in cfgs}
+    if len(rules) != 1:
+        raise ValueError("the runs resolve to different K1 rules (K1_RULE='auto' with RPOINTS = 100 in "
+                         "some runs only); set K1_RULE to 'fixed' or 'adaptive'")
+    k1_rule = rules.pop()
     d = _resolve_device(device)
     setups = [prepare(ic, c, device=d, seed=seed) for ic, c in zip(ics, cfgs)]
     sizes = [s.r.numel() for s in setups]
@@ -272,7 +283,7 @@ def solve_bk_batch(
             break
     ev = dev.BatchEvolver(r, edges, sizes, setups[0].table, c0, n_samples=setups[0].n_samples,
                           precision=c0.PRECISION, rk_method=rk, step=c0.DE_SOLVER_STEP,
-                          nlo=c0.Order.has_nlo_kernels, spt=spt, grid_blocks=grid_blocks)
+                          nlo=c0.Order.has_nlo_kernels, spt=spt, grid_blocks=grid_blocks, k1_rule=k1_rule)
     try:
         history = torch.empty((steps + 1, r.numel()), dtype=torch.float64, device=d)
         history[0].copy_(N0)
@@ -312,7 +323,7 @@ def compute_dndy(
     d = _resolve_device(device)
     setup = prepare(initial_condition, config, device=d, seed=seed)
     history = (torch.zeros(1, dtype=torch.float64, device=d), setup.N0.unsqueeze(0).contiguous())
-    k1, k2kf = rhs(setup, setup.N0, config, history=history)
+    k1, k2kf = rhs(setup, setup.N0, config, history=history, k1_rule=resolve_k1_rule(config, dtype))
     keep = torch.arange(setup.r.numel(), device=d) > 0
     keep = keep & (setup.N0 <= 0.99999)
     return DNDYResult(

@@ -137,7 +137,11 @@ int enqueue_batch_rhs(nlo_bk_batch* bt, const double* in, double* out, hipStream
     const nlo_bk_batch_desc& d = bt->d;
     int rc = launch_spline_build_batch(bt->b, bt->r, in, d.force_positive, bt->t64, bt->t32, s);
     if (rc != NLO_OK) return rc;
-    rc = launch_k1_batch(bt->b, bt->r, in, bt->t64, bt->c, d.nr, d.nt, bt->k1, s);
+    if (d.k1_rule == NLO_K1_ADAPTIVE)
+        rc = launch_k1_adaptive_batch(bt->b, bt->r, in, bt->t64, bt->c, d.k1_rint, d.k1_tint, d.k1_epsrel,
+                                      bt->k1, s);
+    else
+        rc = launch_k1_batch(bt->b, bt->r, in, bt->t64, bt->c, d.nr, d.nt, bt->k1, s);
     if (rc != NLO_OK) return rc;
     if (d.nlo) {
         rc = launch_nlo_batch(bt->b, bt->r, in, bt->t64, bt->t32, bt->edges, d.bins, d.sobol, d.first_index,
@@ -303,6 +307,13 @@ int nlo_bk_batch_create(const nlo_bk_batch_desc* desc, const int32_t* h_R, const
                 "nlo_bk_batch_create: samples per lane must be 0 (tune), 4, 8 or 16");
     NLO_REQUIRE(d.grid_blocks >= 0, "nlo_bk_batch_create: grid_blocks must be >= 0");
     NLO_REQUIRE(d.sobol.dim == 4, "nlo_bk_batch_create: 4-dimensional Sobol table required");
+    NLO_REQUIRE(d.k1_rule == NLO_K1_FIXED || d.k1_rule == NLO_K1_ADAPTIVE,
+                "nlo_bk_batch_create: unknown k1_rule %d", d.k1_rule);
+    if (d.k1_rule == NLO_K1_ADAPTIVE)
+        NLO_REQUIRE(d.k1_rint >= 1 && d.k1_rint <= nlo::kMaxGkIntervals && d.k1_tint >= 1 &&
+                        d.k1_tint <= nlo::kMaxGkIntervals && d.k1_epsrel > 0.0,
+                    "nlo_bk_batch_create: adaptive K1 needs interval limits in [1, %d] and epsrel > 0",
+                    nlo::kMaxGkIntervals);
     NLO_REQUIRE(d_r && d_edges && d_workspace, "nlo_bk_batch_create: null pointer");
     int rc = nlo::validate_physics(&d.physics);
     if (rc != NLO_OK) return rc;

@@ -68,6 +68,9 @@ struct BatchView {
 
 int launch_spline_build_batch(const BatchView& b, const double* r, const double* N, int force_positive,
                               double* t64, float* t32, hipStream_t stream);
+int launch_k1_adaptive_batch(const BatchView& b, const double* r, const double* N, const double* t64,
+                             const BkConsts& c, int rint, int tint, double epsrel, double* k1,
+                             hipStream_t stream);
 int launch_k1_batch(const BatchView& b, const double* r, const double* N, const double* t64,
                     const BkConsts& c, int nr, int nt, double* k1, hipStream_t stream);
 // Persistent K2 + Kf launch: `grid` blocks take (parent, chunk) items from

@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(kK1Threads) k1_kernel(int R, const double* __r
 // 21 (first interval) or 42 (two halves) theta nodes of each bisection step
 // while every lane runs the same list bookkeeping on LDS.  Interval lists
 // stay in order and sums run in list order, as in the reference.
-constexpr int kK1aWaves = 16;
+constexpr int kK1aWaves = 8;
 
 // QUADPACK's 21-point Kronrod extension of the 10-point Gauss rule: nodes
 // kGkX[0..9] (kGkX[1, 3, .., 9] are the Gauss nodes), centre node last.
@@ -468,35 +468,43 @@ __device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const LdsSp
         else
             return k1_theta_term(P, Z, sn * sn, s, c);
     };
-    if (lane < 21) v[lane] = f(gk_node(0.0, kPi, lane));
-    wave_sync();
-    double e, r;
-    gk_from_values(0.0, kPi, v, e, r);
-    wave_sync();
-    if (lane == 0) {
-        L.lo[0] = 0.0;
-        L.hi[0] = kPi;
-        L.est[0] = e;
-        L.err[0] = r;
-    }
-    evals += 21;
-    int count = 1;
-    double value = e;
+    // First pass: the 21 nodes of [0, pi]; then per bisection of [a, b] the
+    // 21 nodes of [a, mid] (lanes 0-20) and of [mid, b] (lanes 21-41).
+    int count = 0, split = 0;
+    double a = 0.0, b = kPi, mid = kPi, value = 0.0;
     for (;;) {
+        if (lane < (count == 0 ? 21 : 42)) {
+            const double x = count == 0 ? gk_node(0.0, kPi, lane)
+                                        : (lane < 21 ? gk_node(a, mid, lane) : gk_node(mid, b, lane - 21));
+            v[lane] = f(x);
+        }
+        wave_sync();
+        if (count == 0) {
+            double e, r;
+            gk_from_values(0.0, kPi, v, e, r);
+            wave_sync();
+            if (lane == 0) {
+                L.lo[0] = 0.0;
+                L.hi[0] = kPi;
+                L.est[0] = e;
+                L.err[0] = r;
+            }
+            evals += 21;
+        } else {
+            double e0, r0, e1, r1;
+            gk_from_values(a, mid, v, e0, r0);
+            gk_from_values(mid, b, v + 21, e1, r1);
+            gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
+            evals += 42;
+        }
+        ++count;
         wave_sync();
         double error;
-        int split;
         gk_scan(L, count, value, error, split);
         if (count >= tint || error <= epsrel * fabs(value)) break;
-        const double a = L.lo[split], b = L.hi[split], mid = (a + b) / 2.0;
-        if (lane < 42) v[lane] = f(lane < 21 ? gk_node(a, mid, lane) : gk_node(mid, b, lane - 21));
-        wave_sync();
-        double e0, r0, e1, r1;
-        gk_from_values(a, mid, v, e0, r0);
-        gk_from_values(mid, b, v + 21, e1, r1);
-        gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
-        ++count;
-        evals += 42;
+        a = L.lo[split];
+        b = L.hi[split];
+        mid = (a + b) / 2.0;
     }
     wave_sync();
     return value;
@@ -510,16 +518,18 @@ __host__ __device__ inline size_t k1a_lds_doubles(int R, int cap) {
            (size_t)kK1aWaves * (4 * (size_t)cap + 42);
 }
 
+// Parent p of one evolution (arrays of that evolution).
 template <bool ANY>
-__global__ void __launch_bounds__(kK1aWaves * 64)
-    k1_adaptive_kernel(int R, const double* __restrict__ r, const double* __restrict__ N,
-                       const double* __restrict__ t64, BkConsts c, K1History H, int rint, int tint,
-                       int cap, double epsrel, double* __restrict__ k1, double* __restrict__ k1_err,
-                       long long* __restrict__ k1_evals) {
+__device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict__ r,
+                                                 const double* __restrict__ N,
+                                                 const double* __restrict__ t64, const BkConsts& c,
+                                                 const K1History& H, int rint, int tint, int cap,
+                                                 double epsrel, int p, double* __restrict__ k1,
+                                                 double* __restrict__ k1_err,
+                                                 long long* __restrict__ k1_evals) {
     extern __shared__ double sh[];
     __shared__ int task_next, n_tasks, o_done;
     __shared__ unsigned long long ev_total;
-    const int p = blockIdx.x;
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
         if (threadIdx.x == 0) {
             k1[p] = 0.0;
@@ -620,6 +630,15 @@ __global__ void __launch_bounds__(kK1aWaves * 64)
         if (k1_err) k1_err[p] = error;
         if (k1_evals) k1_evals[p] = (long long)ev_total;
     }
+}
+
+template <bool ANY>
+__global__ void __launch_bounds__(kK1aWaves * 64)
+    k1_adaptive_kernel(int R, const double* __restrict__ r, const double* __restrict__ N,
+                       const double* __restrict__ t64, BkConsts c, K1History H, int rint, int tint,
+                       int cap, double epsrel, double* __restrict__ k1, double* __restrict__ k1_err,
+                       long long* __restrict__ k1_evals) {
+    k1_adaptive_body<ANY>(R, r, N, t64, c, H, rint, tint, cap, epsrel, blockIdx.x, k1, k1_err, k1_evals);
 }
 
 // ------------------------------------------------------------- K2 + Kf
@@ -1099,6 +1118,20 @@ __global__ void __launch_bounds__(kK1Threads) k1_batch_kernel(const BatchRun* __
                  blockIdx.x - run.node_off, k1 + run.node_off);
 }
 
+// The adaptive rule, one block per node of every run.
+template <bool ANY>
+__global__ void __launch_bounds__(kK1aWaves * 64)
+    k1_adaptive_batch_kernel(const BatchRun* __restrict__ runs, const int32_t* __restrict__ node_run,
+                             const double* __restrict__ r, const double* __restrict__ N,
+                             const double* __restrict__ t64, BkConsts c, int rint, int tint, int cap,
+                             double epsrel, double* __restrict__ k1) {
+    const BatchRun run = runs[node_run[blockIdx.x]];
+    const K1History H{};
+    k1_adaptive_body<ANY>(run.R, r + run.node_off, N + run.node_off, t64 + run.spline_off, c, H, rint,
+                          tint, cap, epsrel, blockIdx.x - run.node_off, k1 + run.node_off, nullptr,
+                          nullptr);
+}
+
 struct BatchNlo {
     const BatchRun* runs;
     const int32_t* node_run;
@@ -1237,6 +1270,8 @@ int init_kernel_attributes() {
             (const void*)spline_build_batch_kernel,
             (const void*)k1_batch_kernel<false>,
             (const void*)k1_batch_kernel<true>,
+            (const void*)k1_adaptive_batch_kernel<false>,
+            (const void*)k1_adaptive_batch_kernel<true>,
             (const void*)nlo_batch_kernel<NLO_PREC_FP32, false>,
             (const void*)nlo_batch_kernel<NLO_PREC_FP32, true>,
             (const void*)nlo_batch_kernel<NLO_PREC_FP64, false>,
@@ -1461,6 +1496,29 @@ int launch_k1_batch(const BatchView& b, const double* r, const double* N, const 
     else
         hipLaunchKernelGGL(k1_batch_kernel<true>, dim3(b.nodes), dim3(kK1Threads), smem, stream, b.runs_d,
                            b.node_run, r, N, t64, c, g, k1);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int launch_k1_adaptive_batch(const BatchView& b, const double* r, const double* N, const double* t64,
+                             const BkConsts& c, int rint, int tint, double epsrel, double* k1,
+                             hipStream_t stream) {
+    const int cap = rint > tint ? rint : tint;
+    const size_t smem = k1a_lds_doubles(b.Rmax, cap) * sizeof(double);
+    int rc = init_kernel_attributes();
+    if (rc != NLO_OK) return rc;
+    int dev = 0, max_lds = 0;
+    NLO_HIP_CHECK(hipGetDevice(&dev));
+    NLO_HIP_CHECK(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    NLO_REQUIRE(smem + 64 <= (size_t)max_lds,
+                "adaptive K1: R=%d with %d intervals needs %zu bytes of LDS (device: %d)", b.Rmax, cap,
+                smem, max_lds);
+    if (c.production)
+        hipLaunchKernelGGL(k1_adaptive_batch_kernel<false>, dim3(b.nodes), dim3(kK1aWaves * 64), smem,
+                           stream, b.runs_d, b.node_run, r, N, t64, c, rint, tint, cap, epsrel, k1);
+    else
+        hipLaunchKernelGGL(k1_adaptive_batch_kernel<true>, dim3(b.nodes), dim3(kK1aWaves * 64), smem,
+                           stream, b.runs_d, b.node_run, r, N, t64, c, rint, tint, cap, epsrel, k1);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }

@@ -171,22 +171,10 @@ def spline_build(r: torch.Tensor, N: torch.Tensor, force_positive: bool = True) 
     return t64, t32
 
 
-def bk_k1(r: torch.Tensor, N: torch.Tensor, t64: torch.Tensor, config,
-          history: tuple[torch.Tensor, torch.Tensor] | None = None) -> torch.Tensor:
-    """K1 for every r node (0 for index 0 and N > 0.99999).  ``history``
-    (rapidities [rows], amplitudes [rows, R], the last row equal to N) turns
-    the kinematical constraint on at the last row's rapidity."""
+K1_RULE_CODES = {"fixed": _lib.NLO_K1_FIXED, "adaptive": _lib.NLO_K1_ADAPTIVE}
 
-    R = r.numel()
-    out = torch.empty(R, dtype=torch.float64, device=r.device)
-    ph = physics(config)
-    if history is None:
-        check(
-            lib().nlo_bk_k1(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
-                            config.K1_ANGULAR_ORDER, ptr(out), stream_handle(r.device)),
-            "nlo_bk_k1",
-        )
-        return out
+
+def _history_rows(history, R: int, device) -> torch.Tensor:
     hy, hN = history
     require_cuda(hy, torch.float64, "history rapidities")
     require_cuda(hN, torch.float64, "history amplitudes")
@@ -195,7 +183,55 @@ def bk_k1(r: torch.Tensor, N: torch.Tensor, t64: torch.Tensor, config,
         raise ValueError("history must be rapidities [rows] and amplitudes [rows, R]")
     if not bool((hy[1:] > hy[:-1]).all()):
         raise ValueError("history rapidities must increase")
-    d_rows = torch.tensor([rows], dtype=torch.int32, device=r.device)
+    return torch.tensor([rows], dtype=torch.int32, device=device)
+
+
+def bk_k1(r: torch.Tensor, N: torch.Tensor, t64: torch.Tensor, config,
+          history: tuple[torch.Tensor, torch.Tensor] | None = None, rule: str = "fixed",
+          stats: dict | None = None) -> torch.Tensor:
+    """K1 for every r node (0 for index 0 and N > 0.99999).  ``history``
+    (rapidities [rows], amplitudes [rows, R], the last row equal to N) turns
+    the kinematical constraint on at the last row's rapidity.  ``rule``:
+    "fixed" (Gauss-Legendre per log-r panel) or "adaptive" (the nested
+    adaptive Gauss-Kronrod rule with RINTPOINTS, THETAINTPOINTS, INTACCURACY);
+    for the adaptive rule ``stats`` (a dict) receives the per-node error
+    estimates and integrand evaluations."""
+
+    R = r.numel()
+    out = torch.empty(R, dtype=torch.float64, device=r.device)
+    ph = physics(config)
+    if rule not in K1_RULE_CODES:
+        raise ValueError(f"unknown K1 rule {rule!r}")
+    if rule == "adaptive":
+        err = torch.empty(R, dtype=torch.float64, device=r.device)
+        evals = torch.empty(R, dtype=torch.int64, device=r.device)
+        hy = hN = d_rows = None
+        cap = 0
+        if history is not None:
+            d_rows = _history_rows(history, R, r.device)
+            hy, hN = history
+            cap = hy.numel()
+        check(
+            lib().nlo_bk_k1_adaptive(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.RINTPOINTS,
+                                     config.THETAINTPOINTS, float(config.INTACCURACY), ptr(hy), ptr(hN),
+                                     ptr(d_rows), cap, ptr(out), ptr(err), ptr(evals),
+                                     stream_handle(r.device)),
+            "nlo_bk_k1_adaptive",
+        )
+        if stats is not None:
+            stats["error"] = err
+            stats["evaluations"] = evals
+        return out
+    if history is None:
+        check(
+            lib().nlo_bk_k1(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
+                            config.K1_ANGULAR_ORDER, ptr(out), stream_handle(r.device)),
+            "nlo_bk_k1",
+        )
+        return out
+    d_rows = _history_rows(history, R, r.device)
+    hy, hN = history
+    rows = hy.numel()
     check(
         lib().nlo_bk_k1_history(R, ptr(r), ptr(N), ptr(t64), ctypes.byref(ph), config.K1_RADIAL_ORDER,
                                 config.K1_ANGULAR_ORDER, ptr(hy), ptr(hN), ptr(d_rows), rows, ptr(out),
@@ -263,6 +299,15 @@ RK_CODES = {"euler": _lib.NLO_RK_EULER, "rk2": _lib.NLO_RK_MIDPOINT, "kutta3": _
             "rk4": _lib.NLO_RK_CLASSIC4, "adaptive_rk2": _lib.NLO_RK_ADAPTIVE2}
 
 
+def _set_k1_rule(d, config, rule: str) -> None:
+    if rule not in K1_RULE_CODES:
+        raise ValueError(f"unknown K1 rule {rule!r}")
+    d.k1_rule = K1_RULE_CODES[rule]
+    d.k1_rint = int(config.RINTPOINTS)
+    d.k1_tint = int(config.THETAINTPOINTS)
+    d.k1_epsrel = float(config.INTACCURACY)
+
+
 class Evolver:
     """One captured hipGraph per RK step (C ABI ``nlo_bk_evolver_*``)."""
 
@@ -281,10 +326,12 @@ class Evolver:
         first_index: int = 0,
         eps: float = float(torch.finfo(torch.float64).eps),
         history: tuple[torch.Tensor, torch.Tensor] | None = None,
+        k1_rule: str = "fixed",
     ) -> None:
         """``history``: (rapidities [capacity], amplitudes [capacity, R]) with
         row 0 filled, for the kinematical constraint (Euler only); each step
-        appends a row.  ``eps``: the step-underflow epsilon of adaptive RK2."""
+        appends a row.  ``eps``: the step-underflow epsilon of adaptive RK2.
+        ``k1_rule``: "fixed" or "adaptive" (see bk_k1)."""
 
         require_cuda(r, torch.float64, "r")
         require_cuda(edges, torch.float64, "edges")
@@ -308,6 +355,7 @@ class Evolver:
         d.abserr = float(config.DE_SOLVER_ABSERR)
         d.relerr = float(config.DE_SOLVER_RELERR)
         d.eps = float(eps)
+        _set_k1_rule(d, config, k1_rule)
         if history is not None:
             hy, hN = history
             require_cuda(hy, torch.float64, "history rapidities")
@@ -382,7 +430,7 @@ class BatchEvolver:
 
     def __init__(self, r: torch.Tensor, edges: torch.Tensor, sizes, table: _lib.SobolTable, config, *,
                  n_samples: int, precision: str, rk_method: str, step: float, nlo: bool = True,
-                 first_index: int = 0, spt: int = 0, grid_blocks: int = 0) -> None:
+                 first_index: int = 0, spt: int = 0, grid_blocks: int = 0, k1_rule: str = "fixed") -> None:
         require_cuda(r, torch.float64, "r")
         require_cuda(edges, torch.float64, "edges")
         sizes = [int(v) for v in sizes]
@@ -407,6 +455,7 @@ class BatchEvolver:
         d.grid_blocks = int(grid_blocks)
         d.physics = physics(config)
         d.sobol = table
+        _set_k1_rule(d, config, k1_rule)
         self.desc = d
         self._sizes = (ctypes.c_int32 * len(sizes))(*sizes)
         nbytes = lib().nlo_bk_batch_workspace(ctypes.byref(d), self._sizes)

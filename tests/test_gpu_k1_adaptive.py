@@ -1,0 +1,189 @@
+"""The adaptive K1 rule on the device (nlo_bk_k1_adaptive, the evolvers'
+NLO_K1_ADAPTIVE) against the oracle's restatement of rapidity_derivative_lo
+(S/bk/derivatives.py:43-110).
+
+Both run the same FP64 algorithm: GK21 on ordered interval lists, bisection
+of the first interval of largest error, sums in list order.  Their integrands
+agree to rounding, so nearly every bisection decision agrees: where the
+integrand evaluation counts are equal (most parents; a systematic difference
+in the algorithm would change nearly all of them) K1
+agrees within 1e-10 relative (or 1e-13 of its integrated rounding scale where
+K1 cancels) and the error estimates within 1 % (they are differences of
+nearly equal Kronrod and Gauss sums); where a decision near the
+tolerance went the other way (the integrand's rounding differs most near
+X -> 0, where the oracle forms X^2 from cos theta) both results lie within
+their error estimates of each other.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.bk_helpers import workload
+from tests.test_gpu_options import _history, configs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from nlo_bk_optimized_amd import device
+
+    return device
+
+
+def cuda(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
+
+
+def _compare(dev, cfg, p, r, N, history=None):
+    rt, Nt = cuda(r), cuda(N)
+    t64, _ = dev.spline_build(rt, Nt)
+    stats = {}
+    hist = None if history is None else (cuda(history[0]), cuda(history[1]))
+    k1 = dev.bk_k1(rt, Nt, t64, cfg, rule="adaptive", stats=stats, history=hist).cpu().numpy()
+    evals = stats["evaluations"].cpu().numpy()
+    err = stats["error"].cpu().numpy()
+    sN, _ = O.evolution_splines(r, N)
+    active = (np.arange(r.size) >= 1) & (N <= 0.99999)
+    oh = None if history is None else O.History(r, history[0], history[1], float(history[0][-1]))
+    ref, ref_err, ref_ev = (np.zeros(r.size), np.zeros(r.size), np.zeros(r.size, dtype=np.int64))
+    ref[active], ref_err[active], ref_ev[active] = O.k1_adaptive(
+        r[active], sN, rint=cfg.RINTPOINTS, tint=cfg.THETAINTPOINTS, epsrel=cfg.INTACCURACY, p=p, history=oh)
+    mag = np.zeros(r.size)
+    mag[active] = O.k1_fixed(r[active], sN, p=p, magnitude=True)
+    print(f"R={r.size}: evaluations per parent {evals[active].min()}..{evals[active].max()}, "
+          f"max rel diff {np.max(np.abs(k1 - ref)[active] / np.abs(ref[active])):.2e}", flush=True)
+    assert k1[0] == 0.0 and np.all(k1[~active] == 0.0)
+    same = active & (evals == ref_ev)
+    print(f"same decisions for {same.sum()} of {active.sum()} parents", flush=True)
+    assert same.sum() >= 0.6 * active.sum()
+    assert np.all((np.abs(k1 - ref) <= 1e-10 * np.abs(ref) + 1e-13 * mag)[same]), (k1, ref)
+    # |K - G| is a difference of nearly equal sums: its rounding is ~eps resabs
+    assert np.all((np.abs(err - ref_err) <= 1e-2 * ref_err + 1e-13 * mag)[same])
+    assert np.all((np.abs(k1 - ref) <= err + ref_err + 1e-13 * mag)[active])
+    return k1, mag
+
+
+@pytest.mark.parametrize("R", [24, 101])
+def test_k1_adaptive_matches_oracle(dev, R):
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    print(f"adaptive K1 production R={R}", flush=True)
+    r, N = workload(R=R)
+    cfg = BKConfig(MINR=1e-6)
+    k1, mag = _compare(dev, cfg, O.params(minr=1e-6), r, N)
+    # the fixed rule integrates the same function: agreement to INTACCURACY
+    # (relative to the integrated |integrand| where K1 cancels)
+    rt, Nt = cuda(r), cuda(N)
+    t64, _ = dev.spline_build(rt, Nt)
+    fixed = dev.bk_k1(rt, Nt, t64, cfg).cpu().numpy()
+    active = (np.arange(r.size) >= 1) & (N <= 0.99999)
+    assert np.all((np.abs(k1 - fixed) <= 1e-3 * np.abs(fixed) + 1e-6 * mag)[active])
+
+
+def test_k1_adaptive_interval_limits_and_tolerance(dev):
+    """Tighter limits: fewer intervals (RINTPOINTS = THETAINTPOINTS = 7) and a
+    tighter tolerance (INTACCURACY = 1e-5), still decision for decision."""
+
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    r, N = workload(R=16)
+    for kw in ({"RINTPOINTS": 7, "THETAINTPOINTS": 7}, {"INTACCURACY": 1e-5}):
+        print(f"adaptive K1 {kw}", flush=True)
+        _compare(dev, BKConfig(MINR=1e-6, **kw), O.params(minr=1e-6), r, N)
+
+
+@pytest.mark.parametrize("name", ["parent_dls", "beuf_dls", "nf5_dls", "nf0_lo"])
+def test_k1_adaptive_other_physics_matches_oracle(dev, name):
+    print(f"adaptive K1 {name}", flush=True)
+    cfg, op = configs(name)
+    r, N = workload(R=20)
+    _compare(dev, cfg, op(), r, N)
+
+
+def test_k1_adaptive_kinematical_matches_oracle(dev):
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    print("adaptive K1 kinematical", flush=True)
+    cfg = BKConfig(MINR=1e-6, KINEMATICAL_CONSTRAINT=True, EULER_METHOD=True)
+    r, _ = workload(R=20)
+    hy, hN = _history(r)
+    _compare(dev, cfg, O.params(minr=1e-6), r, hN[-1], history=(hy, hN))
+
+
+def test_evolver_adaptive_k1_step(dev):
+    """One Euler step of the LO evolver with the adaptive rule: N + h K1 with
+    K1 from nlo_bk_k1_adaptive (the same kernel), clamped to [0, 1]."""
+
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+    from nlo_bk_optimized_amd.bk.evolution import prepare
+
+    print("evolver adaptive K1 step", flush=True)
+    cfg = BKConfig(RPOINTS=40, MINR=1e-6, MAXR=1e2, MCINTPOINTS=1 << 10, Order=_order("LO"),
+                   RK_METHOD="euler", DE_SOLVER_STEP=0.2, K1_RULE="adaptive")
+    setup = prepare(_mv(), cfg, device=torch.device("cuda"), seed=0)
+    t64, _ = dev.spline_build(setup.r, setup.N0)
+    k1 = dev.bk_k1(setup.r, setup.N0, t64, cfg, rule="adaptive")
+    ev = dev.Evolver(setup.r, setup.edges, setup.table, cfg, n_samples=setup.n_samples, precision="fp64",
+                     rk_method="euler", step=0.2, nlo=False, k1_rule="adaptive")
+    try:
+        ev.state.copy_(setup.N0)
+        ev.step()
+        got = ev.state.clone()
+    finally:
+        ev.close()
+    want = torch.clamp(setup.N0 + 0.2 * k1, 0.0, 1.0)
+    assert torch.allclose(got, want, rtol=1e-14, atol=1e-300)
+
+
+def _order(name):
+    from nlo_bk_optimized_amd.bk.config import BKOrder
+
+    return BKOrder[name]
+
+
+def _mv():
+    from nlo_bk_optimized_amd import MV
+
+    return MV(qs0sqr=0.2, x0=0.01, ec=1.0, lambdaqcd=0.241, gamma=1.0)
+
+
+def test_solve_bk_auto_rule_follows_the_reference(dev):
+    """K1_RULE="auto": the fixed rule for float32 at the validated
+    configuration (RPOINTS = 100), the adaptive rule for float64; explicit
+    rules give the same tables."""
+
+    from nlo_bk_optimized_amd import solve_bk
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    print("solve_bk K1_RULE auto", flush=True)
+    base = dict(MINR=1e-6, MAXR=30.0, MCINTPOINTS=1 << 10, DE_SOLVER_STEP=0.1, RK_METHOD="rk2")
+    auto32 = solve_bk(_mv(), 0.2, BKConfig(**base), dtype=torch.float32, seed=0)
+    fixed = solve_bk(_mv(), 0.2, BKConfig(K1_RULE="fixed", **base), dtype=torch.float32, seed=0)
+    assert torch.equal(auto32.N, fixed.N)
+    auto64 = solve_bk(_mv(), 0.2, BKConfig(**base), dtype=torch.float64, seed=0)
+    adaptive = solve_bk(_mv(), 0.2, BKConfig(K1_RULE="adaptive", **base), dtype=torch.float64, seed=0)
+    assert torch.equal(auto64.N, adaptive.N)
+    d = (auto64.N - auto32.N.double()).abs().max().item()
+    print(f"fixed vs adaptive K1 after Y = 0.2: max |dN| = {d:.2e}", flush=True)
+    assert 0.0 < d < 1e-4
+
+
+def test_batch_adaptive_k1_matches_single_runs(dev):
+    from nlo_bk_optimized_amd import MV, solve_bk, solve_bk_batch
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    print("batch adaptive K1", flush=True)
+    ics = [MV(qs0sqr=q, x0=0.01, ec=1.0, lambdaqcd=0.241, gamma=1.0) for q in (0.2, 0.35)]
+    cfgs = [BKConfig(RPOINTS=R, MINR=1e-6, MAXR=1e2, MCINTPOINTS=1 << 10, DE_SOLVER_STEP=0.1,
+                     RK_METHOD="rk2", K1_RULE="adaptive") for R in (30, 45)]
+    tables = solve_bk_batch(ics, 0.2, cfgs, dtype=torch.float64, seed=0)
+    for ic, cfg, t in zip(ics, cfgs, tables):
+        single = solve_bk(ic, 0.2, cfg, dtype=torch.float64, seed=0)
+        assert torch.equal(single.N, t.N)
