@@ -76,7 +76,9 @@ def parse() -> argparse.Namespace:
     p.add_argument("--dis-points", type=int, default=32,
                    help="(Q^2, x) points of the NLO DIS batch (512 = the full 32 x 16 grid; 0 skips it)")
     p.add_argument("--kernel-probe", action="store_true",
-                   help="only run the FP32 K2+Kf launch and a 256 MiB device copy (for counter runs)")
+                   help="only run the K2+Kf launch and a 256 MiB device copy (for counter runs)")
+    p.add_argument("--probe-precision", default="fp32", choices=("fp32", "fp64", "fp64_stable"),
+                   help="integrand precision of the --kernel-probe launches")
     return p.parse_args()
 
 
@@ -142,17 +144,18 @@ def main() -> int:
     if args.kernel_probe:
         # Counter runs (rocprofv3 --pmc) need only the dominant kernel and a copy
         # of known size whose FETCH_SIZE / WRITE_SIZE calibrate the counters.
-        config = BKConfig(PRECISION="fp32", **base)
+        config = BKConfig(PRECISION=args.probe_precision, **base)
         setup = evo.prepare(ic, config, device=device, seed=0)
         t64, t32 = dev.spline_build(setup.r, setup.N0)
         for _ in range(args.kernel_launches):
-            dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32")
+            dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config,
+                       precision=args.probe_precision)
         src = torch.ones(1 << 25, dtype=torch.float64, device=device)  # 256 MiB
         dst = torch.empty_like(src)
         for _ in range(2):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": "nlo_kernel<FP32>", "launches": args.kernel_launches,
+        print(json.dumps({"probe": f"nlo_kernel<{args.probe_precision}>", "launches": args.kernel_launches,
                           "copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}), flush=True)
         return 0
 
@@ -181,7 +184,33 @@ def main() -> int:
     achieved = FLOPS_PER_SAMPLE * active0 * n / (launch_ms * 1e-3) / 1e12
     log(f"nlo fp32 launch: {launch_ms:.3f} ms for {active0} parents x {n} samples -> {achieved:.2f} TFLOP/s")
 
-    extra: dict = {}
+    # K1 per right-hand side on the initial condition: the fixed 8 x 24 rule
+    # the evolution above uses, and the nested adaptive Gauss-Kronrod rule the
+    # reference takes at this grid size (RPOINTS != 100).
+    k1_ms, k1_stats = {}, {}
+    for rule in ("fixed", "adaptive"):
+        dev.bk_k1(setup.r, setup.N0, t64, config, rule=rule)
+        start.record(stream)
+        for _ in range(args.kernel_launches):
+            dev.bk_k1(setup.r, setup.N0, t64, config, rule=rule, stats=k1_stats if rule == "adaptive" else None)
+        stop.record(stream)
+        stop.synchronize()
+        k1_ms[rule] = start.elapsed_time(stop) / args.kernel_launches
+    fixed_k1 = dev.bk_k1(setup.r, setup.N0, t64, config, rule="fixed")
+    adaptive_k1 = dev.bk_k1(setup.r, setup.N0, t64, config, rule="adaptive")
+    act = (torch.arange(setup.r.numel(), device=device) >= 1) & (setup.N0 <= 0.99999)
+    k1_rel = float(((adaptive_k1 - fixed_k1).abs() / fixed_k1.abs().clamp_min(1e-300))[act].max())
+    k1_evals = k1_stats["evaluations"][act].double()
+    log(f"K1 per RHS: fixed {k1_ms['fixed']:.3f} ms ({active0} x {(setup.r.numel() - 1) * 8 * 24} nodes), "
+        f"adaptive {k1_ms['adaptive']:.3f} ms (mean {float(k1_evals.mean()):.0f} evaluations per parent), "
+        f"max rel difference {k1_rel:.2e}")
+
+    extra: dict = {
+        "k1_fixed_ms_per_rhs": k1_ms["fixed"],
+        "k1_adaptive_ms_per_rhs": k1_ms["adaptive"],
+        "k1_adaptive_mean_evaluations_per_parent": float(k1_evals.mean()),
+        "k1_fixed_vs_adaptive_max_rel": k1_rel,
+    }
     if not args.no_fp64:
         log("fp64 (reference K2/Kf formula) evolution")
         fp64 = evolve("fp64", args.steps, 0)
@@ -189,14 +218,14 @@ def main() -> int:
         log("fp64 (FP32 path's formula) evolution")
         fp64s = evolve("fp64_stable", args.steps, 0)
         log(f"fp64_stable: {fp64s['seconds']:.3f} s")
-        extra = {
+        extra.update({
             "fp64_samples_per_s": fp64["evaluations"] * n / fp64["seconds"],
             "fp64_evolution_s": fp64["seconds"],
             "speedup_fp32_vs_fp64": fp64["seconds"] / fp32["seconds"],
             "max_abs_dN_fp32_vs_fp64_same_formula": float((fp32["N"] - fp64s["N"]).abs().max()),
             "max_abs_dN_fp32_vs_fp64_reference_formula": float((fp32["N"] - fp64["N"]).abs().max()),
             "max_abs_dN_fp64_formulas": float((fp64s["N"] - fp64["N"]).abs().max()),
-        }
+        })
 
     if not args.no_dis:
         # DIS: one VEGAS pass (in-kernel Sobol, map, integrand, FP64 sums and
@@ -541,6 +570,7 @@ def main() -> int:
             "r_points": args.rpoints + 1,
             "samples_per_parent": n,
             "rk": "midpoint RK2",
+            "k1": "fixed 8x24 Gauss-Legendre per log-r panel",
             "dY": 0.1,
             "accumulate": "f64",
         },
