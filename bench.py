@@ -34,7 +34,7 @@ METRIC = "NLO-BK evolution wall time + integrand samples/s, FP32 vs FP64 at fixe
 # cancellation-free integrand and the accumulation), counted on the source
 # with add/sub/mul/div/sqrt/exp/log/log1p/expm1/sin/cos = 1, fma = 2,
 # compares and selects = 0.
-FLOPS_PER_SAMPLE = 263
+FLOPS_PER_SAMPLE = 253
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 K2 + Kf kernel on this workload, from the
