@@ -353,7 +353,10 @@ __device__ __forceinline__ void gk_from_values(double a, double b, const double*
     const double resabs = half * (sabs + kGkWCenter * fabs(fc));
     const double resasc = half * (sasc + kGkWCenter * fabs(fc - mean));
     double e = half * fabs(kron - gauss);
-    const double scale = pow(200.0 * e / (resasc > 0.0 ? resasc : 1.0), 1.5);
+    // (200 |K - G| / resasc)^1.5 as q sqrt(q): within an ulp of pow(q, 1.5)
+    // at the cost of one square root
+    const double q = 200.0 * e / (resasc > 0.0 ? resasc : 1.0);
+    const double scale = q * sqrt(q);
     if (resasc > 0.0 && e > 0.0) e = resasc * fmin(scale, 1.0);
     est = half * kron;
     err = fmax(e, 50.0 * 2.220446049250313e-16 * resabs);
@@ -445,12 +448,10 @@ __device__ __forceinline__ void gk_insert_serial(const GkList& L, int count, int
     L.err[split + 1] = r1;
 }
 
-// The theta integral at one daughter size z for parent P, by one wave.
+// The per-daughter terms of the K1 integrand at z = e^{log_z} (K1Z).
 template <bool ANY>
-__device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const LdsSpline<double>& s,
-                                    const BkConsts& c, const K1History& H, int rows, double rapidity,
-                                    int tint, double epsrel, const GkList& L, double* v, int lane,
-                                    long long& evals) {
+__device__ __forceinline__ K1Z k1_daughter(const K1Parent& P, double log_z, const LdsSpline<double>& s,
+                                           const BkConsts& c) {
     K1Z Z;
     Z.log_z = log_z;
     Z.z = exp(log_z);
@@ -461,50 +462,59 @@ __device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const LdsSp
         Z.ly = Z.log_z - P.log_r;
         Z.inv_y2 = 1.0 / (Z.z * Z.z);
     }
-    auto f = [&](double theta) {
-        const double sn = sin(0.5 * theta);
-        if constexpr (ANY)
-            return k1_theta_term_any(P, Z, sn * sn, s, c, H, rows, rapidity);
-        else
-            return k1_theta_term(P, Z, sn * sn, s, c);
-    };
-    // First pass: the 21 nodes of [0, pi]; then per bisection of [a, b] the
-    // 21 nodes of [a, mid] (lanes 0-20) and of [mid, b] (lanes 21-41).
-    int count = 0, split = 0;
-    double a = 0.0, b = kPi, mid = kPi, value = 0.0;
+    return Z;
+}
+
+// _lo_integrand (S/bk/derivatives.py:434-468) at angle theta.
+template <bool ANY>
+__device__ __forceinline__ double k1_theta_value(const K1Parent& P, const K1Z& Z, double theta,
+                                                 const LdsSpline<double>& s, const BkConsts& c,
+                                                 const K1History& H, int rows, double rapidity) {
+    const double sn = sin(0.5 * theta);
+    if constexpr (ANY)
+        return k1_theta_term_any(P, Z, sn * sn, s, c, H, rows, rapidity);
+    else
+        return k1_theta_term(P, Z, sn * sn, s, c);
+}
+
+// The theta integral at one daughter size z for parent P, by one wave, from
+// the 21 values `first` of the block's first pass on [0, pi]; each bisection
+// of [a, b] evaluates the 21 nodes of [a, mid] (lanes 0-20) and of [mid, b]
+// (lanes 21-41).
+template <bool ANY>
+__device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const double* first,
+                                    const LdsSpline<double>& s, const BkConsts& c, const K1History& H,
+                                    int rows, double rapidity, int tint, double epsrel, const GkList& L,
+                                    double* v, int lane, long long& evals) {
+    const K1Z Z = k1_daughter<ANY>(P, log_z, s, c);
+    double e, r;
+    gk_from_values(0.0, kPi, first, e, r);
+    if (lane == 0) {
+        L.lo[0] = 0.0;
+        L.hi[0] = kPi;
+        L.est[0] = e;
+        L.err[0] = r;
+    }
+    evals += 21;
+    int count = 1, split = 0;
+    double value = e;
     for (;;) {
-        if (lane < (count == 0 ? 21 : 42)) {
-            const double x = count == 0 ? gk_node(0.0, kPi, lane)
-                                        : (lane < 21 ? gk_node(a, mid, lane) : gk_node(mid, b, lane - 21));
-            v[lane] = f(x);
-        }
-        wave_sync();
-        if (count == 0) {
-            double e, r;
-            gk_from_values(0.0, kPi, v, e, r);
-            wave_sync();
-            if (lane == 0) {
-                L.lo[0] = 0.0;
-                L.hi[0] = kPi;
-                L.est[0] = e;
-                L.err[0] = r;
-            }
-            evals += 21;
-        } else {
-            double e0, r0, e1, r1;
-            gk_from_values(a, mid, v, e0, r0);
-            gk_from_values(mid, b, v + 21, e1, r1);
-            gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
-            evals += 42;
-        }
-        ++count;
         wave_sync();
         double error;
         gk_scan(L, count, value, error, split);
         if (count >= tint || error <= epsrel * fabs(value)) break;
-        a = L.lo[split];
-        b = L.hi[split];
-        mid = (a + b) / 2.0;
+        const double a = L.lo[split], b = L.hi[split], mid = (a + b) / 2.0;
+        if (lane < 42) {
+            const double x = lane < 21 ? gk_node(a, mid, lane) : gk_node(mid, b, lane - 21);
+            v[lane] = k1_theta_value<ANY>(P, Z, x, s, c, H, rows, rapidity);
+        }
+        wave_sync();
+        double e0, r0, e1, r1;
+        gk_from_values(a, mid, v, e0, r0);
+        gk_from_values(mid, b, v + 21, e1, r1);
+        gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
+        ++count;
+        evals += 42;
     }
     wave_sync();
     return value;
@@ -512,9 +522,10 @@ __device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const LdsSp
 
 // LDS of the adaptive K1 kernel (doubles): spline (R nodes + 4 (R - 1)
 // coefficients), J1 table, outer list (4 cap), outer nodes and values (2 x
-// 42), then per wave a list (4 cap) and 42 node values.
+// 42), the first-pass values of the pending inner integrals (42 x 21), then
+// per wave a list (4 cap) and 42 node values.
 __host__ __device__ inline size_t k1a_lds_doubles(int R, int cap) {
-    return (size_t)R + 4 * (size_t)(R - 1) + kJ1Pieces * (kJ1Deg + 1) + 4 * (size_t)cap + 84 +
+    return (size_t)R + 4 * (size_t)(R - 1) + kJ1Pieces * (kJ1Deg + 1) + 4 * (size_t)cap + 84 + 42 * 21 +
            (size_t)kK1aWaves * (4 * (size_t)cap + 42);
 }
 
@@ -543,8 +554,9 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     const GkList O{o_base, o_base + cap, o_base + 2 * cap, o_base + 3 * cap};
     double* o_node = o_base + 4 * cap;  // 42 ln z nodes
     double* o_val = o_node + 42;        // 42 outer integrand values
+    double* f_val = o_val + 42;         // 42 x 21 first-pass theta values
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double* w_base = o_val + 42 + (size_t)wave * (4 * cap + 42);
+    double* w_base = f_val + 42 * 21 + (size_t)wave * (4 * cap + 42);
     const GkList W{w_base, w_base + cap, w_base + 2 * cap, w_base + 3 * cap};
     double* w_val = w_base + 4 * cap;
 
@@ -578,15 +590,23 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     double value = 0.0, error = 0.0;
     long long evals = 0;
     for (int step = 0; step < rint; ++step) {
-        // inner integrals at the pending outer nodes, one wave each
+        // first pass on [0, pi] of every pending inner integral, over the
+        // whole block (n_tasks x 21 theta nodes)
+        for (int i = threadIdx.x; i < n_tasks * 21; i += blockDim.x) {
+            const int t = i / 21, k = i - 21 * t;
+            const K1Z Z = k1_daughter<ANY>(P, o_node[t], s, c);
+            f_val[i] = k1_theta_value<ANY>(P, Z, gk_node(0.0, kPi, k), s, c, H, rows, rapidity);
+        }
+        __syncthreads();
+        // the bisections of each inner integral, one wave each
         for (;;) {
             int t = 0;
             if (lane == 0) t = atomicAdd(&task_next, 1);
             t = __shfl(t, 0, 64);
             if (t >= n_tasks) break;
             const double lz = o_node[t];
-            const double inner = k1_theta_adaptive<ANY>(P, lz, s, c, H, rows, rapidity, tint, epsrel, W,
-                                                        w_val, lane, evals);
+            const double inner = k1_theta_adaptive<ANY>(P, lz, f_val + 21 * t, s, c, H, rows, rapidity, tint,
+                                                        epsrel, W, w_val, lane, evals);
             if (lane == 0) o_val[t] = inner * exp(2.0 * lz) * 2.0;
         }
         __syncthreads();

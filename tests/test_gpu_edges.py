@@ -174,4 +174,5 @@ def test_batch_of_one_run(dev):
                    K1_RULE="fixed")
     (t,) = solve_bk_batch([ic], 0.3, cfg, seed=0)
     single = solve_bk(ic, 0.3, cfg, seed=0)
-    assert torch.equal(t.N, single.N)
+    # equal up to the grouping of the K2 + Kf sums (the batch tunes its launch shape)
+    torch.testing.assert_close(t.N, single.N, rtol=1e-6, atol=1e-12)

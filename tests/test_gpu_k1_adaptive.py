@@ -184,6 +184,8 @@ def test_batch_adaptive_k1_matches_single_runs(dev):
     cfgs = [BKConfig(RPOINTS=R, MINR=1e-6, MAXR=1e2, MCINTPOINTS=1 << 10, DE_SOLVER_STEP=0.1,
                      RK_METHOD="rk2", K1_RULE="adaptive") for R in (30, 45)]
     tables = solve_bk_batch(ics, 0.2, cfgs, dtype=torch.float64, seed=0)
+    # the K1 kernels are the same; the K2 + Kf sums may group samples per lane
+    # differently (the batch tunes its launch shape), so FP64 rounding differs
     for ic, cfg, t in zip(ics, cfgs, tables):
         single = solve_bk(ic, 0.2, cfg, dtype=torch.float64, seed=0)
-        assert torch.equal(single.N, t.N)
+        torch.testing.assert_close(t.N, single.N, rtol=1e-12, atol=1e-20)
