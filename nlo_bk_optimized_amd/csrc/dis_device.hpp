@@ -30,7 +30,27 @@ struct K01 {
 //   K0 = -(ln(x/2) + gamma) I0 + sum H_k t^k / k!^2,
 //   K1 = 1/x + (x/2)(ln(x/2) + gamma) sum t^k/(k!(k+1)!) - (x/4) sum (H_k + H_{k+1}) t^k/(k!(k+1)!);
 // x > 2: K = e^-x / sqrt(x) * sum c_m T_m(4/x - 1) (Chebyshev fit of the
-// scaled function, coefficients from gen_bessel_tables.py).
+// scaled function, coefficients from gen_bessel_tables.py).  FP32 takes the
+// series' constant divisors as products and 1/x, 1/sqrt(x) and ln from the
+// hardware approximations; FP64 keeps IEEE divisions.
+// Reciprocal, reciprocal square root and natural log: IEEE operations in
+// FP64; in FP32 the hardware approximations (about 1 ulp; the log with an
+// absolute error ~1e-7), where no FP32 result is a small difference of them.
+template <typename T>
+__device__ __forceinline__ T dis_rcp(T x) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_amdgcn_rcpf(x);
+    else
+        return T(1) / x;
+}
+template <typename T>
+__device__ __forceinline__ T dis_rsqrt(T x) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_amdgcn_rsqf(x);
+    else
+        return T(1) / sqrt(x);
+}
+
 template <typename T>
 __device__ __forceinline__ K01<T> bessel_k01(T x) {
     constexpr bool f64 = sizeof(T) == 8;
@@ -46,8 +66,13 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
         T i0 = T(1), s0 = T(0), i1 = T(1), s1 = T(1);
 #pragma unroll
         for (int k = 1; k < terms; ++k) {
-            a *= t / T(k * k);
-            b *= t / T(k * (k + 1));
+            if constexpr (f64) {
+                a *= t / T(k * k);
+                b *= t / T(k * (k + 1));
+            } else {
+                a *= t * T(1.0 / double(k * k));
+                b *= t * T(1.0 / double(k * (k + 1)));
+            }
             h = hn;
             hn += T(1) / T(k + 1);
             i0 += a;
@@ -55,10 +80,15 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
             i1 += b;
             s1 += (h + hn) * b;
         }
-        const T lg = log(T(0.5) * x) + T(kEulerGamma);
-        return {s0 - lg * i0, T(1) / x + T(0.5) * x * lg * i1 - T(0.25) * x * s1};
+        if constexpr (f64) {
+            const T lg = log(T(0.5) * x) + T(kEulerGamma);
+            return {s0 - lg * i0, T(1) / x + T(0.5) * x * lg * i1 - T(0.25) * x * s1};
+        } else {
+            const T lg = __builtin_amdgcn_logf(T(0.5) * x) * T(0.69314718055994531) + T(kEulerGamma);
+            return {s0 - lg * i0, dis_rcp(x) + T(0.5) * x * lg * i1 - T(0.25) * x * s1};
+        }
     }
-    const T s = T(4) / x - T(1);
+    const T s = (f64 ? T(4) / x : T(4) * dis_rcp(x)) - T(1);
     const T s2 = s + s;
     T b0k0 = T(0), b1k0 = T(0), b0k1 = T(0), b1k1 = T(0);
     if constexpr (f64) {
@@ -87,7 +117,7 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
         }
         const T g0 = s * b0k0 - b1k0 + kBesselK0Cheb32[0];
         const T g1 = s * b0k1 - b1k1 + kBesselK1Cheb32[0];
-        const T sc = exp(-x) / sqrt(x);
+        const T sc = exp(-x) * dis_rsqrt(x);
         return {g0 * sc, g1 * sc};
     }
 }
@@ -583,13 +613,41 @@ struct TMoments {
     T b22j, b22k, b12j, b12k, g21j, g21k, g11j, g11k;
 };
 
+// FP32: g_values with 1/y, 1/sqrt(y) and 1/den shared by the lam and lam = 0
+// evaluations of a leg and no divisions (den does not depend on lam).
+__device__ __forceinline__ GValues<float> g_values_f32(float num, float den, float iden, float om, float iy,
+                                                       float siy) {
+    const float arg = sqrtf(num * den * iy);
+    const K01<float> k = bessel_k01(arg);
+    const float k2 = k.k0 + 2.0f * k.k1 * dis_rcp(arg);
+    const float nd = num * iden;
+    const float ratio = sqrtf(nd);
+    GValues<float> g;
+    g.g11 = 2.0f * k.k0 * iy;
+    g.g12 = 4.0f * om * siy * iy * ratio * k.k1;
+    g.g21 = 4.0f * siy * ratio * k.k1;
+    g.g22 = 8.0f * om * iy * nd * k2;
+    return g;
+}
+
 template <typename T>
 __device__ __forceinline__ TMoments<T> t_moments(const Tripole<T>& v, T m2, T y) {
-    const GValues<T> j = g_values(v.qj2, m2, v.x2j2, v.x3j2, v.om_j, v.lam_j, y);
-    const GValues<T> j0 = g_values(v.qj2, m2, v.x2j2, v.x3j2, v.om_j, T(0), y);
-    const GValues<T> k = g_values(v.qk2, m2, v.x2k2, v.x3k2, v.om_k, v.lam_k, y);
-    const GValues<T> k0 = g_values(v.qk2, m2, v.x2k2, v.x3k2, v.om_k, T(0), y);
-    return {j.g22 - j0.g22, k.g22 - k0.g22, j.g12 - j0.g12, k.g12 - k0.g12, j.g21, k.g21, j.g11, k.g11};
+    if constexpr (sizeof(T) == 4) {
+        const float iy = dis_rcp(y), siy = dis_rsqrt(y);
+        const float dj = y * v.x3j2 + v.om_j * v.x2j2, dk = y * v.x3k2 + v.om_k * v.x2k2;
+        const float idj = dis_rcp(dj), idk = dis_rcp(dk);
+        const GValues<float> j = g_values_f32(y * v.lam_j * m2 + v.qj2, dj, idj, v.om_j, iy, siy);
+        const GValues<float> j0 = g_values_f32(v.qj2, dj, idj, v.om_j, iy, siy);
+        const GValues<float> k = g_values_f32(y * v.lam_k * m2 + v.qk2, dk, idk, v.om_k, iy, siy);
+        const GValues<float> k0 = g_values_f32(v.qk2, dk, idk, v.om_k, iy, siy);
+        return {j.g22 - j0.g22, k.g22 - k0.g22, j.g12 - j0.g12, k.g12 - k0.g12, j.g21, k.g21, j.g11, k.g11};
+    } else {
+        const GValues<T> j = g_values(v.qj2, m2, v.x2j2, v.x3j2, v.om_j, v.lam_j, y);
+        const GValues<T> j0 = g_values(v.qj2, m2, v.x2j2, v.x3j2, v.om_j, T(0), y);
+        const GValues<T> k = g_values(v.qk2, m2, v.x2k2, v.x3k2, v.om_k, v.lam_k, y);
+        const GValues<T> k0 = g_values(v.qk2, m2, v.x2k2, v.x3k2, v.om_k, T(0), y);
+        return {j.g22 - j0.g22, k.g22 - k0.g22, j.g12 - j0.g12, k.g12 - k0.g12, j.g21, k.g21, j.g11, k.g11};
+    }
 }
 
 template <typename T>
