@@ -71,6 +71,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--no-hbm", action="store_true", help="skip the Sobol / VEGAS map / histogram bandwidth runs")
     p.add_argument("--large-maxy", type=float, default=0.0,
                    help="run the 1024-node, 2^24-point RK4 evolution to this Y in FP32 and FP64 (0 skips it)")
+    p.add_argument("--large-precision", default="both", choices=("both", "fp32", "fp64_stable"),
+                   help="which path(s) the large run evolves")
+    p.add_argument("--large-save", default=None, help="npz file for N at every unit of Y of the large run")
+    p.add_argument("--large-resume", default=None, help="npz file (from --large-save) to continue from")
     p.add_argument("--batch-runs", type=int, default=64,
                    help="evolutions in the batched uneven-grid run (0 skips it)")
     p.add_argument("--dis-points", type=int, default=32,
@@ -358,19 +362,34 @@ def main() -> int:
         # The large run: 1024 r in [1e-7, 1e2], 2^24 Sobol points per parent per
         # RHS, classical RK4 with dY = 0.05, FP32 integrand with FP64 sums
         # against the FP64 evaluation of the same formula (the GPU oracle of
-        # the FP32 path), max |dN| at every unit of Y up to --large-maxy.
+        # the FP32 path), max |dN| at every unit of Y up to --large-maxy.  A
+        # long run can be split: --large-precision runs one path only,
+        # --large-save writes N at every unit of Y, --large-resume continues
+        # from the last state such a file holds (the grids are adapted on the
+        # initial condition again, deterministically).
+        import numpy as np
+
         nl = 1 << 24
         large = dict(RPOINTS=1023, MINR=1e-7, MAXR=1e2, MCINTPOINTS=nl, MC_BINS=50, VEGAS_ADAPT_ITERATIONS=5,
                      DE_SOLVER_STEP=0.05, RK_METHOD="rk4")
-        steps_l = int(round(args.large_maxy / 0.05))
         per_unit = int(round(1 / 0.05))
+        precs = ("fp32", "fp64_stable") if args.large_precision == "both" else (args.large_precision,)
+        y_start, rows_in = 0, {}
+        if args.large_resume:
+            with np.load(args.large_resume) as f:
+                rows_in = {k: f[k] for k in f.files}
+            y_start = int(rows_in["y"][-1])
+        steps_l = int(round((args.large_maxy - y_start) / 0.05))
         states, times = {}, {}
-        for prec in ("fp32", "fp64_stable"):
+        for prec in precs:
             cfg = BKConfig(PRECISION=prec, **large)
             setup_l = evo.prepare(ic, cfg, device=device, seed=0)
             ev = dev.Evolver(setup_l.r, setup_l.edges, setup_l.table, cfg, n_samples=nl, precision=prec,
                              rk_method="rk4", step=0.05)
-            ev.state.copy_(setup_l.N0)
+            if y_start > 0:
+                ev.state.copy_(torch.as_tensor(rows_in[prec][-1], dtype=torch.float64, device=device))
+            else:
+                ev.state.copy_(setup_l.N0)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             rows = []
@@ -379,20 +398,25 @@ def main() -> int:
                 if k % per_unit == 0:
                     rows.append(ev.state.clone())
                     torch.cuda.synchronize()
-                    log(f"large {prec}: Y = {k * 0.05:.0f} after {time.perf_counter() - t0:.1f} s")
+                    log(f"large {prec}: Y = {y_start + k * 0.05:.0f} after {time.perf_counter() - t0:.1f} s")
             torch.cuda.synchronize()
             times[prec] = time.perf_counter() - t0
             states[prec] = rows
             ev.close()
-        dn = [float((a - b).abs().max()) for a, b in zip(states["fp32"], states["fp64_stable"])]
-        extra.update({
-            "large_maxy": args.large_maxy,
-            "large_fp32_s": times["fp32"],
-            "large_fp64_stable_s": times["fp64_stable"],
-            "large_speedup_fp32_vs_fp64_stable": times["fp64_stable"] / times["fp32"],
-            "large_max_abs_dN_by_unit_Y": dn,
-        })
-        log(f"large: fp32 {times['fp32']:.1f} s, fp64 {times['fp64_stable']:.1f} s, max |dN| per unit Y {dn}")
+        extra.update({"large_y_from": y_start, "large_maxy": args.large_maxy})
+        for prec in precs:
+            extra[f"large_{prec}_s"] = times[prec]
+        if len(precs) == 2:
+            dn = [float((a - b).abs().max()) for a, b in zip(states["fp32"], states["fp64_stable"])]
+            extra.update({
+                "large_speedup_fp32_vs_fp64_stable": times["fp64_stable"] / times["fp32"],
+                "large_max_abs_dN_by_unit_Y": dn,
+            })
+            log(f"large: fp32 {times['fp32']:.1f} s, fp64 {times['fp64_stable']:.1f} s, max |dN| per unit Y {dn}")
+        if args.large_save:
+            out = {prec: np.stack([r.cpu().numpy() for r in states[prec]]) for prec in precs}
+            out["y"] = np.arange(y_start + 1, y_start + 1 + len(states[precs[0]]))
+            np.savez(args.large_save, **out)
 
     if args.batch_runs > 0:
         # Batched uneven evolutions: runs with Qs0^2 ~ U[0.1, 0.5], gamma ~
