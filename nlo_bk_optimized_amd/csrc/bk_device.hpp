@@ -212,10 +212,17 @@ __device__ __forceinline__ T loglog_from_log(const LdsSpline<T>& s, T log_d, T u
 // alpha(d) from ln d (S/coupling.py:81-97): ln(4 C2 / (d^2 Lambda^2)) =
 // log_scale0 - 2 ln d.  The denominator is returned too, so that ratios of
 // couplings need no division: alpha_a / alpha_b = alpha_a * den_b.
+// Far from the freezing scale (|mu0 - right| > 40) the smooth-max term
+// log1p(e^-|d|) < 4.3e-18 is below half an ulp of hi >= mu0 = 10 ln 2.5, so
+// hi + term rounds to hi: skipping the exp and log1p there changes no bit,
+// and whole waves (small distances) skip them.
 __device__ __forceinline__ double alpha_s64_den(double log_d, const BkConsts& c) {
     const double right = (c.log_scale0 - 2.0 * log_d) * 5.0;  // / 0.2 within one rounding
     const double hi = fmax(c.log_mu0, right);
-    return c.b0 * (0.2 * (hi + log1p(exp(-fabs(c.log_mu0 - right)))));
+    const double d = fabs(c.log_mu0 - right);
+    double sum = hi;
+    if (!(d > 40.0)) sum = hi + log1p(exp(-d));  // NaN keeps the full formula
+    return c.b0 * (0.2 * sum);
 }
 
 struct K1Parent {
