@@ -30,6 +30,18 @@ __device__ constexpr double kI1Series[21] = {
     1.2839980333028028e-33, 3.3789421929021126e-36, 8.045100459290744e-39,
 };
 
+// J1(2a)/a for s = -a^2 <= 0 and I1(2a)/a for s = a^2 >= 0: both are the one
+// power series sum_k s^k / (k! (k+1)!) (A&S 9.1.10, 9.6.10), here to k = 20 by
+// Horner's rule.  For |s| <= 16 (2a <= 8) the tail is below 4e-18 of the
+// I1 sum; on the J1 side the alternating terms (at most ~23) leave an
+// absolute rounding error below 1e-14.
+__host__ __device__ inline __attribute__((noinline)) double j1_i1_series(double s) {
+    double sum = kI1Series[20];
+#pragma unroll
+    for (int k = 19; k >= 0; --k) sum = fma(sum, s, kI1Series[k]);
+    return sum;
+}
+
 __host__ __device__ inline __attribute__((noinline)) double i1_over_half_x(double x) {
     if (x < 8.0) {
         const double t = 0.25 * x * x;

@@ -260,11 +260,19 @@ __device__ __forceinline__ double k1_theta_term(const K1Parent& P, const K1Z& Z,
     lo = isfinite(lo) ? lo : 0.0;
     const double xl = 4.0 * (lnX - P.log_r) * Z.ly;
     const double abar = a_min * (c.nc / kPi);  // loop-invariant quotient
-    const double as_x = sqrt(abar * fabs(xl));
+    // J1(2a)/a (xl >= 0) or I1(2a)/a (xl < 0), a^2 = abar |xl| (k1.py:108-132):
+    // one series in s = -abar xl for |s| <= 16 (no divergence between the two
+    // sides), the J1 table or the large-argument I1 beyond
+    const double sres = -abar * xl;
     double dlog = 1.0;
-    // J1(2a)/a or I1(2a)/a (k1.py:108-132)
-    if (P.resum_on != 0.0 && as_x != 0.0)
-        dlog = xl >= 0.0 ? j1_over_half_x(2.0 * as_x, P.j1tab) : i1_over_half_x(2.0 * as_x);
+    if (P.resum_on != 0.0 && sres != 0.0) {
+        if (fabs(sres) <= 16.0) {
+            dlog = j1_i1_series(sres);
+        } else {
+            const double as_x = sqrt(fabs(sres));
+            dlog = sres < 0.0 ? j1_over_half_x(2.0 * as_x, P.j1tab) : i1_over_half_x(2.0 * as_x);
+        }
+    }
     if (!isfinite(dlog)) return 0.0;
     // ln(KSUB r^2 / min(X, Y)^2): the single-log exponent of both slog terms
     const double lmin = X <= Y ? lnX : Z.log_z;
