@@ -333,10 +333,16 @@ __device__ inline double kernel_lo_any(double r, double log_r, double X, double 
                                                             : alpha_any64(mn, c);
     const double abar = a_res * c.nc / kPi;
     const double xl = 4.0 * (lnX - log_r) * (lnY - log_r);
-    const double as_x = sqrt(abar * fabs(xl));
+    const double sres = -abar * xl;  // a^2 with the J1 side negative, as in k1_theta_term
     double dlog = 1.0;
-    if (r > 1.01 * c.minr && as_x != 0.0)
-        dlog = xl >= 0.0 ? j1_over_half_x(2.0 * as_x, j1tab) : i1_over_half_x(2.0 * as_x);
+    if (r > 1.01 * c.minr && sres != 0.0) {
+        if (fabs(sres) <= 16.0) {
+            dlog = j1_i1_series(sres);
+        } else {
+            const double as_x = sqrt(fabs(sres));
+            dlog = sres < 0.0 ? j1_over_half_x(2.0 * as_x, j1tab) : i1_over_half_x(2.0 * as_x);
+        }
+    }
     if (!isfinite(dlog)) return 0.0;
     double slog = 1.0, slog_exp = 0.0;
     if (c.slog) {
