@@ -61,6 +61,30 @@ def test_auto_rk_method_follows_the_reference_selection():
     assert resolve_rk_method(BKConfig(RK_METHOD="rk4", RPOINTS=255), f32) == "rk4"
 
 
+def test_auto_k1_rule_follows_the_reference_selection():
+    """K1_RULE="auto": the fixed rule only where the reference validated it
+    (float32, production physics, RPOINTS = 100 and the default K1 limits;
+    S/bk/derivatives.py:145-166 with S/bk/evolution.py:160-177), else the
+    adaptive rule; explicit choices win; the device lists hold 100 intervals."""
+
+    from nlo_bk_optimized_amd.bk.config import BKOrder, resolve_k1_rule
+
+    f32, f64 = torch.float32, torch.float64
+    assert resolve_k1_rule(BKConfig(), f32) == "fixed"
+    assert resolve_k1_rule(BKConfig(), f64) == "adaptive"
+    assert resolve_k1_rule(BKConfig(RPOINTS=255), f32) == "adaptive"
+    assert resolve_k1_rule(BKConfig(INTACCURACY=1e-4), f32) == "adaptive"
+    assert resolve_k1_rule(BKConfig(Order=BKOrder.NLO), f32) == "adaptive"
+    assert resolve_k1_rule(BKConfig(NF=5), f32) == "adaptive"
+    assert resolve_k1_rule(BKConfig(K1_RULE="fixed", RPOINTS=255), f64) == "fixed"
+    assert resolve_k1_rule(BKConfig(K1_RULE="adaptive"), f32) == "adaptive"
+    with pytest.raises(NotImplementedError, match="100 intervals"):
+        resolve_k1_rule(BKConfig(RINTPOINTS=150), f64)
+    assert resolve_k1_rule(BKConfig(RINTPOINTS=150, K1_RULE="fixed"), f64) == "fixed"
+    with pytest.raises(ValueError, match="K1_RULE"):
+        BKConfig(K1_RULE="gauss")
+
+
 def test_r_grid_matches_oracle():
     c = BKConfig(RPOINTS=255, MINR=1e-6, MAXR=1e2)
     r = _r_grid(c, device="cpu", dtype=torch.float64).numpy()
