@@ -751,11 +751,13 @@ struct FastSpline {
     float inv_h, off, umax;  // umax = R
 };
 
+// u clamped to [0, umax] by one median-of-three (log_d is finite: it is the
+// log of a distance); for u >= 0 the fraction u - floor(u) is exact and
+// (int)u = floor(u), so t and the record index need no floor.
 __device__ __forceinline__ float fspline_log2n(const FastSpline& s, float log_d) {
-    const float u = fminf(fmaxf(fmaf(log_d, s.inv_h, s.off), 0.0f), s.umax);
-    const float k = floorf(u);
-    const float t = u - k;
-    const float4 c = s.rec[(int)k];
+    const float u = __builtin_amdgcn_fmed3f(fmaf(log_d, s.inv_h, s.off), 0.0f, s.umax);
+    const float t = __builtin_amdgcn_fractf(u);
+    const float4 c = s.rec[(int)u];
     return fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
 }
 
