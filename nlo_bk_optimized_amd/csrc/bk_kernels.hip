@@ -874,7 +874,12 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
             if (g & 1u)
                 for (int d = 0; d < 4; ++d) vhi[d] ^= a.dir[d][b];
     }
-    for (int j = 0; j < a.spt; ++j) {
+    // One 64-point group per call.  FIRST: the group holding Sobol index 0
+    // (hi == 0), whose lane 0 takes the float64 first point and may carry
+    // q = 2^30 (the clamped bin); only hi0 == 0, j == 0 is that group, so the
+    // steady-state loop carries no first-point test.
+    auto group = [&](int j, auto first_tag) {
+        constexpr bool FIRST = decltype(first_tag)::value;
         const int64_t hi = hi0 + j;
         if (j > 0) {
             const int b = 6 + __builtin_ctzll((unsigned long long)hi);
@@ -886,8 +891,7 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         const bool odd = (hi & 1) != 0;
         uint32_t q[4];
         for (int d = 0; d < 4; ++d) q[d] = vlo[d][0] ^ (odd ? (vhi[d] ^ a.dir[d][5]) : vhi[d]);
-        const bool first_group = hi == 0;  // wave-uniform: the wave holding Sobol index 0
-        if (first_group && lane == 0)
+        if (FIRST && lane == 0)
             for (int d = 0; d < 4; ++d) q[d] = a.first_q[d];
         T x[4];
         T jac = T(1);
@@ -900,7 +904,7 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
             const uint32_t q4 = q[d] << 2;
             uint32_t b = __umulhi(q4, (uint32_t)a.bins);
             T frac = (T)(q4 * (uint32_t)a.bins) * (T)0x1p-32;
-            if (first_group && q[d] >= (1u << 30)) {
+            if (FIRST && q[d] >= (1u << 30)) {
                 b = (uint32_t)(a.bins - 1);
                 frac = T(1);
             }
@@ -951,7 +955,13 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
             st_w[threadIdx.x + j * 256] = valid ? fabs(wv) : 0.0;
             st_b[threadIdx.x + j * 256] = valid ? packed : 0xffffffffu;
         }
+    };
+    int j0 = 0;
+    if (hi0 == 0) {
+        group(0, std::true_type{});
+        j0 = 1;
     }
+    for (int j = j0; j < a.spt; ++j) group(j, std::false_type{});
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     sk2 = wave_sum(sk2);
