@@ -794,13 +794,17 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         // of the mapped interval, formed in FP64 and rounded once: the sample
         // is then x = X_b + W_b frac with one more rounding, where forming
         // lo + w (left + width frac) in FP32 cancels (lo = ln(r_0 / 2) against
-        // the scaled edge) and leaves several ulp in ln z.
+        // the scaled edge) and leaves several ulp in ln z.  Wave 0 alone
+        // builds the table, so that only one wave of the block evaluates the
+        // two FP64 logarithms of the bounds.
         float2* E2 = reinterpret_cast<float2*>(smem + lay.edges);
-        for (int i = threadIdx.x; i < 4 * a.bins; i += blockDim.x) {
-            const int d = i / a.bins, b = i - d * a.bins;
-            const double lo_d = d < 2 ? lo_r : 0.0, w_d = d < 2 ? w_r : 2.0 * kPi;
-            const double left = ep[d * (a.bins + 1) + b], right = ep[d * (a.bins + 1) + b + 1];
-            E2[i] = make_float2((float)(lo_d + w_d * left), (float)(w_d * (right - left)));
+        if (threadIdx.x < kWave) {
+            for (int i = threadIdx.x; i < 4 * a.bins; i += kWave) {
+                const int d = i / a.bins, b = i - d * a.bins;
+                const double lo_d = d < 2 ? lo_r : 0.0, w_d = d < 2 ? w_r : 2.0 * kPi;
+                const double left = ep[d * (a.bins + 1) + b], right = ep[d * (a.bins + 1) + b + 1];
+                E2[i] = make_float2((float)(lo_d + w_d * left), (float)(w_d * (right - left)));
+            }
         }
     } else {
         for (int i = threadIdx.x; i < ne; i += blockDim.x) E[i] = ep[i];
@@ -833,7 +837,9 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         for (int d = 0; d < 4; ++d) vol = vol * wd[d];
     }
     const T r_t = (T)rp;
-    const T log_r_t = (T)log(rp);
+    // ln r_p as the spline table holds it (the same FP64 log of r_p)
+    const double log_rp = v.t64[2 + p];
+    const T log_r_t = (T)log_rp;
     NloCoupling<float> cpl32;
     NloCoupling<double> cpl64;
     if constexpr (ANYC) {
@@ -938,7 +944,7 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
                 kf = (double)(f.kf * jac);
             } else {
                 const NloStable<double> f = nlo_integrand_stable<double, ANYC>(
-                    rp, log(rp), x[0], x[1], x[2], x[3], spl, cpl64, a.c.nf / a.c.nc, a.c.nc,
+                    rp, log_rp, x[0], x[1], x[2], x[3], spl, cpl64, a.c.nf / a.c.nc, a.c.nc,
                     a.c.norm, a.c.symmetrize != 0);
                 wv = f.total * jac;
                 k2 = f.k2 * jac;
@@ -963,9 +969,11 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
     }
     for (int j = j0; j < a.spt; ++j) group(j, std::false_type{});
     s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
-    sk2 = wave_sum(sk2);
-    skf = wave_sum(skf);
+    if constexpr (FULL) {  // otherwise the three stay 0
+        s2 = wave_sum(s2);
+        sk2 = wave_sum(sk2);
+        skf = wave_sum(skf);
+    }
     if (lane == 0) {
         red[wave][0] = s1;
         red[wave][1] = s2;
