@@ -288,11 +288,13 @@ def main() -> int:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             vals = []
-            for q2, xb in points:
+            for k, (q2, xb) in enumerate(points):
                 Q2, xbj = torch.tensor(q2, dtype=dtype), torch.tensor(xb, dtype=dtype)
                 ft = FT(Q2, xbj, dipole, dcfg, seed=0)
                 fl = FL(Q2, xbj, dipole, dcfg, seed=1000)
                 vals.append((ft.value.item(), fl.value.item(), ft.error.item(), fl.error.item()))
+                if (k + 1) % 64 == 0 and k + 1 < len(points):
+                    log(f"DIS batch {name}: {k + 1} of {len(points)} points")
             torch.cuda.synchronize()
             res[name] = (np.array(vals), time.perf_counter() - t0)
             log(f"DIS batch {name}: {len(points)} points in {res[name][1]:.2f} s")
@@ -306,7 +308,9 @@ def main() -> int:
         from nlo_bk_optimized_amd import _lib
 
         frozen, worst = [], None
-        for q2, xb in points:
+        for k, (q2, xb) in enumerate(points):
+            if k % 64 == 0 and k > 0:
+                log(f"DIS frozen-grid comparison: {k} of {len(points)} points")
             for transverse in (True, False):
                 kinds = (("T1", "T2") if transverse else ("ab", "cd")) + ("I1", "I2_nested", "I3_nested")
                 for quark in dcfg.quarks:
