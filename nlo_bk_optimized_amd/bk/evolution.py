@@ -246,6 +246,10 @@ def solve_bk_batch(
     ("auto" means Kutta's third-order step); the kinematical constraint is not
     batched.  ``spt`` / ``grid_blocks`` fix the K2 + Kf launch shape (0: tuned on
     the first step); ``stats`` (a dict) receives the tuning and the evaluation count.
+    "auto" stepping and K1 rules resolve as ``solve_bk`` resolves them for each
+    run (the dtype of an ``ICDataFile`` run is the file's); the runs must
+    resolve alike, and a resolution to the adaptive RK2 raises
+    NotImplementedError (the batch steps with a fixed step size).
     """
 
     ics = list(initial_conditions)
@@ -259,10 +263,16 @@ def solve_bk_batch(
             raise ValueError(f"batched runs must share {diff}")
     if c0.KINEMATICAL_CONSTRAINT:
         raise NotImplementedError("the kinematical constraint is not batched: use solve_bk per run")
-    rk = "euler" if c0.EULER_METHOD else (c0.RK_METHOD if c0.RK_METHOD != "auto" else "kutta3")
+    dtypes = [ic.r.dtype if isinstance(ic, ICDataFile) else dtype for ic in ics]
+    methods = {resolve_rk_method(c, t) for c, t in zip(cfgs, dtypes)}
+    if len(methods) != 1:
+        raise ValueError(f"the runs resolve to different stepping methods {sorted(methods)}; set RK_METHOD")
+    rk = methods.pop()
     if rk == "adaptive_rk2":
-        raise NotImplementedError("the batched driver takes fixed-step methods only")
-    rules = {resolve_k1_rule(c, dtype) for c in cfgs}
+        raise NotImplementedError("the batched driver takes fixed-step methods only (RK_METHOD resolves to "
+                                  "adaptive_rk2 here, as solve_bk would): set RK_METHOD to euler, rk2, kutta3 "
+                                  "or rk4")
+    rules = {resolve_k1_rule(c, t) for c, t in zip(cfgs, dtypes)}
     if len(rules) != 1:
         raise ValueError("the runs resolve to different K1 rules (K1_RULE='auto' with RPOINTS = 100 in "
                          "some runs only); set K1_RULE to 'fixed' or 'adaptive'")
@@ -300,10 +310,9 @@ def solve_bk_batch(
     finally:
         ev.close()
     out, off = [], 0
-    yt = torch.tensor(ys, dtype=dtype, device=d)
-    for s, ic, R in zip(setups, ics, sizes):
-        out.append(DipoleTable(r=s.r.to(dtype), y=yt.clone(), N=history[:, off:off + R].to(dtype),
-                               x0=ic.x0))
+    for s, ic, R, t in zip(setups, ics, sizes, dtypes):
+        out.append(DipoleTable(r=s.r.to(t), y=torch.tensor(ys, dtype=t, device=d),
+                               N=history[:, off:off + R].to(t), x0=ic.x0))
         off += R
     return out
 

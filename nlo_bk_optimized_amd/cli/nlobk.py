@@ -122,7 +122,7 @@ def main(argv: list[str] | None = None) -> int:
     args = parser.parse_args(argv)
     device = torch.device(args.device)
     dtype = torch.float32 if args.dtype == "float32" else torch.float64
-    ic = (ICDataFile(args.ic_data, x0=args.x0, dtype=torch.float64) if args.ic_data
+    ic = (ICDataFile(args.ic_data, x0=args.x0, dtype=dtype) if args.ic_data
           else MV(qs0sqr=args.qs0sqr, x0=args.x0, ec=args.ec, lambdaqcd=args.lambdaqcd, gamma=args.gamma))
     config = config_from_args(args)
     if args.dndy:

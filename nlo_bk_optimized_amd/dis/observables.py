@@ -12,9 +12,10 @@ integrands, float64 -> FP64 (sums are FP64 in both).
 Differences from the reference, on purpose:
 * VEGAS draws scrambled Sobol points (seeded, deterministic) instead of
   torch.Generator samples;
-* the qqbar-g term always takes the VEGAS path (I1 plus the nested I2/I3 when
-  cuda_nested, else the 6-/7-dimensional I2/I3); the reference's GBW-only
-  fixed-Sobol replicate estimator (S/dis/fixed_sobol.py) is not reproduced.
+* the qqbar-g term always takes the VEGAS path (I1 plus, for float32 with
+  cuda_nested, the nested I2/I3, else the 6-/7-dimensional I2/I3); the
+  reference's GBW-only fixed-Sobol replicate estimator (S/dis/fixed_sobol.py)
+  is not reproduced.
 """
 
 from __future__ import annotations
@@ -273,7 +274,10 @@ def sigma_qg_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: DI
     _validate_kinematics(Q2, xbj)
     precision = _precision(Q2)
     device = _device_of(dipole, Q2)
-    kinds = ("I1", "I2_nested", "I3_nested") if config.cuda_nested else ("I1", "I2", "I3")
+    # the reference nests the I2 / I3 inner integral only on its float32 CUDA
+    # path (use_nested, S/dis/observables.py:492); float64 integrates the
+    # 6- / 7-dimensional forms
+    kinds = ("I1", "I2_nested", "I3_nested") if config.cuda_nested and precision == "fp32" else ("I1", "I2", "I3")
     results = []
     offset = 0
     for quark in config.quarks:

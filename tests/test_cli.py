@@ -122,3 +122,50 @@ def test_nlodis_f2fl_grid_mode(fake_dis, capsys):
     assert rec.calls[0][5] is torch.float64
     # F2 light = FT + FL of the light-quark calls
     assert float(row[3]) == pytest.approx(2.25 + 1.25) and float(row[4]) == pytest.approx(1.25)
+
+
+def _ic_file(tmp_path):
+    path = tmp_path / "ic.dat"
+    path.write_text("".join(f"{1e-6 * 10 ** (k / 4):.10e} {min(1.0, 1e-10 * 10 ** (k / 2)):.10e}\n" for k in range(34)))
+    return path
+
+
+def test_nlobk_ic_data_keeps_the_requested_dtype(tmp_path, monkeypatch):
+    """--ic-data with default flags: the file is read in float32 (the
+    reference passes dtype=dtype to ICDataFile, S/cli/nlobk.py), so
+    solve_bk resolves the production Kutta step and the fixed K1 rule."""
+
+    from nlo_bk_optimized_amd.bk.config import resolve_k1_rule, resolve_rk_method
+
+    seen = {}
+
+    def fake_solve(ic, maxy, config, *, device, dtype, seed):
+        effective = ic.r.dtype  # solve_bk takes the dtype of an ICDataFile
+        seen.update(ic_dtype=effective, dtype=dtype, rk=resolve_rk_method(config, effective),
+                    k1=resolve_k1_rule(config, effective))
+        return SimpleNamespace(y=torch.zeros(1), r=ic.r)
+
+    monkeypatch.setattr(nlobk, "solve_bk", fake_solve)
+    monkeypatch.setattr(nlobk, "save_bk_table", lambda path, table: None)
+    with redirect_stdout(io.StringIO()):
+        assert nlobk.main(["--ic-data", str(_ic_file(tmp_path)), "--device", "cpu"]) == 0
+    assert seen == {"ic_dtype": torch.float32, "dtype": torch.float32, "rk": "kutta3", "k1": "fixed"}
+    with redirect_stdout(io.StringIO()):
+        assert nlobk.main(["--ic-data", str(_ic_file(tmp_path)), "--device", "cpu", "--dtype", "float64"]) == 0
+    assert seen["ic_dtype"] is torch.float64 and seen["rk"] == "adaptive_rk2" and seen["k1"] == "adaptive"
+
+
+def test_nlobk_dndy_output(monkeypatch):
+    def fake_dndy(ic, config, *, device, dtype, seed):
+        assert config.DNDY and dtype is torch.float32 and seed == 4
+        v = torch.tensor([1e-3, 2e-3], dtype=dtype)
+        return SimpleNamespace(r=v, K1=v * 2, K2_Kf=-v, N=v / 10)
+
+    monkeypatch.setattr(nlobk, "compute_dndy", fake_dndy)
+    buf = io.StringIO()
+    with redirect_stdout(buf):
+        assert nlobk.main(["--dndy", "--seed", "4", "--device", "cpu"]) == 0
+    lines = buf.getvalue().strip().splitlines()
+    assert lines[0] == "# r dN/dy[K1] dN/dy[K2+Kf] N" and len(lines) == 3
+    vals = [float(x) for x in lines[2].split()]
+    assert vals == pytest.approx([2e-3, 4e-3, -2e-3, 2e-4], rel=1e-6)

@@ -100,6 +100,35 @@ def test_mv_matches_oracle():
         np.testing.assert_allclose(got, O.mv(r.numpy(), q, g, ec, 0.241), rtol=2e-12)
 
 
+def test_mv_product_matches_reference_vectors(golden):
+    """The package's MV (the amplitude solve_bk starts from) against the
+    reference's own outputs, in the reference's float64 arithmetic."""
+
+    r = torch.tensor(golden["r_grid"], dtype=torch.float64)
+    np.testing.assert_allclose(MV(qs0sqr=0.2, gamma=1.0).dipole_amplitude(r).numpy(), golden["mv_N"],
+                               rtol=1e-15, atol=0)
+    q, g, ec, lam = (float(v) for v in golden["mv2_params"])
+    r2 = torch.tensor(golden["mv2_r"], dtype=torch.float64)
+    got = MV(qs0sqr=q, gamma=g, ec=ec, lambdaqcd=lam).dipole_amplitude(r2).numpy()
+    np.testing.assert_allclose(got, golden["mv2_N"], rtol=1e-15, atol=0)
+    # float32 stays float32, as the reference's tensors do
+    assert MV().dipole_amplitude(r.float()).dtype == torch.float32
+
+
+def test_ic_data_file_errors(tmp_path):
+    bad = tmp_path / "bad.dat"
+    bad.write_text("".join(f"{1e-3 * (k + 1)} {0.01 * k}\n" for k in range(12)) + "0.5\n")
+    with pytest.raises(ValueError, match=r"bad.dat:13: expected the two columns"):
+        ICDataFile(bad)
+    bad.write_text("".join(f"{1e-3 * (k + 1)} {0.01 * k}\n" for k in range(12)) + "0.5 x\n")
+    with pytest.raises(ValueError, match=r"bad.dat:13: r and N must be numbers"):
+        ICDataFile(bad)
+    bad.write_text("# r N extra\n" + "".join(f"{1e-3 * (k + 1)} {0.01 * k} 7\n" for k in range(12)))
+    ic = ICDataFile(bad, dtype=torch.float32)
+    assert ic.r.dtype == torch.float32 and ic.r.numel() == 12
+    assert ic.min_r == pytest.approx(1e-3) and ic.max_r == pytest.approx(1.2e-2)
+
+
 def test_ic_data_file(tmp_path):
     r = np.logspace(-5, 1, 40)
     N = 1 - np.exp(-r**2)
