@@ -189,3 +189,27 @@ def test_batch_adaptive_k1_matches_single_runs(dev):
     for ic, cfg, t in zip(ics, cfgs, tables):
         single = solve_bk(ic, 0.2, cfg, dtype=torch.float64, seed=0)
         torch.testing.assert_close(t.N, single.N, rtol=1e-12, atol=1e-20)
+
+
+@pytest.mark.parametrize("tag,accuracy", [("e3", 1e-3), ("e6", 1e-6)])
+def test_device_adaptive_k1_matches_reference_values(dev, tag, accuracy):
+    """The device rule against the reference's own rapidity_derivative_lo
+    values on the 64-node MV workload (tests/golden/make_bk_rhs_golden.py,
+    accurate J1): within 5 % of the reference's error estimate and 2e-8
+    relative."""
+
+    from pathlib import Path
+
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    g = np.load(Path(__file__).with_name("golden") / "bk_rhs_reference_vectors.npz")
+    r, N, P = g["r_grid"], g["N0"], g["parents"]
+    print(f"adaptive K1 vs reference values, INTACCURACY {accuracy}", flush=True)
+    rt, Nt = cuda(r), cuda(N)
+    t64, _ = dev.spline_build(rt, Nt)
+    cfg = BKConfig(MINR=1e-6, RPOINTS=63, MAXR=1e2, INTACCURACY=accuracy)
+    k1 = dev.bk_k1(rt, Nt, t64, cfg, rule="adaptive").cpu().numpy()[P]
+    ref, ref_err = g[f"k1_{tag}_accj1"], g[f"k1err_{tag}_accj1"]
+    diff = np.abs(k1 - ref)
+    print("relative difference per parent:", diff / np.abs(ref))
+    assert np.all(diff <= 0.05 * ref_err) and np.all(diff <= 2e-8 * np.abs(ref))

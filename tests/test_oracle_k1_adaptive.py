@@ -121,3 +121,29 @@ def test_adaptive_k1_other_physics_agrees_with_fixed_rule():
     ka, _, _ = O.k1_adaptive(r[idx], sN, p=p)
     kf = O.k1_fixed(r[idx], sN, p=p)
     assert np.all(np.abs(ka - kf) <= 1e-3 * np.abs(kf))
+
+
+@pytest.mark.parametrize("tag,accuracy", [("e3", 1e-3), ("e6", 1e-6)])
+def test_adaptive_k1_matches_reference_rapidity_derivative_lo(tag, accuracy):
+    """The oracle's nested adaptive K1 against the reference's own
+    rapidity_derivative_lo (S/bk/derivatives.py:43-110) on the 64-node MV
+    workload (tests/golden/make_bk_rhs_golden.py): within 5 % of the
+    reference's error estimate and 2e-8 relative, with the accurate J1
+    (accj1) and with torch's J1 (torchj1, off by up to 4.7e-7 absolute).
+    A dropped term or a wrong sign moves K1 by far more than the 1e-3 / 1e-6
+    error estimates."""
+
+    from pathlib import Path
+
+    g = np.load(Path(__file__).with_name("golden") / "bk_rhs_reference_vectors.npz")
+    r, N, P = g["r_grid"], g["N0"], g["parents"]
+    np.testing.assert_allclose(r, O.r_grid(1e-6, 1e2, 63), rtol=4e-16)
+    sN, _ = O.evolution_splines(r, N)
+    value, error, _ = O.k1_adaptive(r[P], sN, 85, 85, accuracy, O.params(minr=1e-6))
+    for j1 in ("accj1", "torchj1"):
+        ref, ref_err = g[f"k1_{tag}_{j1}"], g[f"k1err_{tag}_{j1}"]
+        diff = np.abs(value - ref)
+        assert np.all(diff <= 0.05 * ref_err), (j1, diff / ref_err)
+        assert np.all(diff <= 2e-8 * np.abs(ref)), (j1, diff / np.abs(ref))
+        # both report an error estimate below the requested accuracy
+        assert np.all(ref_err <= accuracy * np.abs(ref)) and np.all(error <= accuracy * np.abs(value))
