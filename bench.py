@@ -31,10 +31,12 @@ import time
 METRIC = "NLO-BK evolution wall time + integrand samples/s, FP32 vs FP64 at fixed error"
 
 # Algorithmic flops per K2 + Kf sample of the FP32 path (VEGAS map, the
-# cancellation-free integrand and the accumulation), counted on the source
-# with add/sub/mul/div/sqrt/exp/log/log1p/expm1/sin/cos = 1, fma = 2,
-# compares and selects = 0.
-FLOPS_PER_SAMPLE = 253
+# cancellation-free integrand and the FP64 accumulation): the operation count
+# of the formula in oracle/flop_count.py (add/sub/mul/div, reciprocal, sqrt,
+# rsqrt, exp, exp2, log, log1p, expm1, sin, coth = 1, fma = 2, compares,
+# selects, min/max and integer work = 0), which tests/test_flop_count.py
+# checks against the C oracle by value and against this constant.
+FLOPS_PER_SAMPLE = 216
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per launch of the FP32 K2 + Kf kernel on this workload, from the
@@ -64,6 +66,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--rpoints", type=int, default=255)
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison runs")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-config1", action="store_true",
+                   help="skip the 64-node plain-FP64 RK2 evolution to Y = 1 on the GPU and on the CPU oracle")
     p.add_argument("--kernel-launches", type=int, default=5)
     p.add_argument("--cpu-samples-log2", type=int, default=20,
                    help="Sobol points per parent in the CPU baseline sample")
@@ -119,15 +123,21 @@ def main() -> int:
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
     def evolve(precision: str, steps: int, warmup: int):
+        # set-up: grid adaptation (5 VEGAS passes with histograms on the
+        # initial condition), then the evolver (buffers and the captured step
+        # graph); both are part of a full evolution's wall time
         config = BKConfig(PRECISION=precision, **base)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         setup = evo.prepare(ic, config, device=device, seed=0)
         torch.cuda.synchronize()
         t_adapt = time.perf_counter() - t0
+        t0 = time.perf_counter()
         ev = dev.Evolver(setup.r, setup.edges, setup.table, config, n_samples=n, precision=precision,
                          rk_method="rk2", step=config.DE_SOLVER_STEP)
         ev.state.copy_(setup.N0)
+        torch.cuda.synchronize()
+        t_setup = time.perf_counter() - t0
         for _ in range(warmup):
             ev.step()
         ev.state.copy_(setup.N0)
@@ -142,8 +152,8 @@ def main() -> int:
         flag = int(ev.flag.item())
         N = ev.state.clone()
         ev.close()
-        return dict(setup=setup, seconds=elapsed, adapt_seconds=t_adapt, evaluations=evaluations,
-                    N=N, nonfinite=flag, config=config)
+        return dict(setup=setup, seconds=elapsed, adapt_seconds=t_adapt, setup_seconds=t_setup,
+                    evaluations=evaluations, N=N, nonfinite=flag, config=config)
 
     if args.kernel_probe:
         # Counter runs (rocprofv3 --pmc) need only the dominant kernel and a copy
@@ -153,35 +163,48 @@ def main() -> int:
         t64, t32 = dev.spline_build(setup.r, setup.N0)
         for _ in range(args.kernel_launches):
             dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config,
-                       precision=args.probe_precision)
+                       precision=args.probe_precision, sum_only=True)
         src = torch.ones(1 << 25, dtype=torch.float64, device=device)  # 256 MiB
         dst = torch.empty_like(src)
         for _ in range(2):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"probe": f"nlo_kernel<{args.probe_precision}>", "launches": args.kernel_launches,
+        print(json.dumps({"probe": f"nlo_kernel<{args.probe_precision}, sum only>", "launches": args.kernel_launches,
                           "copy_bytes_read": src.numel() * 8, "copy_bytes_written": dst.numel() * 8}), flush=True)
         return 0
+
+    # One-time initialisation (library load, kernel attributes, the first
+    # allocations of the caching allocator): a first set-up, reported apart
+    # from the set-up every evolution pays.
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evo.prepare(ic, BKConfig(PRECISION="fp32", **base), device=device, seed=0)
+    torch.cuda.synchronize()
+    first_setup_s = time.perf_counter() - t0
 
     log(f"fp32 evolution: {args.steps} steps, 2^{args.samples_log2} samples/parent/RHS")
     fp32 = evolve("fp32", args.steps, args.warmup)
     samples = fp32["evaluations"] * n
     value = samples / fp32["seconds"]
-    log(f"fp32: {fp32['seconds']:.3f} s, {value:.4e} samples/s, adaptation {fp32['adapt_seconds']:.3f} s")
+    log(f"fp32: {fp32['seconds']:.3f} s, {value:.4e} samples/s, adaptation {fp32['adapt_seconds']:.3f} s, "
+        f"evolver set-up {fp32['setup_seconds']:.3f} s (first set-up in the process {first_setup_s:.3f} s)")
 
-    # Dominant kernel: the K2 + Kf integrand launch, timed with HIP events on
-    # the stream it is launched on (torch's current stream).
+    # Dominant kernel: the K2 + Kf integrand launch as the evolver runs it
+    # (sum w only) with its per-parent reduce, timed with HIP events on the
+    # stream it is launched on (torch's current stream).
     setup = fp32["setup"]
     config = fp32["config"]
     t64, t32 = dev.spline_build(setup.r, setup.N0)
     active0 = int(((torch.arange(setup.r.numel(), device=device) >= 1) & (setup.N0 <= 0.99999)).sum())
     for _ in range(2):
-        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32")
+        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32",
+                   sum_only=True)
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for _ in range(args.kernel_launches):
-        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32")
+        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32",
+                   sum_only=True)
     stop.record(stream)
     stop.synchronize()
     launch_ms = start.elapsed_time(stop) / args.kernel_launches
@@ -225,6 +248,7 @@ def main() -> int:
         extra.update({
             "fp64_samples_per_s": fp64["evaluations"] * n / fp64["seconds"],
             "fp64_evolution_s": fp64["seconds"],
+            "fp64_full_evolution_s": fp64["adapt_seconds"] + fp64["setup_seconds"] + fp64["seconds"],
             "speedup_fp32_vs_fp64": fp64["seconds"] / fp32["seconds"],
             "max_abs_dN_fp32_vs_fp64_same_formula": float((fp32["N"] - fp64s["N"]).abs().max()),
             "max_abs_dN_fp32_vs_fp64_reference_formula": float((fp32["N"] - fp64["N"]).abs().max()),
@@ -543,6 +567,59 @@ def main() -> int:
                 f"({100 * nbytes / sec / 8e12:.1f} % of 8 TB/s)")
         del u_h, x_h, jac_h, b_h
 
+    def host_cpu() -> dict:
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+        return {"nproc": len(os.sched_getaffinity(0)), "cpu_count": os.cpu_count(), "cpu_model": model}
+
+    if not args.no_config1:
+        # The 64-node plain-FP64 workload: R = 64 on [1e-6, 1e2], MV Qs0^2 =
+        # 0.2, gamma = 1, 2^14 Sobol points (seed 0) per parent, midpoint RK2
+        # with dY = 0.1 to Y = 1, K1 by the reference's rule at this grid size
+        # (the nested adaptive Gauss-Kronrod rule).  On the GPU through
+        # solve_bk, end to end (grid adaptation, evolver, 10 steps; one
+        # untimed call first); on the CPU the oracle's RK driver (FP64, every
+        # OpenMP thread) on the same grids and Sobol points.
+        sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+        import numpy as np
+
+        from nlo_bk_optimized_amd import solve_bk
+        from oracle import oracle as O
+
+        c1 = BKConfig(RPOINTS=63, MINR=1e-6, MAXR=1e2, MCINTPOINTS=1 << 14, MC_BINS=50, VEGAS_ADAPT_ITERATIONS=5,
+                      DE_SOLVER_STEP=0.1, RK_METHOD="rk2", PRECISION="fp64", K1_RULE="adaptive")
+        solve_bk(ic, 1.0, c1, device=device, dtype=torch.float64, seed=0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        table1 = solve_bk(ic, 1.0, c1, device=device, dtype=torch.float64, seed=0)
+        torch.cuda.synchronize()
+        gpu1_s = time.perf_counter() - t0
+        s1 = evo.prepare(ic, c1, device=device, seed=0)
+        r1, N01, e1 = (x.cpu().numpy() for x in (s1.r, s1.N0, s1.edges))
+        log("64-node FP64 RK2 to Y = 1: CPU oracle run")
+        t0 = time.perf_counter()
+        ref1 = O.evolve(r1, N01, e1, O.sobol_table(4, 0), 1 << 14, O.params(minr=1e-6), steps=10, h=0.1,
+                        method="rk2", k1_rule="adaptive")
+        cpu1_s = time.perf_counter() - t0
+        d1 = np.abs(table1.N.cpu().numpy() - ref1)
+        extra.update({
+            "config1_gpu_fp64_s": gpu1_s,
+            "config1_cpu_oracle_s": cpu1_s,
+            "config1_cpu_threads": O.num_threads(),
+            "config1_speedup_gpu_vs_cpu": cpu1_s / gpu1_s,
+            "config1_max_abs_dN_gpu_vs_cpu": float(d1.max()),
+            **{f"config1_{k}": v for k, v in host_cpu().items()},
+        })
+        log(f"64-node FP64 RK2 to Y = 1: GPU {gpu1_s:.3f} s (solve_bk end to end), CPU oracle {cpu1_s:.2f} s on "
+            f"{O.num_threads()} threads, max |dN| {d1.max():.2e}")
+
     cpu = None
     if not args.no_cpu_baseline:
         # The FP64 oracle (C restatement of the reference CPU algorithm) on a
@@ -574,6 +651,7 @@ def main() -> int:
             "kind": "port",
             "sample": f"{pick.size} active parents x 2^{args.cpu_samples_log2} Sobol points of K2+Kf (FP64 "
                       f"reference formula) plus their K1 (8x24 GL per panel) at Y=0, {cpu_s:.2f} s wall",
+            **host_cpu(),
         }
         log(f"cpu baseline: {cpu['value']:.4e} samples/s on {cpu['cores']} threads")
 
@@ -604,6 +682,9 @@ def main() -> int:
         },
         "evolution_s": fp32["seconds"],
         "adaptation_s": fp32["adapt_seconds"],
+        "evolver_setup_s": fp32["setup_seconds"],
+        "full_evolution_s": fp32["adapt_seconds"] + fp32["setup_seconds"] + fp32["seconds"],
+        "first_setup_in_process_s": first_setup_s,
         "integrand_samples": samples,
         "roofline": {
             "bound": "valu",
@@ -612,7 +693,9 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": measured_traffic(args.rpoints, args.samples_log2),
-            "kernel": "nlo_kernel<FP32> + per-parent reduce",
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of nlo_kernel<FP32, sum only> on this "
+                              "workload, calibrated on a device copy (profiles/nlo_kernel_fp32_256r_counters.md)",
+            "kernel": "nlo_kernel<FP32, sum only> + per-parent reduce (the evolver's instance)",
             "launch_ms": launch_ms,
             "flops_per_sample": FLOPS_PER_SAMPLE,
         },

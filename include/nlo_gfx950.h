@@ -177,6 +177,10 @@ int nlo_bk_spline_build(int32_t R, const double* d_r, const double* d_N, int32_t
 #define NLO_PREC_FP64 0
 #define NLO_PREC_FP32 1
 #define NLO_PREC_FP64_STABLE 2
+/* OR'd into nlo_bk_nlo's precision: only the first column (sum w) is formed
+ * and the other three read 0, as in the evolver's right-hand side (the mean
+ * is all dN/dy needs); not with histograms. */
+#define NLO_BK_SUM_ONLY 0x100
 
 /* K1 on the production fixed rule for every parent r_i, i = 1..R-1, with
  * N_i <= 0.99999 (others: 0): (R-1) log-r panels x GL(nr) in log z x GL(nt)
@@ -226,8 +230,9 @@ int nlo_bk_k1_adaptive(int32_t R, const double* d_r, const double* d_N, const do
  * (S/bk/derivatives.py:424-431).  d_sums: R x 4 doubles
  * (sum w, sum w^2, sum w K2-part, sum w Kf-part); inactive parents get 0.
  * d_hist / d_count (R x 4 x bins, may be NULL) receive the adaptation
- * histograms.  Replaces bk_nlo_mixed_vegas_summaries
- * (K/bk/k2_kf/kernel.cu:877-1062) batched over parents. */
+ * histograms.  precision: NLO_PREC_*, optionally | NLO_BK_SUM_ONLY.
+ * Replaces bk_nlo_mixed_vegas_summaries (K/bk/k2_kf/kernel.cu:877-1062)
+ * batched over parents. */
 size_t nlo_bk_nlo_workspace(int32_t R, int32_t bins, int64_t n, int32_t want_hist);
 int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
                const float* d_table32, const double* d_edges, int32_t bins,

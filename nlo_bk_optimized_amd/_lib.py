@@ -17,6 +17,7 @@ LIBRARY = Path(__file__).resolve().parent / "libnlo_gfx950.so"
 NLO_OK = 0
 NLO_SOBOL_U32, NLO_SOBOL_F32, NLO_SOBOL_F64 = 0, 1, 2
 NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
+NLO_BK_SUM_ONLY = 0x100
 NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4, NLO_RK_ADAPTIVE2 = 0, 1, 2, 3, 4
 NLO_K1_FIXED, NLO_K1_ADAPTIVE = 0, 1
 MAX_GK_INTERVALS = 100

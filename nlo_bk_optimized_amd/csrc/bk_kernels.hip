@@ -1765,9 +1765,12 @@ int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_
                 "nlo_bk_nlo: sample range [%lld, %lld) invalid (first must be a multiple of 64, "
                 "end <= 2^30)",
                 (long long)first_index, (long long)(first_index + n));
+    const bool sum_only = (precision & NLO_BK_SUM_ONLY) != 0;
+    precision &= ~NLO_BK_SUM_ONLY;
     NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32 ||
                     precision == NLO_PREC_FP64_STABLE,
                 "nlo_bk_nlo: unknown precision %d", precision);
+    NLO_REQUIRE(!(sum_only && d_hist), "nlo_bk_nlo: NLO_BK_SUM_ONLY with histograms");
     NLO_REQUIRE(d_r && d_N && d_table64 && d_edges && d_sums, "nlo_bk_nlo: null pointer");
     NLO_REQUIRE(precision != NLO_PREC_FP32 || d_table32, "nlo_bk_nlo: FP32 mode needs d_table32");
     NLO_REQUIRE((d_hist == nullptr) == (d_count == nullptr),
@@ -1776,7 +1779,7 @@ int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_
     if (rc != NLO_OK) return rc;
     rc = nlo::launch_nlo(R, d_r, d_N, d_table64, d_table32, d_edges, bins, *sobol, first_index, n,
                          precision, nlo::make_consts(*physics), d_sums, d_hist, d_count, d_workspace,
-                         workspace_bytes, stream);
+                         workspace_bytes, stream, /*components=*/!sum_only);
     if (rc == NLO_OK) nlo::clear_error();
     return rc;
 }

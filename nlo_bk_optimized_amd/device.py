@@ -258,8 +258,12 @@ def bk_nlo(
     first_index: int = 0,
     precision: str = "fp64",
     histogram: bool = False,
+    sum_only: bool = False,
 ):
-    """Per-parent (sum w, sum w^2, sum w K2, sum w Kf) [R, 4] (+ hist, count)."""
+    """Per-parent (sum w, sum w^2, sum w K2, sum w Kf) [R, 4] (+ hist, count).
+
+    sum_only: only sum w is formed (the other columns read 0), the kernel
+    instance the evolver runs."""
 
     R = r.numel()
     bins = edges.shape[-1] - 1
@@ -274,7 +278,8 @@ def bk_nlo(
     ph = physics(config)
     check(
         lib().nlo_bk_nlo(R, ptr(r), ptr(N), ptr(t64), ptr(t32), ptr(edges), bins, ctypes.byref(table),
-                         first_index, n, PRECISION_CODES[precision], ctypes.byref(ph), ptr(sums),
+                         first_index, n, PRECISION_CODES[precision] | (_lib.NLO_BK_SUM_ONLY if sum_only else 0),
+                         ctypes.byref(ph), ptr(sums),
                          ptr(hist), ptr(count), ptr(ws), nbytes, stream_handle(r.device)),
         "nlo_bk_nlo",
     )

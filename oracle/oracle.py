@@ -679,5 +679,73 @@ def bk_nlo_sums(
     return sums, hist, count
 
 
+# ------------------------------------------------------------ RK driver
+
+# Butcher tableaux of the rapidity steps (S/bk/evolution.py:190-236 for
+# Kutta's third-order step, :265-276 for the midpoint rule; Euler and
+# classical RK4 as the device evolver offers them).
+TABLEAUX = {
+    "euler": ([[]], [1.0]),
+    "rk2": ([[], [0.5]], [0.0, 1.0]),
+    "kutta3": ([[], [0.5], [-1.0, 2.0]], [1 / 6, 4 / 6, 1 / 6]),
+    "rk4": ([[], [0.5], [0.0, 0.5], [0.0, 0.0, 1.0]], [1 / 6, 2 / 6, 2 / 6, 1 / 6]),
+}
+
+
+def rk_step(N: np.ndarray, h: float, method: str, derivative) -> np.ndarray:
+    """One explicit RK step N + h sum_s b_s K_s, K_s = f(N + h sum_j a_sj K_j)."""
+
+    a, b = TABLEAUX[method]
+    K = []
+    for s in range(len(b)):
+        stage = N + h * sum(a[s][j] * K[j] for j in range(len(a[s]))) if s else N
+        K.append(derivative(stage))
+    return N + h * sum(bs * k for bs, k in zip(b, K))
+
+
+def rhs(r, N, edges, table, n, p, *, first=0, nr=8, nt=24, k1_rule="fixed"):
+    """K1 and K2 + Kf per r node (active set of S/bk/evolution.py:501: index
+    >= 1 and N <= 0.99999; others 0): K1 on the fixed rule (nr x nt nodes per
+    log-r panel) or the nested adaptive Gauss-Kronrod rule of
+    rapidity_derivative_lo (85 x 85 intervals, 1e-3), and the VEGAS sum over
+    Sobol points [first, first + n) through each parent's frozen grid."""
+
+    sN, sS = evolution_splines(r, N)
+    active = np.nonzero((np.arange(r.size) >= 1) & (N <= 0.99999))[0]
+    k1 = np.zeros(r.size)
+    k2kf = np.zeros(r.size)
+    if active.size:
+        if k1_rule == "adaptive":
+            k1[active] = k1_adaptive(r[active], sN, p=p)[0]
+        else:
+            k1[active] = k1_fixed(r[active], sN, nr=nr, nt=nt, p=p)
+        lo, width = nlo_bounds(r)
+        spline = sN if p.stable else sS
+        sums, _, _ = bk_nlo_sums(r[active], spline, edges[active], lo, width, table, first, n, p)
+        k2kf[active] = sums[:, 0] / n
+    return k1, k2kf
+
+
+def evolve(r, N0, edges, table, n, p, *, steps, h, method, force_positive=True, k1_rule="fixed"):
+    """Fixed-step rapidity evolution (S/bk/evolution.py:85-159): dN/dy = K1 +
+    K2 + Kf with non-finite values set to 0, clamp to [0, 1] after each step;
+    returns N at every step, [steps + 1, R]."""
+
+    N = np.array(N0, dtype=np.float64)
+    out = [N.copy()]
+
+    def derivative(stage):
+        k1, k2kf = rhs(r, stage, edges, table, n, p, k1_rule=k1_rule)
+        d = k1 + k2kf
+        return np.where(np.isfinite(d), d, 0.0)
+
+    for _ in range(steps):
+        N = rk_step(N, h, method, derivative)
+        if force_positive:
+            N = np.clip(N, 0.0, 1.0)
+        out.append(N.copy())
+    return np.array(out)
+
+
 def num_threads() -> int:
     return int(lib().orc_num_threads())

@@ -30,4 +30,4 @@ def test_flag_defaults(monkeypatch):
     assert (a.gpus, a.steps, a.warmup, a.samples_log2, a.rpoints) == (1, 100, 2, 20, 255)
     assert a.large_maxy == 0.0 and a.large_precision == "both"
     assert a.batch_runs == 64 and a.dis_points == 32 and not a.kernel_probe
-    assert bench.FLOPS_PER_SAMPLE == 253
+    assert not a.no_config1 and not a.no_cpu_baseline

@@ -13,9 +13,10 @@ numpy stages over the C oracle's right-hand side).
 * ``compute_dndy`` (S/bk/evolution.py:61-82) against the oracle's
   right-hand side.
 
-Tolerances: 1e-10 relative in FP64 (the cancellation-free formula); the
-reference K2/Kf formula additionally within the rounding scale it carries at
-r -> 1e-6, which for amplitudes of order 1e-12 is below 1e-16 absolute.
+Tolerances: 1e-10 relative in FP64 with the cancellation-free formula; the
+reference K2/Kf formula additionally within the rounding scale it carries
+where its terms cancel (unit roundoff times the integrated sum of |terms| the
+oracle reports, per right-hand side).
 """
 
 from __future__ import annotations
@@ -43,29 +44,61 @@ def cuda(a, dtype=torch.float64):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype, device="cuda")
 
 
-def test_64_node_fp64_rk2_to_y1_matches_oracle(dev):
-    """R = 64, 2^14 Sobol points (seed 0), plain FP64 (the reference K2/Kf
-    formula on the S spline), midpoint RK2, dY = 0.1, Y: 0 -> 1."""
+def reference_formula_scale(r, N0, edges, n, p):
+    """Per node, the rounding scale of one right-hand side on the reference
+    K2/Kf formula: unit roundoff times the integrated sum of |terms| / n that
+    the oracle reports (its terms cancel by up to 1e12 at r ~ 1e-3 and more
+    below, so two evaluation orders agree only to this scale there)."""
+
+    _, sS = O.evolution_splines(r, N0)
+    lo, width = O.nlo_bounds(r)
+    act = np.nonzero((np.arange(r.size) >= 1) & (N0 <= 0.99999))[0]
+    mag = np.zeros(r.size)
+    mag[act] = O.bk_nlo_sums(r[act], sS, edges[act], lo, width, O.sobol_table(4, 0), 0, n, p,
+                             want_magnitude=True)[3]
+    return 2.0**-52 * mag / n
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_stable"])
+def test_64_node_fp64_rk2_to_y1_matches_oracle(dev, precision):
+    """R = 64, 2^14 Sobol points (seed 0), midpoint RK2, dY = 0.1, Y: 0 -> 1,
+    through solve_bk.  "fp64" is the plain FP64 path (the reference K2/Kf
+    formula on the S spline): below r ~ 1e-3 its terms cancel by many orders
+    of magnitude, so GPU and oracle (different operation order) agree there
+    only to the formula's own rounding scale (unit roundoff times the
+    integrated sum of |terms| per right-hand side), accumulated over the
+    steps.  "fp64_stable" (the FP32 path's cancellation-free formula in FP64)
+    agrees to 1e-10 relative everywhere."""
 
     from nlo_bk_optimized_amd import MV, BKConfig, solve_bk
     from nlo_bk_optimized_amd.bk import evolution as evo
 
-    cfg = BKConfig(RPOINTS=63, MINR=1e-6, MAXR=1e2, MCINTPOINTS=1 << 14, DE_SOLVER_STEP=0.1, RK_METHOD="rk2",
-                   K1_RULE="fixed", PRECISION="fp64", LambdaQCD=0.241, NF=3)
+    n, steps, h = 1 << 14, 10, 0.1
+    cfg = BKConfig(RPOINTS=63, MINR=1e-6, MAXR=1e2, MCINTPOINTS=n, DE_SOLVER_STEP=h, RK_METHOD="rk2",
+                   K1_RULE="fixed", PRECISION=precision, LambdaQCD=0.241, NF=3)
     ic = MV(qs0sqr=0.2, gamma=1.0, ec=1.0)
-    print("64-node FP64 RK2 to Y = 1 through solve_bk", flush=True)
+    print(f"64-node {precision} RK2 to Y = 1 through solve_bk", flush=True)
     table = solve_bk(ic, 1.0, cfg, device="cuda", dtype=torch.float64, seed=0)
     assert table.N.shape == (11, 64)
     np.testing.assert_allclose(table.y.cpu().numpy(), np.arange(11) * 0.1, rtol=0, atol=1e-15)
     setup = evo.prepare(ic, cfg, device=torch.device("cuda"), seed=0)  # deterministic: solve_bk's grids
     r, N0, edges = (x.cpu().numpy() for x in (setup.r, setup.N0, setup.edges))
-    ref = oracle_evolve(r, N0, edges, O.sobol_table(4, 0), 1 << 14, bk_params(False), steps=10, h=0.1,
-                        method="rk2")
+    stable = precision == "fp64_stable"
+    ref = oracle_evolve(r, N0, edges, O.sobol_table(4, 0), n, bk_params(stable), steps=steps, h=h, method="rk2")
     got = table.N.cpu().numpy()
     err = np.abs(got - ref)
-    bad = err > 1e-10 * np.abs(ref) + 1e-16
-    print(f"max |dN| {err.max():.3e}, max rel {np.max(err / np.maximum(np.abs(ref), 1e-300)):.3e}, "
-          f"N(Y=1) range [{ref[-1].min():.3e}, {ref[-1].max():.3e}]")
+    sN, sS = O.evolution_splines(r, N0)
+    act = np.nonzero((np.arange(r.size) >= 1) & (N0 <= 0.99999))[0]
+    # K1: 1e-14 of its integrated |integrand| per right-hand side (its own cancellation at small r)
+    k1mag = np.zeros(r.size)
+    k1mag[act] = O.k1_fixed(r[act], sN, p=bk_params(stable), magnitude=True)
+    tol = 1e-10 * np.abs(ref) + steps * h * 2.0 * 1e-14 * k1mag + 1e-300
+    if not stable:
+        # the scale grows with N over the evolution: twice the initial one, per step
+        tol = tol + steps * h * 2.0 * reference_formula_scale(r, N0, edges, n, bk_params(False))
+    bad = err > tol
+    print(f"max |dN| {err.max():.3e}; max rel {np.max(err / np.maximum(np.abs(ref), 1e-300)):.3e}; "
+          f"max err / tol {np.max(err / tol):.3e}")
     assert not bad.any(), (np.argwhere(bad)[:5], err[bad][:5], ref[bad][:5])
 
 
@@ -75,7 +108,7 @@ def test_evolver_methods_match_oracle(dev, method, precision):
     from nlo_bk_optimized_amd.bk.config import BKConfig
 
     R, n, bins, steps, h = 12, 2048, 50, 3, 0.2
-    r, N0 = workload(R=R, minr=1e-3)  # the reference formula is well conditioned for r >= 1e-3
+    r, N0 = workload(R=R, minr=1e-3)
     edges = random_edges(R, bins, seed=7)
     config = BKConfig(MINR=1e-3)
     table = dev.sobol_table(4, 0)
@@ -91,7 +124,12 @@ def test_evolver_methods_match_oracle(dev, method, precision):
     ev.close()
     p = O.params(minr=1e-3, stable=precision == "fp64_stable")
     ref = oracle_evolve(r, N0, edges, O.sobol_table(4, 0), n, p, steps=steps, h=h, method=method)
-    np.testing.assert_allclose(np.array(got), ref, rtol=1e-10, atol=1e-18)
+    tol = 1e-10 * np.abs(ref) + 1e-18
+    if precision == "fp64":
+        tol = tol + steps * h * 2.0 * reference_formula_scale(r, N0, edges, n, p)
+    err = np.abs(np.array(got) - ref)
+    print(f"max rel {np.max(err / np.maximum(np.abs(ref), 1e-300)):.3e}, max err / tol {np.max(err / tol):.3e}")
+    assert np.all(err <= tol), (np.argwhere(err > tol)[:5], err[err > tol][:5])
     # the methods differ: a wrong tableau cannot hide behind another one
     if method != "euler":
         other = oracle_evolve(r, N0, edges, O.sobol_table(4, 0), n, p, steps=1, h=h, method="euler")

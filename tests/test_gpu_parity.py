@@ -225,6 +225,28 @@ def test_nlo_fp32_within_tolerance_of_fp64_oracle(dev):
     assert np.all(err <= 2e-5), err
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "fp64_stable"])
+def test_nlo_sum_only_equals_first_column(dev, precision):
+    """NLO_BK_SUM_ONLY (the evolver's instance) forms sum w exactly as the full
+    instance does (same order): bit for bit, the other columns 0."""
+
+    from nlo_bk_optimized_amd.bk.config import BKConfig
+
+    R, n = 24, 4096 + 640
+    r, N = workload(R=R)
+    edges = cuda(random_edges(R, 50, seed=5))
+    rt, Nt = cuda(r), cuda(N)
+    t64, t32 = dev.spline_build(rt, Nt)
+    table = dev.sobol_table(4, 2)
+    print(f"nlo sum only {precision}", flush=True)
+    full, _, _ = dev.bk_nlo(rt, Nt, t64, t32, edges, table, n, BKConfig(MINR=1e-6), precision=precision)
+    only, _, _ = dev.bk_nlo(rt, Nt, t64, t32, edges, table, n, BKConfig(MINR=1e-6), precision=precision,
+                            sum_only=True)
+    assert torch.equal(only[:, 0], full[:, 0])
+    assert torch.count_nonzero(only[:, 1:]) == 0
+    assert torch.count_nonzero(full[:, 0]) > 0
+
+
 def test_nlo_histogram_matches_oracle(dev):
     active, sums, hist, count, o = _nlo_case(dev, "fp64_stable", histogram=True, R=24)
     _, oh, oc, _ = o["stable"]
