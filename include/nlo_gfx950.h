@@ -240,6 +240,22 @@ int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_
                const nlo_bk_physics* physics, double* d_sums, double* d_hist, double* d_count,
                void* d_workspace, size_t workspace_bytes, hipStream_t stream);
 
+/* nlo_bk_nlo without a workspace and without any allocation: no counterpart
+ * in the reference; for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as nlo_bk_nlo without histograms
+ * (precision may carry NLO_BK_SUM_ONLY); the per-block partial sums live in a
+ * static device array of NLO_BK_NOALLOC_SCRATCH_BYTES, one per GPU, held until
+ * the process ends.  A workload needing more (nlo_bk_nlo_workspace(R, bins,
+ * n, 0) > NLO_BK_NOALLOC_SCRATCH_BYTES, e.g. R = 1024 with n = 2^22) is
+ * refused with NLO_ERR_WORKSPACE.  Calls that overlap on one GPU (other
+ * streams, other threads) share that array and race; serialise them. */
+#define NLO_BK_NOALLOC_SCRATCH_BYTES (16u << 20)
+int nlo_bk_nlo_noalloc(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+                       const float* d_table32, const double* d_edges, int32_t bins,
+                       const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision,
+                       const nlo_bk_physics* physics, double* d_sums, hipStream_t stream);
+
 /* dN/dy = K1 + sum_w / n for active parents, 0 elsewhere, non-finite -> 0
  * (S/bk/evolution.py:340-351).  d_k2kf (R, may be NULL) receives sum_w / n. */
 int nlo_bk_rhs_finalize(int32_t R, const double* d_N, const double* d_k1, const double* d_sums,

@@ -18,6 +18,7 @@ NLO_OK = 0
 NLO_SOBOL_U32, NLO_SOBOL_F32, NLO_SOBOL_F64 = 0, 1, 2
 NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
 NLO_BK_SUM_ONLY = 0x100
+NLO_BK_NOALLOC_SCRATCH_BYTES = 16 << 20
 NLO_RK_EULER, NLO_RK_MIDPOINT, NLO_RK_KUTTA3, NLO_RK_CLASSIC4, NLO_RK_ADAPTIVE2 = 0, 1, 2, 3, 4
 NLO_K1_FIXED, NLO_K1_ADAPTIVE = 0, 1
 MAX_GK_INTERVALS = 100
@@ -226,6 +227,11 @@ SIGNATURES: dict[str, tuple] = {
             _size,
             _vp,
         ],
+    ),
+    "nlo_bk_nlo_noalloc": (
+        ctypes.c_int,
+        [_i32, _vp, _vp, _vp, _vp, _vp, _i32, ctypes.POINTER(SobolTable), _i64, _i64, _i32,
+         ctypes.POINTER(Physics), _vp, _vp],
     ),
     "nlo_bk_rhs_finalize": (ctypes.c_int, [_i32, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "nlo_bk_evolver_workspace": (_size, [ctypes.POINTER(EvolverDesc)]),

@@ -1662,6 +1662,41 @@ int launch_rhs_finalize_batch(const BatchView& b, const double* N, const double*
     return NLO_OK;
 }
 
+// ------------------------------------------- no-allocation K2 + Kf entry
+
+// Scratch of nlo_bk_nlo_noalloc: the per-block partial sums of the K2 + Kf
+// launch, (R - 1) x blocks-per-parent x 4 doubles, every element written by
+// its block before the reduce reads it (nothing has to start at zero).
+// File-scope device memory: each GPU the library runs on holds its own copy
+// until the process ends.  Host code never hands `scratch` itself to a
+// kernel or a copy: it uses the device address scratch_address() looked up
+// for the current device.
+static __device__ __align__(256) unsigned char scratch[NLO_BK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex scratch_mutex;
+void* scratch_ptr[kMaxScratchDevices] = {};
+}  // namespace
+
+// The current device's address of `scratch`: looked up once per device, with
+// that device current, on its first use, and kept.
+int scratch_address(void** out) {
+    int dev = -1;
+    NLO_HIP_CHECK(hipGetDevice(&dev));
+    NLO_REQUIRE(dev >= 0 && dev < kMaxScratchDevices,
+                "nlo_bk_nlo_noalloc: device %d beyond the %d supported", dev, kMaxScratchDevices);
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void* p = nullptr;
+        NLO_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        NLO_REQUIRE(p != nullptr, "nlo_bk_nlo_noalloc: no device address for the scratch array");
+        scratch_ptr[dev] = p;
+    }
+    *out = scratch_ptr[dev];
+    return NLO_OK;
+}
+
 int launch_fill_i32(int32_t* p, int32_t v, int n, hipStream_t stream) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, p, v, n);
     NLO_LAUNCH_CHECK();
@@ -1752,34 +1787,80 @@ size_t nlo_bk_nlo_workspace(int32_t R, int32_t bins, int64_t n, int32_t want_his
     return nlo::nlo_plan(R, bins, n, want_hist != 0).workspace;
 }
 
+}  // extern "C"
+
+namespace nlo {
+namespace {
+
+// Argument checks shared by nlo_bk_nlo and nlo_bk_nlo_noalloc (fn names the
+// entry point in the messages); precision arrives without NLO_BK_SUM_ONLY.
+int check_nlo_args(const char* fn, int32_t R, const double* d_r, const double* d_N,
+                   const double* d_table64, const float* d_table32, const double* d_edges,
+                   int32_t bins, const nlo_sobol_table* sobol, int64_t first_index, int64_t n,
+                   int32_t precision, const nlo_bk_physics* physics, const double* d_sums) {
+    NLO_REQUIRE(R >= 3 && R <= kMaxR, "%s: R=%d outside [3, %d]", fn, R, kMaxR);
+    NLO_REQUIRE(bins >= 2 && bins <= 255, "%s: bins=%d outside [2, 255]", fn, bins);
+    NLO_REQUIRE(sobol && sobol->dim == 4, "%s: a 4-dimensional Sobol table is required", fn);
+    NLO_REQUIRE(n >= 1 && first_index >= 0 && first_index % 64 == 0 &&
+                    first_index + n <= (int64_t(1) << 30),
+                "%s: sample range [%lld, %lld) invalid (first must be a multiple of 64, "
+                "end <= 2^30)",
+                fn, (long long)first_index, (long long)(first_index + n));
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32 ||
+                    precision == NLO_PREC_FP64_STABLE,
+                "%s: unknown precision %d", fn, precision);
+    NLO_REQUIRE(d_r && d_N && d_table64 && d_edges && d_sums, "%s: null pointer", fn);
+    NLO_REQUIRE(precision != NLO_PREC_FP32 || d_table32, "%s: FP32 mode needs d_table32", fn);
+    return validate_physics(physics);
+}
+
+}  // namespace
+}  // namespace nlo
+
+extern "C" {
+
 int nlo_bk_nlo(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
                const float* d_table32, const double* d_edges, int32_t bins,
                const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision,
                const nlo_bk_physics* physics, double* d_sums, double* d_hist, double* d_count,
                void* d_workspace, size_t workspace_bytes, hipStream_t stream) {
-    NLO_REQUIRE(R >= 3 && R <= nlo::kMaxR, "nlo_bk_nlo: R=%d outside [3, %d]", R, nlo::kMaxR);
-    NLO_REQUIRE(bins >= 2 && bins <= 255, "nlo_bk_nlo: bins=%d outside [2, 255]", bins);
-    NLO_REQUIRE(sobol && sobol->dim == 4, "nlo_bk_nlo: a 4-dimensional Sobol table is required");
-    NLO_REQUIRE(n >= 1 && first_index >= 0 && first_index % 64 == 0 &&
-                    first_index + n <= (int64_t(1) << 30),
-                "nlo_bk_nlo: sample range [%lld, %lld) invalid (first must be a multiple of 64, "
-                "end <= 2^30)",
-                (long long)first_index, (long long)(first_index + n));
     const bool sum_only = (precision & NLO_BK_SUM_ONLY) != 0;
     precision &= ~NLO_BK_SUM_ONLY;
-    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32 ||
-                    precision == NLO_PREC_FP64_STABLE,
-                "nlo_bk_nlo: unknown precision %d", precision);
+    int rc = nlo::check_nlo_args("nlo_bk_nlo", R, d_r, d_N, d_table64, d_table32, d_edges, bins, sobol,
+                                 first_index, n, precision, physics, d_sums);
+    if (rc != NLO_OK) return rc;
     NLO_REQUIRE(!(sum_only && d_hist), "nlo_bk_nlo: NLO_BK_SUM_ONLY with histograms");
-    NLO_REQUIRE(d_r && d_N && d_table64 && d_edges && d_sums, "nlo_bk_nlo: null pointer");
-    NLO_REQUIRE(precision != NLO_PREC_FP32 || d_table32, "nlo_bk_nlo: FP32 mode needs d_table32");
     NLO_REQUIRE((d_hist == nullptr) == (d_count == nullptr),
                 "nlo_bk_nlo: d_hist and d_count must be given together");
-    int rc = nlo::validate_physics(physics);
-    if (rc != NLO_OK) return rc;
     rc = nlo::launch_nlo(R, d_r, d_N, d_table64, d_table32, d_edges, bins, *sobol, first_index, n,
                          precision, nlo::make_consts(*physics), d_sums, d_hist, d_count, d_workspace,
                          workspace_bytes, stream, /*components=*/!sum_only);
+    if (rc == NLO_OK) nlo::clear_error();
+    return rc;
+}
+
+int nlo_bk_nlo_noalloc(int32_t R, const double* d_r, const double* d_N, const double* d_table64,
+                       const float* d_table32, const double* d_edges, int32_t bins,
+                       const nlo_sobol_table* sobol, int64_t first_index, int64_t n, int32_t precision,
+                       const nlo_bk_physics* physics, double* d_sums, hipStream_t stream) {
+    const bool sum_only = (precision & NLO_BK_SUM_ONLY) != 0;
+    precision &= ~NLO_BK_SUM_ONLY;
+    int rc = nlo::check_nlo_args("nlo_bk_nlo_noalloc", R, d_r, d_N, d_table64, d_table32, d_edges, bins,
+                                 sobol, first_index, n, precision, physics, d_sums);
+    if (rc != NLO_OK) return rc;
+    const size_t need = nlo::nlo_plan(R, bins, n, false).workspace;
+    if (need > (size_t)NLO_BK_NOALLOC_SCRATCH_BYTES) {
+        nlo::set_error("nlo_bk_nlo_noalloc: R=%d with n=%lld needs %zu bytes of partial sums, more than the "
+                       "%zu-byte static scratch; use nlo_bk_nlo with a workspace",
+                       R, (long long)n, need, (size_t)NLO_BK_NOALLOC_SCRATCH_BYTES);
+        return NLO_ERR_WORKSPACE;
+    }
+    void* ws = nullptr;
+    rc = nlo::scratch_address(&ws);
+    if (rc != NLO_OK) return rc;
+    rc = nlo::launch_nlo(R, d_r, d_N, d_table64, d_table32, d_edges, bins, *sobol, first_index, n,
+                         precision, nlo::make_consts(*physics), d_sums, nullptr, nullptr, ws,
+                         (size_t)NLO_BK_NOALLOC_SCRATCH_BYTES, stream, /*components=*/!sum_only);
     if (rc == NLO_OK) nlo::clear_error();
     return rc;
 }
