@@ -83,6 +83,8 @@ def parse() -> argparse.Namespace:
                    help="evolutions in the batched uneven-grid run (0 skips it)")
     p.add_argument("--dis-points", type=int, default=32,
                    help="(Q^2, x) points of the NLO DIS batch (512 = the full 32 x 16 grid; 0 skips it)")
+    p.add_argument("--dis-frozen-points", type=int, default=32,
+                   help="points of the DIS batch whose terms are also compared FP32 vs FP64 through one frozen grid")
     p.add_argument("--kernel-probe", action="store_true",
                    help="only run the K2+Kf launch and a 256 MiB device copy (for counter runs)")
     p.add_argument("--probe-precision", default="fp32", choices=("fp32", "fp64", "fp64_stable"),
@@ -296,7 +298,7 @@ def main() -> int:
 
         from nlo_bk_optimized_amd import solve_bk
         from nlo_bk_optimized_amd.dipole import BKDipole
-        from nlo_bk_optimized_amd.dis import FL, FT
+        from nlo_bk_optimized_amd.dis import structure_functions_batch
         from nlo_bk_optimized_amd.dis.config import DISConfig, DISOrder, Quark, QuarkType
 
         log("DIS batch: BK table to Y = 12 (untimed)")
@@ -309,19 +311,18 @@ def main() -> int:
         points = [(q, x) for q in qs_all[::step] for x in xs_all[::step]]
         res = {}
         for dtype, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+            # every integral of every point (FT with seed 0, FL with seed
+            # 1000, as F2 seeds them) in one device batch
+            pts = [(torch.tensor(q2, dtype=dtype), torch.tensor(xb, dtype=dtype)) for q2, xb in points]
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            vals = []
-            for k, (q2, xb) in enumerate(points):
-                Q2, xbj = torch.tensor(q2, dtype=dtype), torch.tensor(xb, dtype=dtype)
-                ft = FT(Q2, xbj, dipole, dcfg, seed=0)
-                fl = FL(Q2, xbj, dipole, dcfg, seed=1000)
-                vals.append((ft.value.item(), fl.value.item(), ft.error.item(), fl.error.item()))
-                if (k + 1) % 64 == 0 and k + 1 < len(points):
-                    log(f"DIS batch {name}: {k + 1} of {len(points)} points")
+            pairs = structure_functions_batch(pts, dipole, dcfg, seed=0)
             torch.cuda.synchronize()
-            res[name] = (np.array(vals), time.perf_counter() - t0)
-            log(f"DIS batch {name}: {len(points)} points in {res[name][1]:.2f} s")
+            wall = time.perf_counter() - t0
+            vals = [(ft.value.item(), fl.value.item(), ft.error.item(), fl.error.item()) for ft, fl in pairs]
+            evals = sum(ft.n_eval + fl.n_eval for ft, fl in pairs)
+            res[name] = (np.array(vals), wall, evals)
+            log(f"DIS batch {name}: {len(points)} points in {wall:.2f} s ({evals:.3e} integrand evaluations)")
         v32, v64 = res["fp32"][0], res["fp64"][0]
         f2_32, f2_64 = v32[:, 0] + v32[:, 1], v64[:, 0] + v64[:, 1]
         # Integrand precision alone: one VEGAS pass of every VEGAS term (dipole
@@ -332,7 +333,7 @@ def main() -> int:
         from nlo_bk_optimized_amd import _lib
 
         frozen, worst = [], None
-        for k, (q2, xb) in enumerate(points):
+        for k, (q2, xb) in enumerate(points[:args.dis_frozen_points]):
             if k % 64 == 0 and k > 0:
                 log(f"DIS frozen-grid comparison: {k} of {len(points)} points")
             for transverse in (True, False):
@@ -354,20 +355,25 @@ def main() -> int:
                         if worst is None or row[1] > worst[0][1]:
                             worst = (row, f"Q2={q2:.3g} x={xb:.3g} {'T' if transverse else 'L'} "
                                           f"m={quark.mass} {kind}")
-        frozen = np.array(frozen)
+        frozen = np.array(frozen).reshape(-1, 2)
+        if not len(frozen):  # no frozen-grid comparison requested
+            frozen = np.full((1, 2), np.nan)
+            worst = (None, None)
         q2a = np.array([p[0] for p in points])
         xa = np.array([p[1] for p in points])
         yv = q2a / (xa * 318.0**2)
         ok = yv <= 1
         c = 1 - yv**2 / (1 + (1 - yv) ** 2)
         sr32, sr64 = v32[:, 0] + c * v32[:, 1], v64[:, 0] + c * v64[:, 1]
-        integrals = len(points) * 2 * len(dcfg.quarks) * 5  # VEGAS integrals per precision
         extra.update({
             "dis_batch_points": len(points),
             "dis_batch_fp32_s": res["fp32"][1],
             "dis_batch_fp64_s": res["fp64"][1],
             "dis_batch_speedup_fp32_vs_fp64": res["fp64"][1] / res["fp32"][1],
-            "dis_batch_fp32_samples_per_s": integrals * (1 << 22) / res["fp32"][1],
+            "dis_batch_fp32_evaluations": res["fp32"][2],
+            "dis_batch_fp32_samples_per_s": res["fp32"][2] / res["fp32"][1],
+            "dis_batch_fp64_samples_per_s": res["fp64"][2] / res["fp64"][1],
+            "dis_frozen_grid_points": min(len(points), args.dis_frozen_points),
             "dis_max_rel_dF2_fp32_vs_fp64": float(np.max(np.abs(f2_32 - f2_64) / np.abs(f2_64))),
             "dis_max_rel_dFL_fp32_vs_fp64": float(np.max(np.abs(v32[:, 1] - v64[:, 1]) / np.abs(v64[:, 1]))),
             "dis_max_rel_dsigma_r_fp32_vs_fp64": (float(np.max(np.abs(sr32 - sr64)[ok] / np.abs(sr64[ok])))

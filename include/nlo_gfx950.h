@@ -505,6 +505,56 @@ int nlo_dis_vegas(const nlo_dis_desc* desc, int32_t contribution, int32_t bins, 
 int nlo_dis_gauss_legendre(const nlo_dis_desc* desc, int32_t contribution, int32_t points,
                            int32_t precision, double* d_out, hipStream_t stream);
 
+/* ------------------------------------------------------------ DIS batches */
+
+/* Many DIS integrals in one call: the integrals of any number of (Q^2, x)
+ * points, polarisations, quarks and terms, each with the schedule the
+ * reference gives it (vegas() with its own grid, samples per iteration,
+ * iteration limits and seed: S/numerics/integration.py:500-589 as driven by
+ * _vegas_with_budget, S/dis/observables.py:661-686; or the tensor
+ * Gauss-Legendre rule with its half-order error, :95-117).  Each pass over
+ * all VEGAS integrals is one launch per contribution kind from a device
+ * work-list of (integral, block); one iteration (passes, fixed-order
+ * histogram reductions, grid refinement, estimate / variance, the
+ * inverse-variance combination and the convergence test) is one captured
+ * graph replayed on the device with no host synchronisation; converged
+ * integrals skip the remaining iterations.  Replaces the per-point Python
+ * loops of photon_proton_cross_section_d2b / F2 (S/dis/observables.py:387-624)
+ * batched over points. */
+#define NLO_DIS_METHOD_VEGAS 0
+#define NLO_DIS_METHOD_GAUSS_LEGENDRE 1
+
+typedef struct nlo_dis_item {
+    nlo_dis_desc desc;        /* point, polarisation, quark(s), couplings, dipole */
+    int32_t contribution;     /* NLO_DIS_* */
+    int32_t method;           /* NLO_DIS_METHOD_* (Gauss-Legendre: 2-D contributions) */
+    int64_t n;                /* VEGAS: samples per iteration (>= 2); Gauss-Legendre: nodes per axis (4..128) */
+    int32_t sobol_table;      /* VEGAS: index into nlo_dis_batch_desc.tables (the contribution's dimension) */
+    int32_t max_iterations;   /* VEGAS: 1..16 */
+    int32_t min_iterations;   /* VEGAS: 1..max_iterations */
+    int32_t reserved;
+    int64_t first_index;      /* VEGAS: first Sobol index of iteration 0 (a multiple of 64); iteration k
+                                 starts at first_index + k * (n rounded up to a multiple of 64) */
+} nlo_dis_item;
+
+typedef struct nlo_dis_batch_desc {
+    int32_t n_items;
+    int32_t n_tables;
+    const nlo_dis_item* items;      /* host array */
+    const nlo_sobol_table* tables;  /* host array */
+    const double* d_initial_edges;  /* device, bins + 1: every VEGAS grid's starting edges per dimension */
+    int32_t bins;                   /* VEGAS bins per dimension, 2..255 */
+    int32_t precision;              /* NLO_PREC_FP32 or NLO_PREC_FP64 (the dtype of Q^2); grid smoothing
+                                       and the variance floor take that dtype's epsilon and tiny */
+    double epsrel, epsabs;          /* converged: error <= max(epsabs, epsrel |value|) */
+} nlo_dis_batch_desc;
+
+size_t nlo_dis_batch_workspace(const nlo_dis_batch_desc* desc);
+/* d_results: n_items x 4 doubles (value, error, integrand evaluations,
+ * converged 0 / 1).  Returns after the stream has finished the batch. */
+int nlo_dis_batch_run(const nlo_dis_batch_desc* desc, double* d_results, void* d_workspace,
+                      size_t workspace_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

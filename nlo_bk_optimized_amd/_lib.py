@@ -32,6 +32,8 @@ SOBOL_MAXDIM = 8
 SOBOL_BITS = 30
 NLO_FN_K0, NLO_FN_K1, NLO_FN_K2, NLO_FN_LI2 = 0, 1, 2, 3
 NLO_DIPOLE_GBW, NLO_DIPOLE_BK_TABLE = 0, 1
+NLO_DIS_METHOD_VEGAS, NLO_DIS_METHOD_GAUSS_LEGENDRE = 0, 1
+DIS_BATCH_MAX_ITERATIONS = 16
 DIS_KINDS = {
     "LO": 0, "T0": 1, "T1": 2, "T2": 3, "Omega_L_const": 4, "ab": 5, "cd": 6,
     "I1": 7, "I2": 8, "I3": 9, "I2_nested": 10, "I3_nested": 11,
@@ -177,6 +179,38 @@ class DisDesc(ctypes.Structure):
     ]
 
 
+class DisItem(ctypes.Structure):
+    """nlo_dis_item."""
+
+    _fields_ = [
+        ("desc", DisDesc),
+        ("contribution", ctypes.c_int32),
+        ("method", ctypes.c_int32),
+        ("n", ctypes.c_int64),
+        ("sobol_table", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32),
+        ("min_iterations", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("first_index", ctypes.c_int64),
+    ]
+
+
+class DisBatchDesc(ctypes.Structure):
+    """nlo_dis_batch_desc."""
+
+    _fields_ = [
+        ("n_items", ctypes.c_int32),
+        ("n_tables", ctypes.c_int32),
+        ("items", ctypes.POINTER(DisItem)),
+        ("tables", ctypes.POINTER(SobolTable)),
+        ("d_initial_edges", ctypes.c_void_p),
+        ("bins", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("epsrel", ctypes.c_double),
+        ("epsabs", ctypes.c_double),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -276,6 +310,8 @@ SIGNATURES: dict[str, tuple] = {
          _vp, _vp, _size, _vp],
     ),
     "nlo_dis_gauss_legendre": (ctypes.c_int, [ctypes.POINTER(DisDesc), _i32, _i32, _i32, _vp, _vp]),
+    "nlo_dis_batch_workspace": (_size, [ctypes.POINTER(DisBatchDesc)]),
+    "nlo_dis_batch_run": (ctypes.c_int, [ctypes.POINTER(DisBatchDesc), _vp, _vp, _size, _vp]),
 }
 
 _lib: ctypes.CDLL | None = None

@@ -648,6 +648,49 @@ def dis_gauss_legendre(desc: _lib.DisDesc, kind: str, points: int, *, precision:
     return out
 
 
+def dis_batch(items, tables, *, bins: int, precision: str, epsrel: float, epsabs: float = 0.0,
+              device: torch.device | str = "cuda", workspace_budget: int = 8 << 30) -> torch.Tensor:
+    """nlo_dis_batch_run over ``items`` (nlo_dis_item) with Sobol ``tables``:
+    [n_items, 4] float64 (value, error, evaluations, converged) on the device.
+    Items are independent, so a batch whose workspace would exceed
+    ``workspace_budget`` bytes runs as consecutive sub-batches."""
+
+    device = torch.device(device)
+    edges0 = torch.linspace(0, 1, bins + 1, dtype=torch.float64, device=device)
+    tarr = (_lib.SobolTable * max(1, len(tables)))(*tables)
+    code = PRECISION_CODES[precision]
+
+    def describe(part) -> _lib.DisBatchDesc:
+        d = _lib.DisBatchDesc()
+        d.n_items = len(part)
+        d.n_tables = len(tables)
+        d.items = (_lib.DisItem * len(part))(*part)
+        d.tables = tarr
+        d.d_initial_edges = ptr(edges0)
+        d.bins = bins
+        d.precision = code
+        d.epsrel, d.epsabs = float(epsrel), float(epsabs)
+        return d
+
+    out = torch.empty((len(items), 4), dtype=torch.float64, device=device)
+    start = 0
+    while start < len(items):
+        count = len(items) - start
+        while True:
+            d = describe(items[start:start + count])
+            nbytes = lib().nlo_dis_batch_workspace(ctypes.byref(d))
+            if nbytes <= workspace_budget or count == 1:
+                break
+            count = max(1, count // 2)
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        view = out[start:start + count]
+        check(lib().nlo_dis_batch_run(ctypes.byref(d), ptr(view), ptr(ws), nbytes, stream_handle(device)),
+              "nlo_dis_batch_run")
+        del ws
+        start += count
+    return out
+
+
 def dtype_epsilon(dtype: torch.dtype) -> float:
     return float(torch.finfo(dtype).eps)
 

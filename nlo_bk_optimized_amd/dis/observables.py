@@ -38,7 +38,7 @@ from nlo_bk_optimized_amd.dis.config import (
     Quark,
     RunningCouplingScheme,
 )
-from nlo_bk_optimized_amd.numerics.integration import IntegralResult, gauss_legendre_device, vegas_device
+from nlo_bk_optimized_amd.numerics.integration import IntegralResult
 
 ZMIN = 1e-6
 Dipole = GBW | BKDipole
@@ -211,86 +211,212 @@ def _sum_results(results, seed) -> IntegralResult:
     return IntegralResult(value, error, sum(r.n_eval for r in results), all(r.converged for r in results), seed)
 
 
-def _vegas_with_budget(desc, kind: str, maxeval: int, epsrel: float, seed: int, config: DISConfig,
-                       precision: str, device) -> IntegralResult:
-    """S/dis/observables.py:661-686: up to 8 iterations of maxeval / iterations samples."""
+class _Plan:
+    """The integrals of one or more cross sections, collected in the order the
+    per-point composition of the reference makes them and run as one device
+    batch (nlo_dis_batch_run): every VEGAS integral keeps its own grid, seed,
+    sample budget and iteration schedule, so the results do not depend on
+    what else is in the batch."""
 
-    if maxeval < 2:
-        raise ValueError("NLO integration requires maxeval >= 2")
-    max_iterations = min(8, max(1, maxeval // 2))
-    samples = max(2, maxeval // max_iterations)
-    return vegas_device(desc, kind, samples_per_iteration=samples, max_iterations=max_iterations,
-                        min_iterations=min(3, max_iterations), bins=config.bins, epsrel=epsrel, seed=seed,
-                        precision=precision, first_index=config.sobol_first_index, device=device)
+    def __init__(self, dipole: Dipole, config: DISConfig, precision: str, device: torch.device) -> None:
+        self.dipole = dipole
+        self.config = config
+        self.precision = precision
+        self.device = device
+        self.items: list = []
+        self.seeds: list[int | None] = []
+        self.tables: list = []
+        self._table_index: dict[tuple[int, int], int] = {}
+        self._keep: list[torch.Tensor] = []  # device arrays the items point to (BK initialisations)
+
+    def _desc(self, Q2, xbj, polarization: Polarization, quarks):
+        desc = _desc(Q2, xbj, polarization, tuple(quarks), self.dipole, self.config)
+        coef = getattr(self.dipole, "device_coef", None)
+        if coef is not None:
+            self._keep.append(coef)
+        return desc
+
+    def _add(self, item, seed) -> int:
+        self.items.append(item)
+        self.seeds.append(seed)
+        return len(self.items) - 1
+
+    def gauss_legendre(self, Q2, xbj, polarization, quarks, kind: str, points: int) -> int:
+        """tensor_gauss_legendre (S/numerics/integration.py:95-117)."""
+
+        from nlo_bk_optimized_amd import _lib
+
+        item = _lib.DisItem()
+        item.desc = self._desc(Q2, xbj, polarization, quarks)
+        item.contribution = _lib.DIS_KINDS[kind]
+        item.method = _lib.NLO_DIS_METHOD_GAUSS_LEGENDRE
+        item.n = int(points)
+        return self._add(item, None)
+
+    def vegas(self, Q2, xbj, polarization, quarks, kind: str, maxeval: int, seed: int) -> int:
+        """_vegas_with_budget (S/dis/observables.py:661-686): up to 8 iterations
+        of maxeval / iterations samples, at least three before convergence."""
+
+        from nlo_bk_optimized_amd import _lib
+        from nlo_bk_optimized_amd import device as dev
+
+        if maxeval < 2:
+            raise ValueError("NLO integration requires maxeval >= 2")
+        max_iterations = min(8, max(1, maxeval // 2))
+        samples = max(2, maxeval // max_iterations)
+        dim = _lib.DIS_DIMENSIONS[kind]
+        key = (dim, int(seed))
+        if key not in self._table_index:
+            self._table_index[key] = len(self.tables)
+            self.tables.append(dev.sobol_table(dim, int(seed)))
+        stride = (samples + 63) // 64 * 64
+        first = self.config.sobol_first_index
+        if first + stride * max_iterations > 1 << _lib.SOBOL_BITS:
+            raise ValueError("the Sobol sequence (2^30 points) is too short for this budget")
+        item = _lib.DisItem()
+        item.desc = self._desc(Q2, xbj, polarization, quarks)
+        item.contribution = _lib.DIS_KINDS[kind]
+        item.method = _lib.NLO_DIS_METHOD_VEGAS
+        item.n = samples
+        item.sobol_table = self._table_index[key]
+        item.max_iterations = max_iterations
+        item.min_iterations = min(3, max_iterations)
+        item.first_index = first
+        return self._add(item, int(seed))
+
+    def run(self, workspace_budget: int = 8 << 30) -> list[IntegralResult]:
+        """Every collected integral as an IntegralResult, in collection order."""
+
+        from nlo_bk_optimized_amd import device as dev
+
+        if not self.items:
+            return []
+        rows = dev.dis_batch(self.items, self.tables, bins=self.config.bins, precision=self.precision,
+                             epsrel=self.config.epsrel, epsabs=0.0, device=self.device,
+                             workspace_budget=workspace_budget).cpu().tolist()
+        return [IntegralResult(torch.tensor(v, dtype=torch.float64), torch.tensor(e, dtype=torch.float64),
+                               int(n), bool(c), seed) for (v, e, n, c), seed in zip(rows, self.seeds)]
+
+
+def _lo_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, quadrature_points: int):
+    """S/dis/observables.py:334-385."""
+
+    dipole, config = plan.dipole, plan.config
+    _validate_kinematics(Q2, xbj)
+    Y = max(math.log(dipole.X0() / _scalar(xbj)), 0.0)
+    if isinstance(dipole, BKDipole):
+        dipole.initialize_interpolation(Y)
+    k = plan.gauss_legendre(Q2, xbj, polarization, tuple(config.quarks), "LO", quadrature_points)
+    return lambda R: _scale_result(R[k], 4 * AlphaEM * NC / (2 * math.pi) ** 2)
+
+
+def _dip_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, quadrature_points: int, seed: int):
+    """S/dis/observables.py:388-460."""
+
+    config = plan.config
+    _validate_kinematics(Q2, xbj)
+    contributions = ((("Omega_L_const", 2), ("ab", 3), ("cd", 4)) if polarization is Polarization.L
+                     else (("T0", 2), ("T1", 3), ("T2", 4)))
+    per_quark = []
+    offset = 0
+    for quark in config.quarks:
+        maxeval = config.maxeval * (10 if quark.mass < 0.4 else 1)
+        terms = []
+        for contribution, dims in contributions:
+            if dims == 2:
+                k = plan.gauss_legendre(Q2, xbj, polarization, (quark,), contribution, quadrature_points)
+            else:
+                k = plan.vegas(Q2, xbj, polarization, (quark,), contribution, maxeval, seed + offset)
+                offset += 1
+            terms.append((k, quark.charge**2))
+        per_quark.append(terms)
+    factor = 4 * NC * AlphaEM / (2 * math.pi) ** 2 * 2 * math.pi
+
+    def compose(R):
+        results = [_sum_results([_scale_result(R[k], c) for k, c in terms], seed) for terms in per_quark]
+        return _scale_result(_sum_results(results, seed), factor)
+
+    return compose
+
+
+def _qg_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, seed: int):
+    """S/dis/observables.py:463-538 (VEGAS path)."""
+
+    config = plan.config
+    _validate_kinematics(Q2, xbj)
+    # the reference nests the I2 / I3 inner integral only on its float32 CUDA
+    # path (use_nested, S/dis/observables.py:492); float64 integrates the
+    # 6- / 7-dimensional forms
+    kinds = (("I1", "I2_nested", "I3_nested") if config.cuda_nested and plan.precision == "fp32"
+             else ("I1", "I2", "I3"))
+    per_quark = []
+    offset = 0
+    for quark in config.quarks:
+        terms = []
+        for kind in kinds:
+            k = plan.vegas(Q2, xbj, polarization, (quark,), kind, config.maxeval, seed + offset)
+            offset += 1
+            terms.append((k, quark.charge**2))
+        per_quark.append(terms)
+    factor = 4 * NC * AlphaEM / (2 * math.pi) ** 3 * 2 * math.pi
+
+    def compose(R):
+        results = [_sum_results([_scale_result(R[k], c) for k, c in terms], seed) for terms in per_quark]
+        return _scale_result(_sum_results(results, seed), factor)
+
+    return compose
+
+
+def _cross_section_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, quadrature_points: int, seed: int):
+    """S/dis/observables.py:541-578."""
+
+    config, dipole = plan.config, plan.dipole
+    if config.order is DISOrder.LO:
+        return _lo_recipe(plan, Q2, xbj, polarization, quadrature_points)
+    x0 = torch.tensor(dipole.X0(), dtype=Q2.dtype) if isinstance(Q2, torch.Tensor) else dipole.X0()
+    lo = _lo_recipe(plan, Q2, x0, polarization, quadrature_points)
+    dip = _dip_recipe(plan, Q2, xbj, polarization, quadrature_points, seed)
+    qg = _qg_recipe(plan, Q2, xbj, polarization, seed + 100)
+    return lambda R: _sum_results((lo(R), dip(R), qg(R)), seed)
+
+
+def _structure_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, quadrature_points: int, seed: int):
+    """F_{T,L} = Q^2 / (4 pi^2 alpha_em) sigma x transverse area (S/dis/observables.py:581-610)."""
+
+    sigma = _cross_section_recipe(plan, Q2, xbj, polarization, quadrature_points, seed)
+    factor = _scalar(Q2) / (4 * math.pi**2 * AlphaEM) * plan.config.transverse_area
+    return lambda R: _scale_result(sigma(R), factor)
+
+
+def _plan(Q2, dipole: Dipole, config: DISConfig) -> _Plan:
+    return _Plan(dipole, config, _precision(Q2), _device_of(dipole, Q2))
 
 
 def photon_proton_cross_section_LO_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: DISConfig, *,
                                        quadrature_points: int = 24, batch_size: int = 65_536) -> IntegralResult:
     """S/dis/observables.py:334-385."""
 
-    _validate_kinematics(Q2, xbj)
-    Y = max(math.log(dipole.X0() / _scalar(xbj)), 0.0)
-    if isinstance(dipole, BKDipole):
-        dipole.initialize_interpolation(Y)
-    desc = _desc(Q2, xbj, polarization, tuple(config.quarks), dipole, config)
-    res = gauss_legendre_device(desc, "LO", points=quadrature_points, epsrel=config.epsrel,
-                                precision=_precision(Q2), device=_device_of(dipole, Q2))
-    return _scale_result(res, 4 * AlphaEM * NC / (2 * math.pi) ** 2)
+    plan = _plan(Q2, dipole, config)
+    recipe = _lo_recipe(plan, Q2, xbj, polarization, quadrature_points)
+    return recipe(plan.run())
 
 
 def sigma_dip_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: DISConfig, *,
                   quadrature_points: int = 24, batch_size: int = 65_536, seed: int = 0) -> IntegralResult:
     """S/dis/observables.py:388-460."""
 
-    _validate_kinematics(Q2, xbj)
-    contributions = ((("Omega_L_const", 2), ("ab", 3), ("cd", 4)) if polarization is Polarization.L
-                     else (("T0", 2), ("T1", 3), ("T2", 4)))
-    precision = _precision(Q2)
-    device = _device_of(dipole, Q2)
-    results = []
-    offset = 0
-    for quark in config.quarks:
-        per_quark = []
-        maxeval = config.maxeval * (10 if quark.mass < 0.4 else 1)
-        desc = _desc(Q2, xbj, polarization, (quark,), dipole, config)
-        for contribution, dims in contributions:
-            if dims == 2:
-                res = gauss_legendre_device(desc, contribution, points=quadrature_points, epsrel=config.epsrel,
-                                            precision=precision, device=device)
-            else:
-                res = _vegas_with_budget(desc, contribution, maxeval, config.epsrel, seed + offset, config,
-                                         precision, device)
-                offset += 1
-            per_quark.append(_scale_result(res, quark.charge**2))
-        results.append(_sum_results(per_quark, seed))
-    factor = 4 * NC * AlphaEM / (2 * math.pi) ** 2 * 2 * math.pi
-    return _scale_result(_sum_results(results, seed), factor)
+    plan = _plan(Q2, dipole, config)
+    recipe = _dip_recipe(plan, Q2, xbj, polarization, quadrature_points, seed)
+    return recipe(plan.run())
 
 
 def sigma_qg_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: DISConfig, *,
                  batch_size: int = 65_536, seed: int = 0) -> IntegralResult:
     """S/dis/observables.py:463-538 (VEGAS path)."""
 
-    _validate_kinematics(Q2, xbj)
-    precision = _precision(Q2)
-    device = _device_of(dipole, Q2)
-    # the reference nests the I2 / I3 inner integral only on its float32 CUDA
-    # path (use_nested, S/dis/observables.py:492); float64 integrates the
-    # 6- / 7-dimensional forms
-    kinds = ("I1", "I2_nested", "I3_nested") if config.cuda_nested and precision == "fp32" else ("I1", "I2", "I3")
-    results = []
-    offset = 0
-    for quark in config.quarks:
-        desc = _desc(Q2, xbj, polarization, (quark,), dipole, config)
-        per_quark = []
-        for kind in kinds:
-            res = _vegas_with_budget(desc, kind, config.maxeval, config.epsrel, seed + offset, config, precision,
-                                     device)
-            offset += 1
-            per_quark.append(_scale_result(res, quark.charge**2))
-        results.append(_sum_results(per_quark, seed))
-    factor = 4 * NC * AlphaEM / (2 * math.pi) ** 3 * 2 * math.pi
-    return _scale_result(_sum_results(results, seed), factor)
+    plan = _plan(Q2, dipole, config)
+    recipe = _qg_recipe(plan, Q2, xbj, polarization, seed)
+    return recipe(plan.run())
 
 
 def photon_proton_cross_section_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: DISConfig, *,
@@ -298,61 +424,101 @@ def photon_proton_cross_section_d2b(Q2, xbj, polarization: Polarization, dipole:
                                     seed: int = 0) -> IntegralResult:
     """S/dis/observables.py:541-578."""
 
-    if config.order is DISOrder.LO:
-        return photon_proton_cross_section_LO_d2b(Q2, xbj, polarization, dipole, config,
-                                                  quadrature_points=quadrature_points)
-    x0 = torch.tensor(dipole.X0(), dtype=Q2.dtype) if isinstance(Q2, torch.Tensor) else dipole.X0()
-    lo = photon_proton_cross_section_LO_d2b(Q2, x0, polarization, dipole, config,
-                                            quadrature_points=quadrature_points)
-    dip = sigma_dip_d2b(Q2, xbj, polarization, dipole, config, quadrature_points=quadrature_points, seed=seed)
-    qg = sigma_qg_d2b(Q2, xbj, polarization, dipole, config, seed=seed + 100)
-    return _sum_results((lo, dip, qg), seed)
+    plan = _plan(Q2, dipole, config)
+    recipe = _cross_section_recipe(plan, Q2, xbj, polarization, quadrature_points, seed)
+    return recipe(plan.run())
 
 
-def FL(Q2, xbj, dipole: Dipole, config: DISConfig, **integration_options) -> IntegralResult:
+def FL(Q2, xbj, dipole: Dipole, config: DISConfig, *, quadrature_points: int = 24, batch_size: int = 65_536,
+       seed: int = 0) -> IntegralResult:
     """S/dis/observables.py:581-594."""
 
-    sigma = photon_proton_cross_section_d2b(Q2, xbj, Polarization.L, dipole, config, **integration_options)
-    return _scale_result(sigma, _scalar(Q2) / (4 * math.pi**2 * AlphaEM) * config.transverse_area)
+    plan = _plan(Q2, dipole, config)
+    recipe = _structure_recipe(plan, Q2, xbj, Polarization.L, quadrature_points, seed)
+    return recipe(plan.run())
 
 
-def FT(Q2, xbj, dipole: Dipole, config: DISConfig, **integration_options) -> IntegralResult:
+def FT(Q2, xbj, dipole: Dipole, config: DISConfig, *, quadrature_points: int = 24, batch_size: int = 65_536,
+       seed: int = 0) -> IntegralResult:
     """S/dis/observables.py:597-610."""
 
-    sigma = photon_proton_cross_section_d2b(Q2, xbj, Polarization.T, dipole, config, **integration_options)
-    return _scale_result(sigma, _scalar(Q2) / (4 * math.pi**2 * AlphaEM) * config.transverse_area)
+    plan = _plan(Q2, dipole, config)
+    recipe = _structure_recipe(plan, Q2, xbj, Polarization.T, quadrature_points, seed)
+    return recipe(plan.run())
 
 
 def F2(Q2, xbj, dipole: Dipole, config: DISConfig, **integration_options) -> IntegralResult:
     """FT + FL, the L integrals seeded 1000 later (S/dis/observables.py:613-624)."""
 
-    options_t = dict(integration_options)
-    options_l = dict(integration_options)
-    if "seed" in options_l:
-        options_l["seed"] += 1_000
-    return _sum_results((FT(Q2, xbj, dipole, config, **options_t), FL(Q2, xbj, dipole, config, **options_l)),
-                        integration_options.get("seed"))
+    return F2_batch([(Q2, xbj)], dipole, config, **integration_options)[0]
 
 
 def sigma_r(Q2, xbj, sqrt_s: float, dipole: Dipole, config: DISConfig, **integration_options) -> IntegralResult:
     """Reduced cross section F2 - y^2 / (1 + (1 - y)^2) FL, y = Q^2 / (xbj s).
     Not in the reference; built from its F2 and FL."""
 
-    y = _scalar(Q2) / (_scalar(xbj) * sqrt_s**2)
-    if not 0 < y <= 1:
-        raise ValueError(f"inelasticity y = {y} outside (0, 1]")
-    options_l = dict(integration_options)
-    if "seed" in options_l:
-        options_l["seed"] += 1_000
-    ft = FT(Q2, xbj, dipole, config, **integration_options)
-    fl = FL(Q2, xbj, dipole, config, **options_l)
-    c = 1 - y * y / (1 + (1 - y) ** 2)  # sigma_r = FT + FL - (y^2 / Y+) FL, same seeds as F2
-    return IntegralResult(ft.value + c * fl.value, torch.sqrt(ft.error**2 + (c * fl.error) ** 2),
-                          ft.n_eval + fl.n_eval, ft.converged and fl.converged, integration_options.get("seed"))
+    return sigma_r_batch([(Q2, xbj)], sqrt_s, dipole, config, **integration_options)[0]
+
+
+def structure_functions_batch(points, dipole: Dipole, config: DISConfig, *, quadrature_points: int = 24,
+                              batch_size: int = 65_536, seed: int | None = None,
+                              workspace_budget: int = 8 << 30) -> list[tuple[IntegralResult, IntegralResult]]:
+    """(FT, FL) at every (Q^2, xbj) of ``points`` with F2's seeds (a given
+    seed for FT and seed + 1000 for FL; without one, both take the default 0),
+    all integrals of all points in one device batch.  Each point's results
+    equal FT / FL called for that point alone.  Q^2 values of one dtype:
+    float32 tensors take the FP32 integrands."""
+
+    points = list(points)
+    if not points:
+        return []
+    s0 = 0 if seed is None else seed
+    s_l = 0 if seed is None else seed + 1_000
+    plan = _plan(points[0][0], dipole, config)
+    recipes = []
+    for Q2, xbj in points:
+        if _precision(Q2) != plan.precision:
+            raise ValueError("all Q2 of a batch must have one dtype")
+        ft = _structure_recipe(plan, Q2, xbj, Polarization.T, quadrature_points, s0)
+        fl = _structure_recipe(plan, Q2, xbj, Polarization.L, quadrature_points, s_l)
+        recipes.append((ft, fl))
+    R = plan.run(workspace_budget)
+    return [(ft(R), fl(R)) for ft, fl in recipes]
+
+
+def F2_batch(points, dipole: Dipole, config: DISConfig, **integration_options) -> list[IntegralResult]:
+    """F2 = FT + FL at every (Q^2, xbj) of ``points`` in one device batch
+    (S/dis/observables.py:613-624 per point)."""
+
+    seed = integration_options.get("seed")
+    return [_sum_results((ft, fl), seed)
+            for ft, fl in structure_functions_batch(points, dipole, config, **integration_options)]
+
+
+def sigma_r_batch(points, sqrt_s: float, dipole: Dipole, config: DISConfig,
+                  **integration_options) -> list[IntegralResult]:
+    """sigma_r = FT + (1 - y^2 / (1 + (1 - y)^2)) FL at every point, y = Q^2 / (xbj s),
+    with F2's seeds, in one device batch."""
+
+    points = list(points)
+    ys = []
+    for Q2, xbj in points:
+        y = _scalar(Q2) / (_scalar(xbj) * sqrt_s**2)
+        if not 0 < y <= 1:
+            raise ValueError(f"inelasticity y = {y} outside (0, 1]")
+        ys.append(y)
+    out = []
+    for (ft, fl), y in zip(structure_functions_batch(points, dipole, config, **integration_options), ys):
+        c = 1 - y * y / (1 + (1 - y) ** 2)
+        out.append(IntegralResult(ft.value + c * fl.value, torch.sqrt(ft.error**2 + (c * fl.error) ** 2),
+                                  ft.n_eval + fl.n_eval, ft.converged and fl.converged,
+                                  integration_options.get("seed")))
+    return out
 
 
 __all__ = [
     "F2",
+    "F2_batch",
     "FL",
     "FT",
     "evolution_rapidity_dipole",
@@ -367,6 +533,8 @@ __all__ = [
     "sigma_dip_d2b",
     "sigma_qg_d2b",
     "sigma_r",
+    "sigma_r_batch",
+    "structure_functions_batch",
     "tripole_amplitude",
     "z2_lower_bound",
 ]

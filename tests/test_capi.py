@@ -133,6 +133,8 @@ STRUCTS = {
     "nlo_bk_batch_desc": _lib.BatchDesc,
     "nlo_dis_dipole": _lib.DisDipole,
     "nlo_dis_desc": _lib.DisDesc,
+    "nlo_dis_item": _lib.DisItem,
+    "nlo_dis_batch_desc": _lib.DisBatchDesc,
 }
 
 

@@ -31,30 +31,45 @@ def _result(value, error=0.0, n_eval=1, converged=True, seed=None):
 
 @pytest.fixture
 def fakes(monkeypatch):
-    """Record every integral the composition asks for; answer from tables."""
+    """Replace the device batch (and Sobol tables) by fakes that record every
+    integral the composition asks for and answer from tables: the real plan
+    builds the items (budgets, iterations, seeds, descriptors)."""
+
+    from nlo_bk_optimized_amd import _lib
+    from nlo_bk_optimized_amd import device as dev
 
     calls = []
     gl_values = {}
     vegas_values = {}
+    kinds = {v: k for k, v in _lib.DIS_KINDS.items()}
 
-    def fake_desc(Q2, xbj, polarization, quarks, dipole, config):
-        return SimpleNamespace(Q2=float(Q2), xbj=float(xbj), polarization=polarization, quarks=tuple(quarks))
+    def fake_table(dim, seed):
+        t = _lib.SobolTable()
+        t.dim = dim
+        t.shift[0] = seed  # marks the seed for the fake batch
+        return t
 
-    def fake_gl(desc, kind, *, points, epsrel, precision, device):
-        calls.append(("gl", kind, desc, dict(points=points, precision=precision)))
-        value, error = gl_values.get(kind, (1.0, 0.0))
-        return _result(value, error, n_eval=points**2 + max(2, points // 2) ** 2)
+    def fake_batch(items, tables, *, bins, precision, epsrel, epsabs, device, workspace_budget):
+        rows = []
+        for it in items:
+            kind = kinds[it.contribution]
+            desc = SimpleNamespace(Q2=it.desc.Q2, xbj=it.desc.xbj, transverse=it.desc.transverse,
+                                   n_quarks=it.desc.n_quarks)
+            if it.method == _lib.NLO_DIS_METHOD_GAUSS_LEGENDRE:
+                calls.append(("gl", kind, desc, dict(points=it.n, precision=precision)))
+                value, error = gl_values.get(kind, (1.0, 0.0))
+                rows.append((value, error, it.n**2 + max(2, it.n // 2) ** 2, 1.0))
+            else:
+                seed = int(tables[it.sobol_table].shift[0])
+                calls.append(("vegas", kind, desc, dict(samples=it.n, iterations=it.max_iterations,
+                                                         min_iterations=it.min_iterations, seed=seed,
+                                                         precision=precision, first=it.first_index)))
+                value, error = vegas_values.get(kind, (1.0, 0.0))
+                rows.append((value, error, it.n * it.max_iterations, 1.0))
+        return torch.tensor(rows, dtype=torch.float64)
 
-    def fake_vegas(desc, kind, *, samples_per_iteration, max_iterations, min_iterations, bins, epsrel, seed,
-                   precision, first_index, device):
-        calls.append(("vegas", kind, desc, dict(samples=samples_per_iteration, iterations=max_iterations,
-                                                 min_iterations=min_iterations, seed=seed, precision=precision)))
-        value, error = vegas_values.get(kind, (1.0, 0.0))
-        return _result(value, error, n_eval=samples_per_iteration * max_iterations, seed=seed)
-
-    monkeypatch.setattr(obs, "_desc", fake_desc)
-    monkeypatch.setattr(obs, "gauss_legendre_device", fake_gl)
-    monkeypatch.setattr(obs, "vegas_device", fake_vegas)
+    monkeypatch.setattr(dev, "sobol_table", fake_table)
+    monkeypatch.setattr(dev, "dis_batch", fake_batch)
     return SimpleNamespace(calls=calls, gl=gl_values, vegas=vegas_values)
 
 
@@ -71,7 +86,7 @@ def test_lo_normalisation_known_answer(fakes):
     torch.testing.assert_close(res.error, torch.tensor(0.25 * LO_FACTOR, dtype=torch.float64))
     (kind, name, desc, opts), = fakes.calls
     assert (kind, name, opts["points"], opts["precision"]) == ("gl", "LO", 24, "fp32")
-    assert len(desc.quarks) == 2  # LO sums every quark inside the integrand
+    assert desc.n_quarks == 2  # LO sums every quark inside the integrand
 
 
 def test_qg_normalisation_known_answer(fakes):
@@ -143,6 +158,37 @@ def test_cross_section_and_structure_functions_compose(fakes):
     lo_only = obs.photon_proton_cross_section_d2b(Q2, x, Polarization.T, dip,
                                                   DISConfig(quarks=(charm,), order=DISOrder.LO))
     torch.testing.assert_close(lo_only.value, torch.tensor(2.0 * LO_FACTOR, dtype=torch.float64))
+
+
+def test_batch_composition_equals_per_point(fakes):
+    """F2_batch / sigma_r_batch over several points: the per-point results,
+    and the same integrals (kinds, seeds, budgets) as the per-point calls."""
+
+    light, charm = Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)
+    cfg = DISConfig(quarks=(light, charm), maxeval=1600, order=DISOrder.NLO)
+    fakes.vegas.update({"T1": (0.5, 0.1), "T2": (0.25, 0.2), "I1": (0.125, 0.05), "ab": (1.5, 0.3)})
+    pts = [(torch.tensor(q), torch.tensor(x)) for q, x in ((2.0, 1e-3), (20.0, 1e-4), (50.0, 1e-2))]
+    batch = obs.F2_batch(pts, GBW(), cfg, seed=7)
+    batch_calls = list(fakes.calls)
+    fakes.calls.clear()
+    single = [obs.F2(q, x, GBW(), cfg, seed=7) for q, x in pts]
+    for b, s1 in zip(batch, single):
+        assert torch.equal(b.value, s1.value) and torch.equal(b.error, s1.error)
+        assert b.n_eval == s1.n_eval and b.converged == s1.converged and b.seed == 7
+    key = [(c[0], c[1], c[2].Q2, c[2].xbj, c[2].transverse, tuple(sorted(c[3].items()))) for c in batch_calls]
+    assert key == [(c[0], c[1], c[2].Q2, c[2].xbj, c[2].transverse, tuple(sorted(c[3].items())))
+                   for c in fakes.calls]
+    # sigma_r = FT + (1 - y^2 / Y+) FL with F2's seeds
+    sr = obs.sigma_r_batch([pts[0], pts[2]], 318.0, GBW(), cfg, seed=7)
+    ft = obs.FT(*pts[0], GBW(), cfg, seed=7)
+    fl = obs.FL(*pts[0], GBW(), cfg, seed=1007)
+    y = 2.0 / (1e-3 * 318.0**2)
+    torch.testing.assert_close(sr[0].value, ft.value + (1 - y * y / (1 + (1 - y) ** 2)) * fl.value)
+    # without a seed F2 takes the default 0 for both polarisations, as the reference does
+    fakes.calls.clear()
+    obs.F2(*pts[0], GBW(), cfg)
+    seeds = {c[3]["seed"] for c in fakes.calls if c[0] == "vegas"}
+    assert min(seeds) == 0 and max(seeds) < 1000
 
 
 def _fake_device(monkeypatch, sums_per_iteration):
