@@ -83,6 +83,8 @@ def parse() -> argparse.Namespace:
                    help="evolutions in the batched uneven-grid run (0 skips it)")
     p.add_argument("--dis-points", type=int, default=32,
                    help="(Q^2, x) points of the NLO DIS batch (512 = the full 32 x 16 grid; 0 skips it)")
+    p.add_argument("--dis-precisions", default="fp32,fp64",
+                   help="integrand precisions of the DIS batch (fp32 always runs)")
     p.add_argument("--dis-frozen-points", type=int, default=32,
                    help="points of the DIS batch whose terms are also compared FP32 vs FP64 through one frozen grid")
     p.add_argument("--kernel-probe", action="store_true",
@@ -311,6 +313,9 @@ def main() -> int:
         points = [(q, x) for q in qs_all[::step] for x in xs_all[::step]]
         res = {}
         for dtype, name in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+            if name not in args.dis_precisions:
+                res[name] = res["fp32"]
+                continue
             # every integral of every point (FT with seed 0, FL with seed
             # 1000, as F2 seeds them) in one device batch
             pts = [(torch.tensor(q2, dtype=dtype), torch.tensor(xb, dtype=dtype)) for q2, xb in points]

@@ -695,20 +695,19 @@ __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x)
 
 __device__ __forceinline__ float frsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// sin y for y in [-pi, pi]: fold |y| > pi/2 with pi - |y| computed exactly
-// (Sterbenz) plus the low part of pi, then the Taylor polynomial to y^11
-// (relative error < 6e-8, accurate for small y).
-__device__ __forceinline__ float fsin_pi(float y) {
+// |sin y| for y in [-pi, pi] (the integrand needs only squares of sines):
+// fold to v = min(|y|, pi - |y|), with pi - |y| computed exactly for
+// |y| >= pi/2 (Sterbenz) plus the low part of pi, then the Taylor polynomial
+// to v^11 (relative error < 6e-8, accurate for small v).
+__device__ __forceinline__ float fsin_abs(float y) {
     const float ay = fabsf(y);
-    float v = ay;
-    if (ay > 1.57079637f) v = (3.14159274f - ay) + -8.74227766e-8f;
+    const float v = fminf(ay, (3.14159274f - ay) + -8.74227766e-8f);
     const float v2 = v * v;
     float p = fmaf(v2, -2.50521084e-8f, 2.75573192e-6f);
     p = fmaf(v2, p, -1.98412698e-4f);
     p = fmaf(v2, p, 8.33333333e-3f);
     p = fmaf(v2, p, -1.66666667e-1f);
-    p = fmaf(v2 * v, p, v);
-    return y < 0.0f ? -p : p;
+    return fmaf(v2 * v, p, v);
 }
 
 // ln(1 + u) for |u| < 0.5: 2 atanh(s), s = u / (2 + u), |s| < 0.2, series to s^9.
@@ -767,17 +766,17 @@ __device__ __forceinline__ float fspline_n(const FastSpline& s, float log_d) {
 
 // N and S = 1 - N.  Near N = 1 the subtraction would keep only the absolute
 // precision of N, so S = -expm1(p ln 2) comes from the exponent p = log2 N
-// by its series when |p ln 2| < 1/4 (relative error < 2e-8).
+// by its series when |p ln 2| < 1/4 (relative error < 2e-8), written in p:
+// expm1(p ln 2) = sum_k (ln 2)^k p^k / k!.
 __device__ __forceinline__ void fspline_ns(const FastSpline& s, float log_d, float& n, float& sv) {
     const float p = fspline_log2n(s, log_d);
     n = __builtin_amdgcn_exp2f(p);
-    const float x = p * 0.693147182f;
-    float e = fmaf(x, 1.0f / 720.0f, 1.0f / 120.0f);
-    e = fmaf(x, e, 1.0f / 24.0f);
-    e = fmaf(x, e, 1.0f / 6.0f);
-    e = fmaf(x, e, 0.5f);
-    e = fmaf(x * x, e, x);  // expm1(x)
-    sv = fabsf(x) < 0.25f ? -e : 1.0f - n;
+    float e = fmaf(p, 1.54035304e-4f, 1.33335581e-3f);  // (ln 2)^6 / 720, (ln 2)^5 / 120
+    e = fmaf(p, e, 9.61812911e-3f);                     // (ln 2)^4 / 24
+    e = fmaf(p, e, 5.55041087e-2f);                     // (ln 2)^3 / 6
+    e = fmaf(p, e, 2.40226507e-1f);                     // (ln 2)^2 / 2
+    e = fmaf(p, e, 6.93147181e-1f);                     // ln 2
+    sv = fabsf(p) < 0.360673760f ? -(p * e) : 1.0f - n;  // |p ln 2| < 1/4
 }
 
 // The FP32 K2 + Kf integrand: the cancellation-free formula of
@@ -790,8 +789,8 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
     const NloCoupling<float>& cpl, float nf_over_nc, float nc, float norm, bool symmetrize) {
     NloStable<float> out{0.0f, 0.0f, 0.0f};
     const float z = fexp(lz), z2 = fexp(lz2);
-    const float s1 = fsin_pi(0.5f * tz), s2 = fsin_pi(0.5f * tz2);
-    const float s3 = fsin_pi(0.5f * (tz - tz2));
+    const float s1 = fsin_abs(0.5f * tz), s2 = fsin_abs(0.5f * tz2);
+    const float s3 = fsin_abs(0.5f * (tz - tz2));
     const float xsq = fmaf(4.0f * r * z, s1 * s1, (r - z) * (r - z));
     const float x2sq = fmaf(4.0f * r * z2, s2 * s2, (r - z2) * (r - z2));
     const float zsq = fmaf(4.0f * z * z2, s3 * s3, (z - z2) * (z - z2));
