@@ -427,12 +427,24 @@ typedef struct nlo_dis_dipole {
     const double* d_N;
     double y_init, maxr_interpolate;
     const double* d_init_coef;
+    /* BK table, SPLINE_LINEAR: the local splines of nlo_dis_bk_local_splines
+     * (T (R - 1) x 4 doubles), or NULL to solve each local spline per query */
+    const double* d_local_coef;
 } nlo_dis_dipole;
 
 /* N(r_i, Y_i) for n device points (BKDipole.dipole_amplitude /
  * GBW.dipole_amplitude, S/dipole/amplitude.py:38-43, 133-168). */
 int nlo_dis_dipole_amplitude(const nlo_dis_dipole* dipole, int64_t n, const double* d_r,
                              const double* d_Y, double* d_out, hipStream_t stream);
+
+/* The local splines of BKDipole._local_spline_amplitude
+ * (S/dipole/amplitude.py:204-240) for every table row y and r interval i:
+ * the cubic, in t = r - r_i, of the natural spline through the 7 nodes
+ * around interval i (the last 9 at the top of the grid) of row y, on
+ * interval i; d_coef: T (R - 1) x 4 doubles.  The spline is linear in its
+ * data, so the amplitude at (r, Y) between rows y and y + 1 is the same
+ * linear interpolation of the two rows' cubics, and a query needs no solve. */
+int nlo_dis_bk_local_splines(const nlo_dis_dipole* dipole, double* d_coef, hipStream_t stream);
 
 /* BKDipole.initialize_interpolation (S/dipole/amplitude.py:111-131, 242-260)
  * at rapidity Y: the natural spline in r through the amplitude at the table

@@ -153,6 +153,7 @@ class DisDipole(ctypes.Structure):
         ("y_init", ctypes.c_double),
         ("maxr_interpolate", ctypes.c_double),
         ("d_init_coef", ctypes.c_void_p),
+        ("d_local_coef", ctypes.c_void_p),
     ]
 
 
@@ -310,6 +311,7 @@ SIGNATURES: dict[str, tuple] = {
          _vp, _vp, _size, _vp],
     ),
     "nlo_dis_gauss_legendre": (ctypes.c_int, [ctypes.POINTER(DisDesc), _i32, _i32, _i32, _vp, _vp]),
+    "nlo_dis_bk_local_splines": (ctypes.c_int, [ctypes.POINTER(DisDipole), _vp, _vp]),
     "nlo_dis_batch_workspace": (_size, [ctypes.POINTER(DisBatchDesc)]),
     "nlo_dis_batch_run": (ctypes.c_int, [ctypes.POINTER(DisBatchDesc), _vp, _vp, _size, _vp]),
 }

@@ -207,6 +207,9 @@ struct DisDipole {
     double y_init, maxr_interpolate;
     const double* init_coef;
     int bilinear;  // AmplitudeInterpolation.LINEAR_LINEAR
+    // SPLINE_LINEAR: per row y and interval i the cubic in t = r - r_i of the
+    // local natural spline on interval i (T (R - 1) x 4), or null (solve per query)
+    const double* local_coef;
 };
 
 // GBW: 1 - exp(-(r^2 Qs^2(Y))^gamma / 4) (as arg below 1e-7), Qs^2 =
@@ -244,6 +247,43 @@ __device__ __forceinline__ T local_natural_spline(const T* x, const T* y, int n,
     const T a = x[k + 1] - xq, b = xq - x[k];
     return (m[k] * a * a * a + m[k + 1] * b * b * b) / (T(6) * h) + (y[k] / h - m[k] * h / T(6)) * a +
            (y[k + 1] / h - m[k + 1] * h / T(6)) * b;
+}
+
+// The same spline's piece on interval k as a cubic in t = x - x[k]:
+// c0 + t (c1 + t (c2 + t c3)) with c0 = y_k, c1 = (y_{k+1} - y_k) / h -
+// h (2 m_k + m_{k+1}) / 6, c2 = m_k / 2, c3 = (m_{k+1} - m_k) / (6 h).
+__device__ __forceinline__ void local_natural_piece(const double* x, const double* y, int n, int k, double* c) {
+    double m[9], cp[9], dp[9];
+    const int ni = n - 2;
+    for (int i = 0; i < ni; ++i) {
+        const double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+        const double diag = 2.0 * (h0 + h1);
+        const double rhs = 6.0 * ((y[i + 2] - y[i + 1]) / h1 - (y[i + 1] - y[i]) / h0);
+        const double lower = i > 0 ? h0 : 0.0;
+        const double den = diag - lower * (i > 0 ? cp[i - 1] : 0.0);
+        cp[i] = h1 / den;
+        dp[i] = (rhs - lower * (i > 0 ? dp[i - 1] : 0.0)) / den;
+    }
+    m[0] = 0.0;
+    m[n - 1] = 0.0;
+    for (int i = ni - 1; i >= 0; --i) m[i + 1] = dp[i] - (i < ni - 1 ? cp[i] * m[i + 2] : 0.0);
+    const double h = x[k + 1] - x[k];
+    c[0] = y[k];
+    c[1] = (y[k + 1] - y[k]) / h - h * (2.0 * m[k] + m[k + 1]) / 6.0;
+    c[2] = m[k] / 2.0;
+    c[3] = (m[k + 1] - m[k]) / (6.0 * h);
+}
+
+// The window of BKDipole._local_spline_amplitude for r in interval rind:
+// 7 nodes from rind - 3 (clamped at 0), or the last 9 at the top of the grid.
+__device__ __forceinline__ void local_window(int rind, int R, int& start, int& pts) {
+    if (rind + 3 > R - 1) {
+        start = R - 9;
+        pts = 9;
+    } else {
+        start = rind - 3 < 0 ? 0 : rind - 3;
+        pts = 7;
+    }
 }
 
 // Largest i with v[i] <= q (searchsorted(..., right=True) - 1), -1 below v[0].
@@ -305,19 +345,28 @@ __device__ __forceinline__ double bk_amplitude(double r, double Y, const DisDipo
             v = (N11 * (r2 - r) * (Y2 - Y) + N12 * (r - r1) * (Y2 - Y) + N21 * (r2 - r) * (Y - Y1) +
                  N22 * (r - r1) * (Y - Y1)) / (dr * dY);
         }
+    } else if (d.local_coef) {
+        // the local splines are linear in their data: the Y-linear
+        // combination of the two rows' precomputed cubics on interval rind
+        int rind = search_right(d.r, d.R, r);
+        rind = rind < 0 ? 0 : (rind > d.R - 2 ? d.R - 2 : rind);
+        const int yind = search_right(d.Y, d.T, Y);
+        const int y1 = yind < 0 ? 0 : yind;
+        const int y2 = y1 + 1 < d.T ? y1 + 1 : d.T - 1;
+        const double frac = y2 != y1 ? (Y - d.Y[y1]) / (d.Y[y2] - d.Y[y1]) : 0.0;
+        const double t = r - d.r[rind];
+        const double* c1 = d.local_coef + 4 * ((size_t)y1 * (d.R - 1) + rind);
+        const double* c2 = d.local_coef + 4 * ((size_t)y2 * (d.R - 1) + rind);
+        const double lo = c1[0] + t * (c1[1] + t * (c1[2] + t * c1[3]));
+        const double up = c2[0] + t * (c2[1] + t * (c2[2] + t * c2[3]));
+        v = lo + frac * (up - lo);
     } else {
         const int rind = search_right(d.r, d.R, r);
         const int yind = search_right(d.Y, d.T, Y);
         const int y1 = yind < 0 ? 0 : yind;
         const int y2 = y1 + 1 < d.T ? y1 + 1 : d.T - 1;
         int start, pts;
-        if (rind + 3 > d.R - 1) {
-            start = d.R - 9;
-            pts = 9;
-        } else {
-            start = rind - 3 < 0 ? 0 : rind - 3;
-            pts = 7;
-        }
+        local_window(rind, d.R, start, pts);
         const double frac = y2 != y1 ? (Y - d.Y[y1]) / (d.Y[y2] - d.Y[y1]) : 0.0;
         double xs[9], ys[9];
         for (int i = 0; i < pts; ++i) {

@@ -286,6 +286,23 @@ __global__ void __launch_bounds__(64) bk_init_kernel(DisDipole d, double Y, doub
     }
 }
 
+// The local-spline table: thread (y, i) for row y and r interval i.
+__global__ void __launch_bounds__(256) bk_local_kernel(DisDipole d, double* __restrict__ coef) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d.T * (d.R - 1)) return;
+    const int y = k / (d.R - 1), i = k - y * (d.R - 1);
+    int start, pts;
+    local_window(i, d.R, start, pts);
+    double xs[9], ys[9];
+    for (int j = 0; j < pts; ++j) {
+        xs[j] = d.r[start + j];
+        ys[j] = d.N[(size_t)y * d.R + start + j];
+    }
+    double c[4];
+    local_natural_piece(xs, ys, pts, i - start, c);
+    for (int j = 0; j < 4; ++j) coef[4 * (size_t)k + j] = c[j];
+}
+
 template <typename T, int K>
 __global__ void __launch_bounds__(256) integrand_kernel(DisArgs a, int64_t n, const double* __restrict__ x,
                                                         double* __restrict__ out) {
@@ -893,6 +910,7 @@ DisDipole make_dipole(const nlo_dis_dipole& d) {
     o.maxr_interpolate = d.maxr_interpolate;
     o.init_coef = d.d_init_coef;
     o.bilinear = d.bilinear;
+    o.local_coef = d.d_local_coef;
     return o;
 }
 
@@ -1274,6 +1292,20 @@ int nlo_dis_bk_initialize(const nlo_dis_dipole* dipole, double Y, double* d_coef
     NLO_REQUIRE(d_coef && d_maxr, "NULL output");
     const size_t lds = (size_t)3 * dipole->R * sizeof(double);
     hipLaunchKernelGGL(bk_init_kernel, dim3(1), dim3(64), lds, stream, make_dipole(*dipole), Y, d_coef, d_maxr);
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int nlo_dis_bk_local_splines(const nlo_dis_dipole* dipole, double* d_coef, hipStream_t stream) {
+    clear_error();
+    int rc = check_dipole(dipole);
+    if (rc != NLO_OK) return rc;
+    NLO_REQUIRE(dipole->kind == NLO_DIPOLE_BK_TABLE, "nlo_dis_bk_local_splines needs a BK table dipole");
+    NLO_REQUIRE(d_coef != nullptr, "NULL output");
+    DisDipole d = make_dipole(*dipole);
+    d.local_coef = nullptr;
+    const int n = d.T * (d.R - 1);
+    hipLaunchKernelGGL(bk_local_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, d, d_coef);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }

@@ -536,6 +536,9 @@ def dis_dipole_struct(dipole) -> _lib.DisDipole:
     d.d_r, d.d_Y, d.d_N = ptr(dipole.device_r), ptr(dipole.device_Y), ptr(dipole.device_N)
     d.x0 = 1.0
     d.bilinear = 1 if dipole.bilinear else 0
+    local = getattr(dipole, "device_local", None)
+    if local is not None and not dipole.bilinear:
+        d.d_local_coef = ptr(local)
     if dipole.device_coef is not None:
         d.initialized = 1
         d.y_init = dipole.interpolator_Y
@@ -556,6 +559,19 @@ def dis_dipole_amplitude(dipole, r: torch.Tensor, Y: torch.Tensor) -> torch.Tens
     check(lib().nlo_dis_dipole_amplitude(ctypes.byref(d), r.numel(), ptr(r), ptr(Y), ptr(out),
                                          stream_handle(r.device)), "nlo_dis_dipole_amplitude")
     return out
+
+
+def dis_bk_local_splines(dipole) -> torch.Tensor:
+    """The local-spline table of a BK-table dipole (nlo_dis_bk_local_splines):
+    T (R - 1) x 4 doubles on the dipole's device."""
+
+    d = dis_dipole_struct(dipole)
+    d.d_local_coef = None
+    T, R = dipole.device_Y.numel(), dipole.device_r.numel()
+    coef = torch.empty(4 * T * (R - 1), dtype=torch.float64, device=dipole.device_r.device)
+    check(lib().nlo_dis_bk_local_splines(ctypes.byref(d), ptr(coef), stream_handle(coef.device)),
+          "nlo_dis_bk_local_splines")
+    return coef
 
 
 def dis_bk_initialize(dipole, Y: float) -> tuple[torch.Tensor, float]:

@@ -107,6 +107,13 @@ class BKDipole:
         self.device_coef: torch.Tensor | None = None
         self._interpolator_Y = -1.0
         self._maxr_interpolate = -1.0
+        # the local splines of every row and r interval, computed once: a
+        # query combines two precomputed cubics instead of solving a spline
+        self.device_local: torch.Tensor | None = None
+        if not self.bilinear:
+            from nlo_bk_optimized_amd import device as dev
+
+            self.device_local = dev.dis_bk_local_splines(self)
 
     @property
     def min_r(self) -> float:

@@ -143,6 +143,28 @@ def test_dipole_amplitudes_match_oracle(dev, g):
     np.testing.assert_allclose(got, g["bk_N_bilinear_init"], rtol=1e-13, atol=1e-16)
 
 
+def test_bk_local_spline_table_equals_per_query_solve(dev):
+    """The precomputed local-spline table (every row and interval) against
+    solving the 7- / 9-point natural spline of the Y-interpolated rows per
+    query, at random (r, Y) over and beyond the table (clamped)."""
+
+    bk = _bk_dipole()
+    assert bk.device_local is not None
+    rng = np.random.default_rng(8)
+    r = np.exp(rng.uniform(np.log(bk.min_r) - 1, np.log(bk.max_r) + 1, 20000))
+    Y = rng.uniform(0.0, bk.max_Y, 20000)
+    r = np.concatenate([r, np.asarray(bk._r_host)])
+    Y = np.concatenate([Y, np.full(bk._r_host.numel(), bk.max_Y * 0.5)])
+    table = bk.dipole_amplitude(cuda(r), cuda(Y)).cpu().numpy()
+    local = bk.device_local
+    bk.device_local = None
+    try:
+        solve = bk.dipole_amplitude(cuda(r), cuda(Y)).cpu().numpy()
+    finally:
+        bk.device_local = local
+    np.testing.assert_allclose(table, solve, rtol=1e-13, atol=1e-16)
+
+
 # --------------------------------------------------------------- integrands
 
 POINT_CASES = [("LO", "T"), ("LO", "L"), ("T0", "T"), ("T1", "T"), ("T2", "T"), ("Omega_L_const", "L"),
