@@ -91,6 +91,9 @@ def parse() -> argparse.Namespace:
                    help="only run the K2+Kf launch and a 256 MiB device copy (for counter runs)")
     p.add_argument("--probe-precision", default="fp32", choices=("fp32", "fp64", "fp64_stable"),
                    help="integrand precision of the --kernel-probe launches")
+    p.add_argument("--probe-dis", action="store_true",
+                   help="with --kernel-probe: also one NLO DIS batch (8 points, GBW dipole, light + charm, "
+                        "2^22 samples per integral) in the probe precision")
     return p.parse_args()
 
 
@@ -168,6 +171,19 @@ def main() -> int:
         for _ in range(args.kernel_launches):
             dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config,
                        precision=args.probe_precision, sum_only=True)
+        for _ in range(args.kernel_launches):  # K1 on the fixed rule, as the evolution runs it
+            dev.bk_k1(setup.r, setup.N0, t64, config, rule="fixed")
+        if args.probe_dis:
+            from nlo_bk_optimized_amd.dipole import GBW
+            from nlo_bk_optimized_amd.dis import structure_functions_batch
+            from nlo_bk_optimized_amd.dis.config import DISConfig, DISOrder, Quark, QuarkType
+
+            dt = torch.float32 if args.probe_precision == "fp32" else torch.float64
+            dcfg = DISConfig(order=DISOrder.NLO, quarks=(Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)),
+                             maxeval=1 << 22, epsrel=1e-12)
+            pts = [(torch.tensor(q2, dtype=dt), torch.tensor(xb, dtype=dt))
+                   for q2 in (2.0, 30.0) for xb in (1e-5, 1e-4, 1e-3, 1e-2)]
+            structure_functions_batch(pts, GBW(Qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0), dcfg, seed=0)
         src = torch.ones(1 << 25, dtype=torch.float64, device=device)  # 256 MiB
         dst = torch.empty_like(src)
         for _ in range(2):

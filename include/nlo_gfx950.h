@@ -428,7 +428,9 @@ typedef struct nlo_dis_dipole {
     double y_init, maxr_interpolate;
     const double* d_init_coef;
     /* BK table, SPLINE_LINEAR: the local splines of nlo_dis_bk_local_splines
-     * (T (R - 1) x 4 doubles), or NULL to solve each local spline per query */
+     * (T (R - 1) x 4 doubles); required by the integrand, VEGAS and batch
+     * entry points; nlo_dis_dipole_amplitude solves each local spline per
+     * query when it is NULL */
     const double* d_local_coef;
 } nlo_dis_dipole;
 

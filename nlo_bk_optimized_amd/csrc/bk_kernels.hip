@@ -81,21 +81,27 @@ __device__ void stage_spline(const T* table, int R, int row, T* x, T* rec, LdsSp
 
 // -------------------------------------------------------------- splines
 
-// S/numerics/interpolation.py:27-48, :124-149 and S/bk/evolution.py:548-560:
-// one wave.  The tridiagonal system of the natural spline depends on the
-// grid only, so lane 0 factorises it once (u_i and 1/den_i, the only
-// divisions of the sweep) while the right-hand sides of both rows are formed
-// in parallel; lanes 0 and 1 then run the division-free forward and back
-// substitutions of the N and S rows.
+// S/numerics/interpolation.py:27-48, :124-149 and S/bk/evolution.py:548-560.
+// The interior second derivatives M_1..M_{R-2} of the N and S rows solve one
+// tridiagonal system each (natural ends, same matrix); the block solves both
+// by parallel cyclic reduction: every row i of a_i M_{i-1} + b_i M_i +
+// c_i M_{i+1} = d_i is combined with rows i - s and i + s (s = 1, 2, 4, ..),
+// which removes its neighbours, until it holds M_i alone (log2(R) steps with
+// two FP64 divisions per row instead of R dependent steps).  The matrix is
+// diagonally dominant, so no pivoting is needed.
+constexpr int kSplineThreads = 1024;
+constexpr int kSplineRows = (kMaxR + kSplineThreads - 1) / kSplineThreads;  // rows per thread
+
 __device__ __forceinline__ void spline_build_body(int R, const double* __restrict__ r,
                                                   const double* __restrict__ N, int force_positive,
                                                   double* __restrict__ t64, float* __restrict__ t32) {
     extern __shared__ double sh[];
     double* x = sh;
     double* ly = x + R;        // 2 R
-    double* up = ly + 2 * R;   // R: u_i
-    double* inv = up + R;      // R: 1 / den_i
-    double* so = inv + R;      // 2 R: right-hand sides, then solutions
+    double* so = ly + 2 * R;   // 2 R: right-hand sides, then solutions
+    double* pa = so + 2 * R;   // R: sub-diagonal
+    double* pb = pa + R;       // R: diagonal
+    double* pc = pb + R;       // R: super-diagonal
     for (int i = threadIdx.x; i < R; i += blockDim.x) {
         x[i] = log(r[i]);
         double v = N[i] > 1.0 ? 1.0 : N[i];
@@ -108,27 +114,60 @@ __device__ __forceinline__ void spline_build_body(int R, const double* __restric
     }
     __syncthreads();
     const int m = R - 2;
-    for (int item = threadIdx.x; item < 2 * m; item += blockDim.x) {
-        const int row = item / m, i = item - row * m;
-        const double* y = ly + row * R;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
         const double hi = x[i + 1] - x[i], hi1 = x[i + 2] - x[i + 1];
-        so[row * R + i] = 6.0 * ((y[i + 2] - y[i + 1]) / hi1 - (y[i + 1] - y[i]) / hi);
-    }
-    if (threadIdx.x == 0 && m >= 1) {
-        for (int i = 0; i < m; ++i) {
-            const double hi = x[i + 1] - x[i], hi1 = x[i + 2] - x[i + 1];
-            const double den = 2.0 * (hi + hi1) - (i > 0 ? hi * up[i - 1] : 0.0);
-            const double iv = 1.0 / den;
-            inv[i] = iv;
-            up[i] = hi1 * iv;
+        pa[i] = i > 0 ? hi : 0.0;
+        pb[i] = 2.0 * (hi + hi1);
+        pc[i] = i < m - 1 ? hi1 : 0.0;
+        for (int row = 0; row < 2; ++row) {
+            const double* y = ly + row * R;
+            so[row * R + i] = 6.0 * ((y[i + 2] - y[i + 1]) / hi1 - (y[i + 1] - y[i]) / hi);
         }
     }
     __syncthreads();
-    if (threadIdx.x < 2 && m >= 1) {
-        double* sol = so + threadIdx.x * R;
-        sol[0] = sol[0] * inv[0];
-        for (int i = 1; i < m; ++i) sol[i] = (sol[i] - (x[i + 1] - x[i]) * sol[i - 1]) * inv[i];
-        for (int i = m - 2; i >= 0; --i) sol[i] = sol[i] - up[i] * sol[i + 1];
+    for (int st = 1; st < m; st <<= 1) {
+        double na[kSplineRows], nb[kSplineRows], nc[kSplineRows], n0[kSplineRows], n1[kSplineRows];
+#pragma unroll
+        for (int k = 0; k < kSplineRows; ++k) {
+            const int i = threadIdx.x + k * kSplineThreads;
+            if (i >= m) continue;
+            double a = 0.0, b = pb[i], c = 0.0, d0 = so[i], d1 = so[R + i];
+            if (i - st >= 0) {
+                const double al = -pa[i] / pb[i - st];
+                a = al * pa[i - st];
+                b += al * pc[i - st];
+                d0 += al * so[i - st];
+                d1 += al * so[R + i - st];
+            }
+            if (i + st < m) {
+                const double ga = -pc[i] / pb[i + st];
+                c = ga * pc[i + st];
+                b += ga * pa[i + st];
+                d0 += ga * so[i + st];
+                d1 += ga * so[R + i + st];
+            }
+            na[k] = a;
+            nb[k] = b;
+            nc[k] = c;
+            n0[k] = d0;
+            n1[k] = d1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSplineRows; ++k) {
+            const int i = threadIdx.x + k * kSplineThreads;
+            if (i >= m) continue;
+            pa[i] = na[k];
+            pb[i] = nb[k];
+            pc[i] = nc[k];
+            so[i] = n0[k];
+            so[R + i] = n1[k];
+        }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        so[i] = so[i] / pb[i];
+        so[R + i] = so[R + i] / pb[i];
     }
     __syncthreads();
     const int iv = R - 1;
@@ -173,7 +212,7 @@ __device__ __forceinline__ void spline_build_body(int R, const double* __restric
     }
 }
 
-__global__ void __launch_bounds__(64) spline_build_kernel(int R, const double* __restrict__ r,
+__global__ void __launch_bounds__(kSplineThreads) spline_build_kernel(int R, const double* __restrict__ r,
                                                           const double* __restrict__ N,
                                                           int force_positive, double* __restrict__ t64,
                                                           float* __restrict__ t32) {
@@ -188,20 +227,10 @@ struct GaussRule {
     double xt[kMaxGaussT], wt[kMaxGaussT];
 };
 
-__device__ __forceinline__ double block_sum_256(double v, double* red) {
-    v = wave_sum(v);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
-    return t;
-}
-
 // K/bk/k1/kernel.cu:2040-2092 (production rule) for parent p = blockIdx.x:
 // panels [x_i, x_{i+1}] of the log-r grid, GL(nr) in ln z, GL(nt) in v with
 // theta = pi v^2, weight half_width w_r * w_t pi v, integrand * 2 z^2.  Each
-// thread owns whole z nodes (panel, i) and loops over the nt theta nodes,
+// work item is one z node (panel, i) with one half of the nt theta nodes,
 // whose sin^2(theta/2) and weights sit in LDS.
 constexpr int kK1Threads = 1024;
 
@@ -214,7 +243,7 @@ __device__ __forceinline__ void k1_body(int R, const double* __restrict__ r, con
                                         const GaussRule& g, const K1History& H, int p,
                                         double* __restrict__ k1) {
     extern __shared__ double sh[];
-    __shared__ double red[kK1Threads / 64];
+    __shared__ double red[kK1Threads / 64][2];
     __shared__ double gxr[kMaxGaussR], gwr[kMaxGaussR], s2t[kMaxGaussT], wtt[kMaxGaussT];
     __shared__ double j1tab[kJ1Pieces * (kJ1Deg + 1)];
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
@@ -250,8 +279,13 @@ __device__ __forceinline__ void k1_body(int R, const double* __restrict__ r, con
     // kinematical constraint: history rows and the current rapidity (last row)
     const int rows = (ANY && H.rows) ? min(max(*H.rows, 1), H.capacity) : 0;
     const double rapidity = rows > 0 ? H.y[rows - 1] : 0.0;
-    double total = 0.0;
-    for (int zi = threadIdx.x; zi < znodes; zi += blockDim.x) {
+    // each (z node, half of the theta nodes) is one work item: twice the
+    // waves of one item per z node, so the FP64 latency is better hidden
+    const int th = (g.nt + 1) / 2;
+    double total = 0.0, total_err = 0.0;  // compensated
+    for (int item = threadIdx.x; item < 2 * znodes; item += blockDim.x) {
+        const int zi = item >> 1;
+        const int j0 = (item & 1) * th, j1 = min(j0 + th, g.nt);
         const int panel = zi / g.nr;
         const int i = zi - panel * g.nr;
         const double lower = s.x[panel], upper = s.x[panel + 1];
@@ -262,24 +296,27 @@ __device__ __forceinline__ void k1_body(int R, const double* __restrict__ r, con
         Z.n_y = fmin(loglog_from_log(s, Z.log_z, 0.0, 1.0), 1.0);
         double acc = 0.0;
         if constexpr (ANY) {
-            for (int j = 0; j < g.nt; ++j)
+            for (int j = j0; j < j1; ++j)
                 acc += k1_theta_term_any(P, Z, s2t[j], s, c, H, rows, rapidity) * wtt[j];
         } else {
             Z.den_y = alpha_s64_den(Z.log_z, c);
             Z.a_y = 1.0 / Z.den_y;
             Z.ly = Z.log_z - P.log_r;
             Z.inv_y2 = 1.0 / (Z.z * Z.z);
-            for (int j = 0; j < g.nt; ++j) acc += k1_theta_term(P, Z, s2t[j], s, c) * wtt[j];
+            for (int j = j0; j < j1; ++j) acc += k1_theta_term(P, Z, s2t[j], s, c) * wtt[j];
         }
-        total += acc * ((2.0 * Z.z * Z.z) * (half * gwr[i]));
+        comp_add(total, total_err, acc * ((2.0 * Z.z * Z.z) * (half * gwr[i])));
     }
-    total = wave_sum(total);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = total;
+    wave_comp_sum(total, total_err);
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6][0] = total;
+        red[threadIdx.x >> 6][1] = total_err;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double t = 0.0;
-        for (int w = 0; w < kK1Threads / 64; ++w) t += red[w];
-        k1[p] = t;
+        double t = red[0][0], e = red[0][1];
+        for (int w = 1; w < kK1Threads / 64; ++w) comp_merge(t, e, red[w][0], red[w][1]);
+        k1[p] = t + e;
     }
 }
 
@@ -734,7 +771,7 @@ template <int MODE, bool HIST, bool FULL, bool ANYC>
 __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int p, int blk) {
     using T = typename std::conditional<MODE == NLO_PREC_FP32, float, double>::type;
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double red[4][4];
+    __shared__ double red[4][FULL ? 4 : 1][2];  // [wave][sum][value, error]
     __shared__ double wmax[4];
     const size_t pidx = (size_t)(p - 1) * a.bpp + blk;
     const NloLayout lay = nlo_layout(v.R, a.bins, a.spt, HIST, sizeof(T));
@@ -864,7 +901,9 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         vlo[d][1] = v1 ^ a.shift[d];
     }
 
+    // compensated per-lane sums (value, error): w, w^2, K2 and Kf parts
     double s1 = 0.0, s2 = 0.0, sk2 = 0.0, skf = 0.0;
+    double c1 = 0.0, c2 = 0.0, ck2 = 0.0, ckf = 0.0;
     // Each wave walks spt consecutive 64-point groups hi; the wave-uniform
     // high part of the gray code (direction bits 6..29) is built once and
     // then updated by one XOR per group: gray(h) ^ gray(h - 1) = lowest set
@@ -950,11 +989,11 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
                 k2 = f.k2 * jac;
                 kf = f.kf * jac;
             }
-            s1 += wv;
+            comp_add(s1, c1, wv);
             if constexpr (FULL) {
-                s2 += wv * wv;
-                sk2 += k2;
-                skf += kf;
+                comp_add(s2, c2, wv * wv);
+                comp_add(sk2, ck2, k2);
+                comp_add(skf, ckf, kf);
             }
         }
         if (HIST) {
@@ -968,23 +1007,14 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         j0 = 1;
     }
     for (int j = j0; j < a.spt; ++j) group(j, std::false_type{});
-    s1 = wave_sum(s1);
-    if constexpr (FULL) {  // otherwise the three stay 0
-        s2 = wave_sum(s2);
-        sk2 = wave_sum(sk2);
-        skf = wave_sum(skf);
-    }
-    if (lane == 0) {
-        red[wave][0] = s1;
-        red[wave][1] = s2;
-        red[wave][2] = sk2;
-        red[wave][3] = skf;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double t = 0.0;
-        for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
-        v.part[pidx * 4 + threadIdx.x] = t;
+    // the block's partials, each one rounding of its compensated pair
+    if constexpr (FULL) {
+        double t[4] = {s1, s2, sk2, skf}, e[4] = {c1, c2, ck2, ckf};
+        block_comp_sums<4>(t, e, red, v.part + pidx * 4);
+    } else {  // the other three read 0
+        double t[1] = {s1}, e[1] = {c1};
+        block_comp_sums<1>(t, e, red, v.part + pidx * 4);
+        if (threadIdx.x >= 1 && threadIdx.x < 4) v.part[pidx * 4 + threadIdx.x] = 0.0;
     }
     if (HIST) {
         // block max of |w| -> fixed-point scale; integer LDS sums are exact
@@ -1015,15 +1045,35 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
     }
 }
 
+// The evolver's FP32 instance (sum of w only) fits 72 VGPRs without spills:
+// 7 waves per SIMD instead of the 6 that its natural 76 would allow.
+template <int MODE, bool FULL, bool ANYC>
+constexpr int kNloWaves = (MODE == NLO_PREC_FP32 && !FULL && !ANYC) ? 7 : 1;
+
 template <int MODE, bool HIST, bool FULL, bool ANYC>
-__global__ void __launch_bounds__(256) nlo_kernel(NloArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kNloWaves<MODE, FULL, ANYC>))) nlo_kernel(NloArgs a) {
     const RunView v{a.R, a.r, a.N, a.t64, a.t32, a.edges, a.part};
     nlo_body<MODE, HIST, FULL, ANYC>(a, v, blockIdx.y + 1, blockIdx.x);
 }
 
-// Fixed-order sum of the per-block partials of each parent: each thread adds
-// its strided partials in order, then a fixed butterfly over lanes and a
-// fixed sum over waves.
+// Compensated fixed-order sums of the bpp partials (4 sums each) at
+// part[base ..]: each thread adds its strided partials in order, then a fixed
+// butterfly over lanes and a fixed sum over the (at most four) waves, all as
+// (value, error) pairs rounded once into out[0..3].  Block of <= 256 threads.
+__device__ __forceinline__ void reduce_partials4(const double* __restrict__ part, size_t base, int bpp,
+                                                 double (*red)[4][2], double* __restrict__ out) {
+    double t[4] = {0.0, 0.0, 0.0, 0.0}, e[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < bpp; b += blockDim.x) {
+        const double4 v = *reinterpret_cast<const double4*>(part + (base + b) * 4);
+        comp_add(t[0], e[0], v.x);
+        comp_add(t[1], e[1], v.y);
+        comp_add(t[2], e[2], v.z);
+        comp_add(t[3], e[3], v.w);
+    }
+    block_comp_sums<4>(t, e, red, out);
+}
+
+// Fixed-order sum of the per-block partials of each parent (reduce_partials4).
 __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bins, int hist,
                                                          const double* __restrict__ part,
                                                          const double* __restrict__ ph,
@@ -1031,7 +1081,7 @@ __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bin
                                                          double* __restrict__ sums,
                                                          double* __restrict__ hsum,
                                                          double* __restrict__ csum) {
-    __shared__ double red[4][4];
+    __shared__ double red[4][4][2];
     const int p = blockIdx.x;  // 0 .. R-1
     if (p == 0) {
         if (threadIdx.x < 4) sums[threadIdx.x] = 0.0;
@@ -1043,34 +1093,33 @@ __global__ void __launch_bounds__(256) nlo_reduce_kernel(int R, int bpp, int bin
         return;
     }
     const size_t base = (size_t)(p - 1) * bpp;
-    double t[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < bpp; b += blockDim.x) {
-        const double4 v = *reinterpret_cast<const double4*>(part + (base + b) * 4);
-        t[0] += v.x;
-        t[1] += v.y;
-        t[2] += v.z;
-        t[3] += v.w;
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c = 0; c < 4; ++c) {
-        const double v = wave_sum(t[c]);
-        if (lane == 0) red[wave][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double v = 0.0;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) v += red[w][threadIdx.x];
-        sums[(size_t)p * 4 + threadIdx.x] = v;
-    }
+    reduce_partials4(part, base, bpp, red, sums + (size_t)p * 4);
     if (hist) {
+        const size_t stride = (size_t)4 * bins;
         for (int k = threadIdx.x; k < 4 * bins; k += blockDim.x) {
+            const double* qh = ph + base * stride + k;
+            const double* qc = pc + base * stride + k;
             double th = 0.0, tc = 0.0;
-            for (int b = 0; b < bpp; ++b) {
-                th += ph[(base + b) * 4 * bins + k];
-                tc += pc[(base + b) * 4 * bins + k];
+            int b = 0;
+            for (; b + 8 <= bpp; b += 8) {  // eight loads in flight, added in block order
+                double vh[8], vc[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    vh[i] = qh[(size_t)(b + i) * stride];
+                    vc[i] = qc[(size_t)(b + i) * stride];
+                }
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    th += vh[i];
+                    tc += vc[i];
+                }
             }
-            hsum[(size_t)p * 4 * bins + k] = th;
-            csum[(size_t)p * 4 * bins + k] = tc;
+            for (; b < bpp; ++b) {
+                th += qh[(size_t)b * stride];
+                tc += qc[(size_t)b * stride];
+            }
+            hsum[(size_t)p * stride + k] = th;
+            csum[(size_t)p * stride + k] = tc;
         }
     }
 }
@@ -1132,7 +1181,7 @@ __global__ void rk_final_kernel(int R, double* __restrict__ N, const double* __r
 
 // ------------------------------------------------------------ batches
 
-__global__ void __launch_bounds__(64) spline_build_batch_kernel(const BatchRun* __restrict__ runs,
+__global__ void __launch_bounds__(kSplineThreads) spline_build_batch_kernel(const BatchRun* __restrict__ runs,
                                                                 const double* __restrict__ r,
                                                                 const double* __restrict__ N,
                                                                 int force_positive, double* __restrict__ t64,
@@ -1189,7 +1238,7 @@ struct BatchNlo {
 // run's slices; each item writes its own partial sums, so the result does not
 // depend on which block took it.
 template <int MODE, bool ANYC>
-__global__ void __launch_bounds__(256) nlo_batch_kernel(NloArgs a, BatchNlo w) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kNloWaves<MODE, false, ANYC>))) nlo_batch_kernel(NloArgs a, BatchNlo w) {
     __shared__ int64_t item_s;
     for (;;) {
         if (threadIdx.x == 0) item_s = (int64_t)atomicAdd(w.counter, 1ull);
@@ -1219,7 +1268,7 @@ __global__ void __launch_bounds__(256) nlo_reduce_batch_kernel(const BatchRun* _
                                                                const int32_t* __restrict__ node_run, int bpp,
                                                                const double* __restrict__ part,
                                                                double* __restrict__ sums) {
-    __shared__ double red[4][4];
+    __shared__ double red[4][4][2];
     const BatchRun run = runs[node_run[blockIdx.x]];
     const int p = blockIdx.x - run.node_off;
     if (p == 0) {
@@ -1227,25 +1276,7 @@ __global__ void __launch_bounds__(256) nlo_reduce_batch_kernel(const BatchRun* _
         return;
     }
     const size_t base = (size_t)(run.parent_off + p - 1) * bpp;
-    double t[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int b = threadIdx.x; b < bpp; b += blockDim.x) {
-        const double4 v = *reinterpret_cast<const double4*>(part + (base + b) * 4);
-        t[0] += v.x;
-        t[1] += v.y;
-        t[2] += v.z;
-        t[3] += v.w;
-    }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int c = 0; c < 4; ++c) {
-        const double v = wave_sum(t[c]);
-        if (lane == 0) red[wave][c] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        double v = 0.0;
-        for (int wv = 0; wv < (int)(blockDim.x >> 6); ++wv) v += red[wv][threadIdx.x];
-        sums[(size_t)blockIdx.x * 4 + threadIdx.x] = v;
-    }
+    reduce_partials4(part, base, bpp, red, sums + (size_t)blockIdx.x * 4);
 }
 
 __global__ void rhs_finalize_batch_kernel(int nodes, const BatchRun* __restrict__ runs,
@@ -1340,11 +1371,11 @@ int init_kernel_attributes() {
 
 int launch_spline_build(int R, const double* r, const double* N, int force_positive, double* t64,
                         float* t32, hipStream_t stream) {
-    const size_t smem = (size_t)7 * R * sizeof(double);
+    const size_t smem = (size_t)8 * R * sizeof(double);
     int rc = init_kernel_attributes();
     if (rc != NLO_OK) return rc;
-    hipLaunchKernelGGL(spline_build_kernel, dim3(1), dim3(64), smem, stream, R, r, N, force_positive,
-                       t64, t32);
+    hipLaunchKernelGGL(spline_build_kernel, dim3(1), dim3(kSplineThreads), smem, stream, R, r, N,
+                       force_positive, t64, t32);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }
@@ -1509,11 +1540,11 @@ int launch_rk_final(int R, double* N, const double* K0, const double* K1, const 
 
 int launch_spline_build_batch(const BatchView& b, const double* r, const double* N, int force_positive,
                               double* t64, float* t32, hipStream_t stream) {
-    const size_t smem = (size_t)7 * b.Rmax * sizeof(double);
+    const size_t smem = (size_t)8 * b.Rmax * sizeof(double);
     int rc = init_kernel_attributes();
     if (rc != NLO_OK) return rc;
-    hipLaunchKernelGGL(spline_build_batch_kernel, dim3(b.runs), dim3(64), smem, stream, b.runs_d, r, N,
-                       force_positive, t64, t32);
+    hipLaunchKernelGGL(spline_build_batch_kernel, dim3(b.runs), dim3(kSplineThreads), smem, stream, b.runs_d,
+                       r, N, force_positive, t64, t32);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }

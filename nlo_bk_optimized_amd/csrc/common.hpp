@@ -41,16 +41,68 @@ constexpr int kWave = 64;
 
 __host__ __device__ inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
-// --------------------------------------------------------- wave reductions
+// ------------------------------------------------- compensated FP64 sums
+//
+// A running sum is the unevaluated pair s + c.  Each addition is Knuth's
+// TwoSum, whose error term is the exact rounding error of s + v (for any
+// order and magnitudes of the operands, barring overflow); c gathers those
+// errors.  After n additions s + c is within about n u^2 sum |v| of the exact
+// sum (u = 2^-53), where plain recursive summation is within n u sum |v|; a
+// final rounding s + c adds u |sum|.
 
-// Fixed-order butterfly sum over the 64 lanes of a wavefront; every lane
-// receives the total.  The order of additions depends only on lane ids, so
-// the result is reproducible run to run.
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, kWave);
-    return v;
+__device__ __forceinline__ void comp_add(double& s, double& c, double v) {
+    const double t = s + v;
+    const double z = t - s;
+    c += (s - (t - z)) + (v - z);
+    s = t;
 }
+
+// (s, c) += (s2, c2).  Symmetric bit for bit: TwoSum's error term is exact
+// whatever the order of s and s2, and c + c2 is formed before it is added, so
+// both partners of a butterfly hold the same pair.
+__device__ __forceinline__ void comp_merge(double& s, double& c, double s2, double c2) {
+    const double t = s + s2;
+    const double z = t - s;
+    const double e = (s - (t - z)) + (s2 - z);
+    c = (c + c2) + e;
+    s = t;
+}
+
+// Fixed-order butterfly of compensated pairs over the 64 lanes; every lane
+// receives the total pair.
+__device__ __forceinline__ void wave_comp_sum(double& s, double& c) {
+#pragma unroll
+    for (int offset = 32; offset > 0; offset >>= 1) {
+        const double so = __shfl_xor(s, offset, kWave), co = __shfl_xor(c, offset, kWave);
+        comp_merge(s, c, so, co);
+    }
+}
+
+// Compensated sums of NV per-thread pairs (t[k], e[k]) over a block of at most
+// 256 threads: a lane butterfly per wave, then the waves in order through
+// red (shared, [waves][NV][2]); thread k < NV writes the rounded total of sum
+// k to out[k].  Every thread of the block must call it.
+template <int NV>
+__device__ __forceinline__ void block_comp_sums(double (&t)[NV], double (&e)[NV], double (*red)[NV][2],
+                                                double* __restrict__ out) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        wave_comp_sum(t[k], e[k]);
+        if (lane == 0) {
+            red[wave][k][0] = t[k];
+            red[wave][k][1] = e[k];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        double s = red[0][threadIdx.x][0], c = red[0][threadIdx.x][1];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) comp_merge(s, c, red[w][threadIdx.x][0], red[w][threadIdx.x][1]);
+        out[threadIdx.x] = s + c;
+    }
+}
+
+// --------------------------------------------------------- wave maxima
 
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll

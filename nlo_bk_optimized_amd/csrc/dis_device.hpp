@@ -312,6 +312,9 @@ __device__ __forceinline__ bool bk_initialized(double Y, const DisDipole& d) {
 // Y matches, else the local 7-point (9 at the top) natural spline of the
 // Y-linear table rows (:184-240); clamped to [0, 1].  Evaluated in FP64 (the
 // table and its interpolation are shared by both arithmetic paths).
+// SOLVE = false (the integrands): the local-spline table is required (the
+// host checks it), so no per-query solve is compiled in.
+template <bool SOLVE = true>
 __device__ __forceinline__ double bk_amplitude(double r, double Y, const DisDipole& d) {
     const double rmin = d.r[0], rmax = d.r[d.R - 1];
     if (r < rmin) r = rmin * 1.000001;
@@ -345,7 +348,7 @@ __device__ __forceinline__ double bk_amplitude(double r, double Y, const DisDipo
             v = (N11 * (r2 - r) * (Y2 - Y) + N12 * (r - r1) * (Y2 - Y) + N21 * (r2 - r) * (Y - Y1) +
                  N22 * (r - r1) * (Y - Y1)) / (dr * dY);
         }
-    } else if (d.local_coef) {
+    } else if (!SOLVE || d.local_coef) {
         // the local splines are linear in their data: the Y-linear
         // combination of the two rows' precomputed cubics on interval rind
         int rind = search_right(d.r, d.R, r);
@@ -360,7 +363,7 @@ __device__ __forceinline__ double bk_amplitude(double r, double Y, const DisDipo
         const double lo = c1[0] + t * (c1[1] + t * (c1[2] + t * c1[3]));
         const double up = c2[0] + t * (c2[1] + t * (c2[2] + t * c2[3]));
         v = lo + frac * (up - lo);
-    } else {
+    } else if constexpr (SOLVE) {
         const int rind = search_right(d.r, d.R, r);
         const int yind = search_right(d.Y, d.T, Y);
         const int y1 = yind < 0 ? 0 : yind;
@@ -379,10 +382,10 @@ __device__ __forceinline__ double bk_amplitude(double r, double Y, const DisDipo
     return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
 }
 
-template <typename T>
+template <typename T, bool SOLVE = true>
 __device__ __forceinline__ T dipole_amplitude(T r, T Y, const DisDipole& d) {
     if (d.kind == NLO_DIPOLE_GBW) return gbw_amplitude<T>(r, Y, d);
-    return (T)bk_amplitude((double)r, (double)Y, d);
+    return (T)bk_amplitude<SOLVE>((double)r, (double)Y, d);
 }
 
 // ----------------------------------------------------------- LO (a19)

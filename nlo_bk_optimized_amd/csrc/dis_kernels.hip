@@ -99,7 +99,7 @@ struct DisArgs {
 
 template <typename T>
 __device__ __forceinline__ T amp(const DisArgs& a, T r, T Y) {
-    return dipole_amplitude<T>(r, Y, a.dip);
+    return dipole_amplitude<T, false>(r, Y, a.dip);
 }
 
 // S/dis/observables.py:86-104
@@ -111,7 +111,7 @@ __device__ __forceinline__ T tripole_amp(const DisArgs& a, T x01, T x02, T x21, 
 }
 
 template <typename T, int K>
-__device__ T dis_value(const DisArgs& a, const double* u, const RuleView& rule) {
+__device__ __forceinline__ T dis_value(const DisArgs& a, const double* u, const RuleView& rule) {
     const T Q2 = (T)a.Q2;
     if constexpr (K == NLO_DIS_LO) {
         // S/dis/observables.py:345-352 and S/dis/lo.py:12-51
@@ -392,7 +392,7 @@ template <typename T, int K, bool HIST, typename Src>
 __device__ __forceinline__ void vegas_pass(const DisArgs& a, const VegasView& v, const Src& src, int blk) {
     constexpr int D = kDims[K];
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double red[4][2];
+    __shared__ double red[4][2][2];  // [wave][sum][value, error]
     __shared__ double wmax[4];
     __shared__ double ry[kMaxNested], rw[kMaxNested];
     RuleView rule{0, ry, rw};
@@ -421,14 +421,17 @@ __device__ __forceinline__ void vegas_pass(const DisArgs& a, const VegasView& v,
     const int lane = threadIdx.x & 63;
     const int wave = uniform(threadIdx.x >> 6);
     const uint32_t glo0 = (uint32_t)(lane ^ (lane >> 1));
-    uint32_t vlo[D][2];
+    // per-lane low part of the state (direction bits 0..5 of the lane's gray
+    // code) and direction 5, which odd groups add: the wave-uniform parity
+    // selects between two uniform values, so no per-lane indexing is needed
+    uint32_t vlo[D], d5[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
         uint32_t v0 = 0;
         for (int b = 0; b < 6; ++b)
             if ((glo0 >> b) & 1u) v0 ^= dir[d * NLO_SOBOL_BITS + b];
-        vlo[d][0] = v0 ^ src.shift(d);
-        vlo[d][1] = v0 ^ dir[d * NLO_SOBOL_BITS + 5] ^ src.shift(d);
+        vlo[d] = v0 ^ src.shift(d);
+        d5[d] = (uint32_t)uniform((int)dir[d * NLO_SOBOL_BITS + 5]);
     }
     const int64_t hi0 = (v.first >> 6) + ((int64_t)blk * 4 + wave) * v.spt;
     uint32_t vhi[D];
@@ -439,16 +442,16 @@ __device__ __forceinline__ void vegas_pass(const DisArgs& a, const VegasView& v,
         for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
             if (g & 1u)
 #pragma unroll
-                for (int d = 0; d < D; ++d) vhi[d] ^= dir[d * NLO_SOBOL_BITS + b];
+                for (int d = 0; d < D; ++d) vhi[d] ^= (uint32_t)uniform((int)dir[d * NLO_SOBOL_BITS + b]);
     }
-    double s1 = 0.0, s2 = 0.0;
+    double s1 = 0.0, s2 = 0.0, c1 = 0.0, c2 = 0.0;  // compensated sums of w and w^2
     for (int j = 0; j < v.spt; ++j) {
         const int64_t hi = hi0 + j;
         if (j > 0) {
             const int b = 6 + __builtin_ctzll((unsigned long long)hi);
             if (b < NLO_SOBOL_BITS)
 #pragma unroll
-                for (int d = 0; d < D; ++d) vhi[d] ^= dir[d * NLO_SOBOL_BITS + b];
+                for (int d = 0; d < D; ++d) vhi[d] ^= (uint32_t)uniform((int)dir[d * NLO_SOBOL_BITS + b]);
         }
         const int64_t sample = (hi << 6) + lane - v.first;
         const bool valid = sample < v.n;
@@ -458,7 +461,7 @@ __device__ __forceinline__ void vegas_pass(const DisArgs& a, const VegasView& v,
         uint64_t packed = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            uint32_t q = vhi[d] ^ vlo[d][par];
+            uint32_t q = vlo[d] ^ (par ? (vhi[d] ^ d5[d]) : vhi[d]);
             if (hi == 0 && lane == 0) q = src.first(d);
             // bin = min(floor(u B), B - 1), frac = u B - bin, u = q 2^-30
             // (map_importance_grid, S/dis/fixed_sobol.py:100-113)
@@ -475,25 +478,17 @@ __device__ __forceinline__ void vegas_pass(const DisArgs& a, const VegasView& v,
         double wv = 0.0;
         if (valid) {
             wv = (double)dis_value<T, K>(a, u, rule) * jac;
-            s1 += wv;
-            s2 += wv * wv;
+            comp_add(s1, c1, wv);
+            comp_add(s2, c2, wv * wv);
         }
         if (HIST) {
             st_w[threadIdx.x + j * 256] = valid ? fabs(wv) : 0.0;
             st_b[threadIdx.x + j * 256] = valid ? packed : ~0ull;
         }
     }
-    s1 = wave_sum(s1);
-    s2 = wave_sum(s2);
-    if (lane == 0) {
-        red[wave][0] = s1;
-        red[wave][1] = s2;
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        double t = 0.0;
-        for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
-        v.part[(size_t)blk * 2 + threadIdx.x] = t;
+    {
+        double t[2] = {s1, s2}, e[2] = {c1, c2};
+        block_comp_sums<2>(t, e, red, v.part + (size_t)blk * 2);
     }
     if (HIST) {
         // block max of |w| sets a fixed-point scale; the integer LDS sums are
@@ -562,26 +557,14 @@ __global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells
                                                            const double* __restrict__ gh,
                                                            const double* __restrict__ gc, double* __restrict__ sums,
                                                            double* __restrict__ hist, double* __restrict__ cnt) {
-    __shared__ double red[4][2];
+    __shared__ double red[4][2][2];  // [wave][sum][value, error]
     if (blockIdx.x == 0) {
-        double t0 = 0.0, t1 = 0.0;
+        double t[2] = {0.0, 0.0}, e[2] = {0.0, 0.0};
         for (int b = threadIdx.x; b < blocks; b += blockDim.x) {
-            t0 += part[2 * (size_t)b];
-            t1 += part[2 * (size_t)b + 1];
+            comp_add(t[0], e[0], part[2 * (size_t)b]);
+            comp_add(t[1], e[1], part[2 * (size_t)b + 1]);
         }
-        t0 = wave_sum(t0);
-        t1 = wave_sum(t1);
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) {
-            red[wave][0] = t0;
-            red[wave][1] = t1;
-        }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            double t = 0.0;
-            for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
-            sums[threadIdx.x] = t;
-        }
+        block_comp_sums<2>(t, e, red, sums);
         return;
     }
     if (hist == nullptr) return;
@@ -601,7 +584,7 @@ __global__ void __launch_bounds__(256) vegas_reduce_kernel(int blocks, int cells
 // sums in a fixed order.
 template <typename T, int K>
 __device__ __forceinline__ void gl_pass(const DisArgs& a, int points, int blk, double* __restrict__ out) {
-    __shared__ double red[4];
+    __shared__ double red[4][1][2];
     __shared__ double gx[128], gw[128];
     const int n = blk == 0 ? points : (points / 2 > 2 ? points / 2 : 2);
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -612,16 +595,13 @@ __device__ __forceinline__ void gl_pass(const DisArgs& a, int points, int blk, d
     }
     __syncthreads();
     const RuleView none{0, gx, gw};
-    double s = 0.0;
+    double s[1] = {0.0}, e[1] = {0.0};  // compensated
     for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
         const int i = idx / n, j = idx - i * n;
         const double u[2] = {gx[i], gx[j]};
-        s += gw[i] * gw[j] * (double)dis_value<T, K>(a, u, none);
+        comp_add(s[0], e[0], gw[i] * gw[j] * (double)dis_value<T, K>(a, u, none));
     }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) out[blk] = ((red[0] + red[1]) + red[2]) + red[3];
+    block_comp_sums<1>(s, e, red, out + blk);
 }
 
 template <typename T, int K>
@@ -729,7 +709,7 @@ __global__ void __launch_bounds__(64) vegas_batch_groups_kernel(BatchDev b, int 
 // Fixed-order sums of every active integral (vegas_reduce_kernel per
 // integral): block (v, 0) the two sums, blocks (v, r > 0) 256 histogram cells.
 __global__ void __launch_bounds__(256) vegas_batch_reduce_kernel(BatchDev b, int rblocks) {
-    __shared__ double red[4][2];
+    __shared__ double red[4][2][2];  // [wave][sum][value, error]
     const int v = blockIdx.x / rblocks, r = blockIdx.x - v * rblocks;
     const VItem& it = b.vi[v];
     if (!vitem_active(it, b.st[v])) return;
@@ -740,24 +720,12 @@ __global__ void __launch_bounds__(256) vegas_batch_reduce_kernel(BatchDev b, int
     double* cnt = g + cells;
     double* sums = g + 2 * cells;
     if (r == 0) {
-        double t0 = 0.0, t1 = 0.0;
+        double t[2] = {0.0, 0.0}, e[2] = {0.0, 0.0};
         for (int bb = threadIdx.x; bb < it.blocks; bb += blockDim.x) {
-            t0 += part[2 * (size_t)bb];
-            t1 += part[2 * (size_t)bb + 1];
+            comp_add(t[0], e[0], part[2 * (size_t)bb]);
+            comp_add(t[1], e[1], part[2 * (size_t)bb + 1]);
         }
-        t0 = wave_sum(t0);
-        t1 = wave_sum(t1);
-        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-        if (lane == 0) {
-            red[wave][0] = t0;
-            red[wave][1] = t1;
-        }
-        __syncthreads();
-        if (threadIdx.x < 2) {
-            double t = 0.0;
-            for (int w = 0; w < 4; ++w) t += red[w][threadIdx.x];
-            sums[threadIdx.x] = t;
-        }
+        block_comp_sums<2>(t, e, red, sums);
         return;
     }
     const int k = (r - 1) * blockDim.x + threadIdx.x;
@@ -982,6 +950,9 @@ int make_args(const nlo_dis_desc* desc, DisArgs& a, const double* ends = nullptr
     }
     NLO_REQUIRE(desc->nested_points >= 8 && desc->nested_points <= kMaxNested, "nested_points must be 8..%d",
                 kMaxNested);
+    NLO_REQUIRE(desc->dipole.kind != NLO_DIPOLE_BK_TABLE || desc->dipole.bilinear || desc->dipole.d_local_coef,
+                "a SPLINE_LINEAR BK-table dipole needs its local-spline table (nlo_dis_bk_local_splines) "
+                "for the integrands");
     a.nested_points = desc->nested_points;
     return NLO_OK;
 }

@@ -80,6 +80,28 @@ def _close(value, ref, tol, what, floor=1e-13):
     assert not bad.any(), (what, np.abs(value[m] - ref[m])[bad][:5], tol[m][bad][:5])
 
 
+def _fp32_input_sensitivity(f, x):
+    """Sum over the coordinates of |f(x with coordinate d moved by one FP32 ulp)
+    - f(x)|: what rounding the sample point to FP32 alone changes (a coordinate
+    next to 1 whose complement sets an exponential's argument can carry a
+    relative condition number of thousands)."""
+
+    base = np.nan_to_num(f(x), nan=0.0, posinf=0.0, neginf=0.0)
+    total = np.zeros_like(base)
+    for d in range(x.shape[1]):
+        xp = x.copy()
+        xp[:, d] = x[:, d] + np.spacing(x[:, d].astype(np.float32)).astype(np.float64)
+        total += np.abs(np.nan_to_num(f(xp), nan=0.0, posinf=0.0, neginf=0.0) - base)
+    return total
+
+
+def _fp32_tolerance(ref, tol, f, x):
+    """FP32 path: 1e-4 relative, the FP64 Bessel sensitivity scaled to FP32
+    rounding, and twice the sensitivity to FP32 rounding of the sample point."""
+
+    return 1e-4 * np.abs(ref) + (tol - 1e-13 * np.abs(ref)) * (2e-7 / 2e-15) + 2.0 * _fp32_input_sensitivity(f, x)
+
+
 # ------------------------------------------------------------ special functions
 
 
@@ -172,10 +194,10 @@ POINT_CASES = [("LO", "T"), ("LO", "L"), ("T0", "T"), ("T1", "T"), ("T2", "T"), 
                ("I3", "L"), ("I2_nested", "T"), ("I2_nested", "L"), ("I3_nested", "T"), ("I3_nested", "L")]
 
 
-def _oracle_values(kind, pol, q, setup, x, amp, Q2=9.0, xbj=1e-3, r_range=(1e-30, 1e3)):
+def _oracle_values(kind, pol, q, setup, x, amp, Q2=9.0, xbj=1e-3, r_range=(1e-30, 1e3), lo_quarks=(LIGHT, CHARM)):
     transverse = pol == "T"
     if kind == "LO":
-        return D.lo_integrand(x, Q2, xbj, transverse, (LIGHT, CHARM), amp, 1.0, setup)
+        return D.lo_integrand(x, Q2, xbj, transverse, lo_quarks, amp, 1.0, setup)
     if kind in ("T0", "T1", "T2", "Omega_L_const", "ab", "cd"):
         return D.dipole_integrand(x, Q2, xbj, transverse, q, kind, amp, 1.0, setup, *r_range)
     if kind.endswith("_nested"):
@@ -203,10 +225,44 @@ def test_integrands_match_oracle_gbw(dev, monkeypatch, kind, pol):
             got = dev.dis_integrand(desc, kind, cuda(x)).cpu().numpy()
             _close(got, ref, tol, (kind, pol, q, name, "fp64"))
             got32 = dev.dis_integrand(desc, kind, cuda(x), precision="fp32").cpu().numpy()
-            # FP32: 1e-4 relative plus the FP64 sensitivity scaled to a few FP32
-            # ulps (the light-quark differences also round their FP32 arguments)
-            tol32 = 1e-4 * np.abs(ref) + (tol - 1e-13 * np.abs(ref)) * (2e-7 / 2e-15)
+            tol32 = _fp32_tolerance(ref, tol, lambda xx: _oracle_values(kind, pol, q, setup, xx, amp), x)
             _close(got32, ref, tol32, (kind, pol, q, name, "fp32"), floor=1e-5)
+
+
+# the masses of the light and charm quarks of the 32 x 16 reduced cross-section grid
+LIGHT_GRID = D.QuarkSpec(0.14, math.sqrt(2 / 3))
+CHARM_GRID = D.QuarkSpec(1.27, 2 / 3)
+
+
+@pytest.mark.parametrize("Q2,xbj", [(100.0, 1e-5), (1.0, 1e-2), (100.0, 1e-2), (1.0, 1e-5)])
+@pytest.mark.parametrize("kind,pol", POINT_CASES)
+def test_integrands_match_oracle_at_grid_corners(dev, monkeypatch, kind, pol, Q2, xbj):
+    """The corners of the 32 x 16 (Q^2 in [1, 100], x in [1e-5, 1e-2]) grid with
+    its masses m = 0.14 and 1.27, FP64 and FP32, GBW dipole, default setup."""
+
+    from nlo_bk_optimized_amd import _lib
+    from nlo_bk_optimized_amd.dipole import GBW
+    from nlo_bk_optimized_amd.dis.config import Quark, QuarkType
+
+    dim = _lib.DIS_DIMENSIONS[kind]
+    rng = np.random.default_rng(zlib.crc32(f"{kind}{pol}{Q2}{xbj}".encode()))
+    x = rng.uniform(0, 1, (32 if kind.endswith("_nested") else 128, dim))
+    amp = lambda r, Y: D.gbw(r, Y, GBW_ORACLE)  # noqa: E731
+    gbw = GBW(Qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0)
+    cfg, setup = _configs()["default"]
+    light, charm = Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)
+    for q, quark in ((LIGHT_GRID, light), (CHARM_GRID, charm)):
+        quarks = (light, charm) if kind == "LO" else (quark,)
+        desc = dev.dis_desc(Q2, xbj, pol == "T", quarks, gbw, cfg)
+        def f(xx):
+            return _oracle_values(kind, pol, q, setup, xx, amp, Q2=Q2, xbj=xbj, lo_quarks=(LIGHT_GRID, CHARM_GRID))
+
+        ref, tol = _cancellation_tolerance(lambda: f(x), monkeypatch)
+        assert np.isfinite(ref).any()
+        got = dev.dis_integrand(desc, kind, cuda(x)).cpu().numpy()
+        _close(got, ref, tol, (kind, pol, q, Q2, xbj, "fp64"))
+        got32 = dev.dis_integrand(desc, kind, cuda(x), precision="fp32").cpu().numpy()
+        _close(got32, ref, _fp32_tolerance(ref, tol, f, x), (kind, pol, q, Q2, xbj, "fp32"), floor=1e-5)
 
 
 @pytest.mark.parametrize("kind,pol", [("T1", "T"), ("cd", "L"), ("I1", "T"), ("I2_nested", "L"),
