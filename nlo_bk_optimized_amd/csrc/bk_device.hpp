@@ -12,6 +12,8 @@
 namespace nlo {
 
 constexpr double kPi = 3.14159265358979323846;
+constexpr double kLn2 = 0.69314718055994530942;
+constexpr double kLog2e = 1.44269504088896340736;
 
 // Physics constants derived once on the host from nlo_bk_physics.
 struct BkConsts {
@@ -678,18 +680,19 @@ __device__ __forceinline__ NloStable<T> nlo_integrand_stable(T r, T log_r, T lz,
 }
 
 // ------------------------------------------- FP32 path: fast primitives
+//
+// The FP32 integrand works in base-2 logarithms: the VEGAS map hands it
+// t = log2 z (its bin table is scaled by log2 e), so z = 2^t and every
+// logarithm of a distance comes from the hardware log2 with no conversion;
+// the spline coordinate, the kernel's L and the coupling take their constants
+// in the same units.
 
-// e^x from the hardware 2^x with the product x log2(e) split so that its
-// rounding error (up to 1e-6 for |x| ~ 15) is carried: 2^(t + e) = 2^t (1 + e ln 2).
-__device__ __forceinline__ float fexp(float x) {
-    const float t = x * 1.44269502f;
-    const float e = fmaf(x, 1.44269502f, -t) + x * 1.9259629e-8f;
-    return __builtin_amdgcn_exp2f(t) * fmaf(e, 0.693147182f, 1.0f);
-}
+constexpr float kLn2f = 0.693147182f;
+constexpr float kLog2ef = 1.44269502f;
 
-// ln x from the hardware log2 (absolute error ~1e-7: used only where the
-// result is not itself a small difference).
-__device__ __forceinline__ float flog(float x) { return __builtin_amdgcn_logf(x) * 0.693147182f; }
+__device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+
+__device__ __forceinline__ float flog2(float x) { return __builtin_amdgcn_logf(x); }
 
 __device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -710,29 +713,32 @@ __device__ __forceinline__ float fsin_abs(float y) {
     return fmaf(v2 * v, p, v);
 }
 
-// ln(1 + u) for |u| < 0.5: 2 atanh(s), s = u / (2 + u), |s| < 0.2, series to s^9.
-__device__ __forceinline__ float flog1p_small(float u) {
+// log2(1 + u) / 2 for |u| < 0.5: atanh(s) log2(e) with s = u / (2 + u),
+// |s| < 0.2, the series to s^9 with log2(e) folded into its coefficients.
+__device__ __forceinline__ float fhalf_log2_1p_small(float u) {
     const float s = u * frcp(2.0f + u);
     const float s2 = s * s;
-    float p = fmaf(s2, 1.0f / 9.0f, 1.0f / 7.0f);
-    p = fmaf(s2, p, 0.2f);
-    p = fmaf(s2, p, 1.0f / 3.0f);
-    p = fmaf(s2, p, 1.0f);
-    return 2.0f * s * p;
+    float p = fmaf(s2, 0.160299450f, 0.206099287f);  // log2(e) / 9, / 7
+    p = fmaf(s2, p, 0.288539022f);                  // log2(e) / 5
+    p = fmaf(s2, p, 0.480898350f);                  // log2(e) / 3
+    p = fmaf(s2, p, 1.44269502f);                   // log2(e)
+    return s * p;
 }
 
-// x / (e^x - 1) and x e^x / (e^x - 1) for x = 2L: Bernoulli series
-// 1 - x/2 + x^2/12 - x^4/720 + x^6/30240 for |x| < 0.25, else from e^x
-// (no cancellation there; the second form avoids 1 - e^x for x << 0).
-__device__ __forceinline__ void fh_pair(float x, float& h, float& hm) {
+// h(L) = 2L / (e^{2L} - 1) and h(-L) = h(L) e^{2L} for x = 2L given also in
+// base 2 (x2 = 2L log2 e, so e^{2L} = 2^x2 with no rounding of the argument):
+// Bernoulli series 1 - x/2 + x^2/12 - x^4/720 + x^6/30240 for |x| < 0.25,
+// else from e^{2L} (no cancellation there; the second form avoids 1 - e^x
+// for x << 0).  x2 is clamped at 124 (e^86).
+__device__ __forceinline__ void fh_pair(float x, float x2, float& h, float& hm) {
     if (fabsf(x) < 0.25f) {
-        const float x2 = x * x;
-        float p = fmaf(x2, 1.0f / 30240.0f, -1.0f / 720.0f);
-        p = fmaf(x2, p, 1.0f / 12.0f);
-        h = fmaf(x2, p, fmaf(x, -0.5f, 1.0f));
+        const float xx = x * x;
+        float p = fmaf(xx, 1.0f / 30240.0f, -1.0f / 720.0f);
+        p = fmaf(xx, p, 1.0f / 12.0f);
+        h = fmaf(xx, p, fmaf(x, -0.5f, 1.0f));
         hm = h + x;
     } else {
-        const float E = fexp(fminf(x, 86.0f));
+        const float E = fexp2(fminf(x2, 124.0f));
         h = x * frcp(E - 1.0f);
         hm = h * E;
     }
@@ -744,33 +750,33 @@ __device__ __forceinline__ void fh_pair(float x, float& h, float& hm) {
 struct FastSpline {
     // rec[0] and rec[R] are constant records for ln d below ln r_0 (N = 2^-126,
     // i.e. 0 to FP32 precision, S = 1) and above ln r_{R-1} (N = 1, S = 0);
-    // rec[k], k = 1..R-1, is interval k - 1.  u = ln d * inv_h + off is the
-    // position in records, off = 1 - ln r_0 * inv_h.
+    // rec[k], k = 1..R-1, is interval k - 1.  u = log2 d * inv_h2 + off is the
+    // position in records: inv_h2 = ln 2 / h, off = 1 - ln r_0 / h.
     const float4* rec;
-    float inv_h, off, umax;  // umax = R
+    float inv_h2, off, umax;  // umax = R
 };
 
-// u clamped to [0, umax] by one median-of-three (log_d is finite: it is the
+// u clamped to [0, umax] by one median-of-three (log2_d is finite: it is the
 // log of a distance); for u >= 0 the fraction u - floor(u) is exact and
 // (int)u = floor(u), so t and the record index need no floor.
-__device__ __forceinline__ float fspline_log2n(const FastSpline& s, float log_d) {
-    const float u = __builtin_amdgcn_fmed3f(fmaf(log_d, s.inv_h, s.off), 0.0f, s.umax);
+__device__ __forceinline__ float fspline_log2n(const FastSpline& s, float log2_d) {
+    const float u = __builtin_amdgcn_fmed3f(fmaf(log2_d, s.inv_h2, s.off), 0.0f, s.umax);
     const float t = __builtin_amdgcn_fractf(u);
     const float4 c = s.rec[(int)u];
     return fmaf(t, fmaf(t, fmaf(t, c.w, c.z), c.y), c.x);
 }
 
-__device__ __forceinline__ float fspline_n(const FastSpline& s, float log_d) {
-    return __builtin_amdgcn_exp2f(fspline_log2n(s, log_d));
+__device__ __forceinline__ float fspline_n(const FastSpline& s, float log2_d) {
+    return fexp2(fspline_log2n(s, log2_d));
 }
 
 // N and S = 1 - N.  Near N = 1 the subtraction would keep only the absolute
 // precision of N, so S = -expm1(p ln 2) comes from the exponent p = log2 N
 // by its series when |p ln 2| < 1/4 (relative error < 2e-8), written in p:
 // expm1(p ln 2) = sum_k (ln 2)^k p^k / k!.
-__device__ __forceinline__ void fspline_ns(const FastSpline& s, float log_d, float& n, float& sv) {
-    const float p = fspline_log2n(s, log_d);
-    n = __builtin_amdgcn_exp2f(p);
+__device__ __forceinline__ void fspline_ns(const FastSpline& s, float log2_d, float& n, float& sv) {
+    const float p = fspline_log2n(s, log2_d);
+    n = fexp2(p);
     float e = fmaf(p, 1.54035304e-4f, 1.33335581e-3f);  // (ln 2)^6 / 720, (ln 2)^5 / 120
     e = fmaf(p, e, 9.61812911e-3f);                     // (ln 2)^4 / 24
     e = fmaf(p, e, 5.55041087e-2f);                     // (ln 2)^3 / 6
@@ -781,14 +787,15 @@ __device__ __forceinline__ void fspline_ns(const FastSpline& s, float log_d, flo
 
 // The FP32 K2 + Kf integrand: the cancellation-free formula of
 // nlo_integrand_stable, every transcendental on the fast primitives above,
-// cos(theta) = 1 - 2 sin^2(theta / 2), logs of distances from the log1p
-// forms, N from the local-coordinate spline.
+// cos(theta) = 1 - 2 sin^2(theta / 2), logs of distances (base 2) from the
+// log1p forms, N from the local-coordinate spline.  t1 = log2 z and
+// t2 = log2 z', log2_r = log2 r.
 template <bool ANYC>
 __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
-    float r, float log_r, float lz, float lz2, float tz, float tz2, const FastSpline& sp,
+    float r, float log2_r, float t1, float t2, float tz, float tz2, const FastSpline& sp,
     const NloCoupling<float>& cpl, float nf_over_nc, float nc, float norm, bool symmetrize) {
     NloStable<float> out{0.0f, 0.0f, 0.0f};
-    const float z = fexp(lz), z2 = fexp(lz2);
+    const float z = fexp2(t1), z2 = fexp2(t2);
     const float s1 = fsin_abs(0.5f * tz), s2 = fsin_abs(0.5f * tz2);
     const float s3 = fsin_abs(0.5f * (tz - tz2));
     const float xsq = fmaf(4.0f * r * z, s1 * s1, (r - z) * (r - z));
@@ -797,25 +804,27 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
     if (!(xsq >= 1e-37f) || !(x2sq >= 1e-37f) || !(zsq >= 1e-37f)) return out;
     // the distances enter only as reciprocals: 1 / X = rsq(X^2)
     const float iX = frsq(xsq), iX2 = frsq(x2sq), iZ = frsq(zsq);
-    // ln(X / Y): (X^2 - z^2) / z^2 = rho (rho - 2 cos t), cos t = 1 - 2 s^2
+    // log2(X / Y): (X^2 - z^2) / z^2 = rho (rho - 2 cos t), cos t = 1 - 2 s^2
     const float rho1 = r * frcp(z), rho2 = r * frcp(z2);
     const float u1 = rho1 * (rho1 - 2.0f + 4.0f * s1 * s1);
     const float u2 = rho2 * (rho2 - 2.0f + 4.0f * s2 * s2);
-    const float lxy = fabsf(u1) < 0.5f ? 0.5f * flog1p_small(u1) : 0.5f * flog(xsq) - lz;
-    const float lx2y2 = fabsf(u2) < 0.5f ? 0.5f * flog1p_small(u2) : 0.5f * flog(x2sq) - lz2;
-    const float log_x = lz + lxy, log_x2 = lz2 + lx2y2, log_zz = 0.5f * flog(zsq);
+    const float lxy = fabsf(u1) < 0.5f ? fhalf_log2_1p_small(u1) : 0.5f * flog2(xsq) - t1;
+    const float lx2y2 = fabsf(u2) < 0.5f ? fhalf_log2_1p_small(u2) : 0.5f * flog2(x2sq) - t2;
+    const float log_x = t1 + lxy, log_x2 = t2 + lx2y2, log_zz = 0.5f * flog2(zsq);
     float n_x, s_x, n_y, s_y, n_x2, s_x2, n_y2, s_y2;
     fspline_ns(sp, log_x, n_x, s_x);
-    fspline_ns(sp, lz, n_y, s_y);
+    fspline_ns(sp, t1, n_y, s_y);
     fspline_ns(sp, log_x2, n_x2, s_x2);
-    fspline_ns(sp, lz2, n_y2, s_y2);
+    fspline_ns(sp, t2, n_y2, s_y2);
     const float n_z = fspline_n(sp, log_zz);
     float dip = s_x * (n_z + n_y2 - n_y - n_z * n_y2);
     float dips = s_x2 * (n_z + n_y - n_y2 - n_z * n_y);
 
-    const float L = lxy - lx2y2;
+    // L = ln(X Y' / (X' Y)) from its base-2 form
+    const float L2 = lxy - lx2y2;
+    const float L = L2 * kLn2f;
     float h, hm;
-    fh_pair(2.0f * L, h, hm);
+    fh_pair(2.0f * L, 2.0f * L2, h, hm);
     const float aL = fabsf(L);
     float g;
     if (aL < 1.0f) {
@@ -852,18 +861,22 @@ __device__ __forceinline__ NloStable<float> nlo_integrand_fp32(
         const float dfs = s_y2 * (n_x2 - n_x);
         kfr = -(symmetrize ? (kf * df + kfs * dfs) * 0.5f : kf * df);
     }
-    const float log_min = fminf(fminf(fminf(log_r, log_x), lz), fminf(fminf(log_x2, lz2), log_zz));
+    // smallest distance, base 2
+    const float log_min = fminf(fminf(fminf(log2_r, log_x), t1), fminf(fminf(log_x2, t2), log_zz));
     float as;
     if (!ANYC || cpl.mode == 0) {  // smooth coupling at the smallest distance (production)
-        const float right = 5.0f * (cpl.log_scale0 - 2.0f * log_min);
-        const float hi = fmaxf(cpl.log_mu0, right);
-        // e^-|d| <= 1 is added to 1: the plain 2^x is accurate enough here
-        const float t = __builtin_amdgcn_exp2f(-fabsf(cpl.log_mu0 - right) * 1.44269502f);
-        as = frcp(cpl.b0 * (0.2f * (hi + flog(1.0f + t))));
+        // in base 2: right = 5 (ln s0 - 2 ln d) log2 e, mu = ln mu0 log2 e,
+        // alpha = 1 / (b0 0.2 ln 2 (max(mu, right) + log2(1 + 2^-|mu - right|)))
+        const float right = fmaf(-10.0f, log_min, cpl.log_scale0 * (5.0f * kLog2ef));
+        const float mu = cpl.log_mu0 * kLog2ef;
+        const float hi = fmaxf(mu, right);
+        // 2^-|d| <= 1 is added to 1: the plain log2 is accurate enough here
+        const float t = fexp2(-fabsf(mu - right));
+        as = frcp((cpl.b0 * (0.2f * kLn2f)) * (hi + flog2(1.0f + t)));
     } else if (cpl.mode == 1) {
         as = cpl.as_const;
     } else {
-        as = alpha_thresholds_log<float>(log_min, cpl.log_4c2);
+        as = alpha_thresholds_log<float>(log_min * kLn2f, cpl.log_4c2);
     }
     const float scale = (as * nc) * (as * nc) * norm;
     k2r *= scale;

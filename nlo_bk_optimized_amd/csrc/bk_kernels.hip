@@ -812,7 +812,7 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         const double x0 = v.t64[2], xl = v.t64[2 + v.R - 1];
         const double inv_h = (double)(v.R - 1) / (xl - x0);
         fsp.rec = rec4;
-        fsp.inv_h = (float)inv_h;
+        fsp.inv_h2 = (float)(inv_h * kLn2);  // the integrand's logarithms are base 2
         fsp.off = (float)(1.0 - x0 * inv_h);
         fsp.umax = (float)v.R;
     } else {
@@ -831,16 +831,20 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
         // of the mapped interval, formed in FP64 and rounded once: the sample
         // is then x = X_b + W_b frac with one more rounding, where forming
         // lo + w (left + width frac) in FP32 cancels (lo = ln(r_0 / 2) against
-        // the scaled edge) and leaves several ulp in ln z.  Wave 0 alone
-        // builds the table, so that only one wave of the block evaluates the
-        // two FP64 logarithms of the bounds.
+        // the scaled edge) and leaves several ulp in ln z.  The two log-r
+        // dimensions are tabulated in base 2 (times log2 e), so the sample is
+        // t = log2 z and z = 2^t exactly up to the hardware exp2; the Jacobian
+        // then carries (log2 e)^2, which the volume factor takes back.  Wave 0
+        // alone builds the table, so that only one wave of the block evaluates
+        // the two FP64 logarithms of the bounds.
         float2* E2 = reinterpret_cast<float2*>(smem + lay.edges);
         if (threadIdx.x < kWave) {
             for (int i = threadIdx.x; i < 4 * a.bins; i += kWave) {
                 const int d = i / a.bins, b = i - d * a.bins;
                 const double lo_d = d < 2 ? lo_r : 0.0, w_d = d < 2 ? w_r : 2.0 * kPi;
+                const double unit = d < 2 ? kLog2e : 1.0;
                 const double left = ep[d * (a.bins + 1) + b], right = ep[d * (a.bins + 1) + b + 1];
-                E2[i] = make_float2((float)(lo_d + w_d * left), (float)(w_d * (right - left)));
+                E2[i] = make_float2((float)((lo_d + w_d * left) * unit), (float)(w_d * (right - left) * unit));
             }
         }
     } else {
@@ -868,15 +872,18 @@ __device__ __forceinline__ void nlo_body(const NloArgs& a, const RunView& v, int
     const T wd[4] = {(T)w_r, (T)w_r, (T)(2.0 * kPi), (T)(2.0 * kPi)};
     T vol = T(1);
     if constexpr (MODE == NLO_PREC_FP32) {
-        const T bb = (T)a.bins;  // the widths W_b already carry the volume
-        vol = (bb * bb) * (bb * bb);
+        // the widths W_b already carry the volume; (ln 2)^2 undoes the base-2
+        // units of the two log-r widths (one rounding of B^4 (ln 2)^2)
+        const double bb = (double)a.bins;
+        vol = (T)((bb * bb) * (bb * bb) * (kLn2 * kLn2));
     } else {
         for (int d = 0; d < 4; ++d) vol = vol * wd[d];
     }
     const T r_t = (T)rp;
     // ln r_p as the spline table holds it (the same FP64 log of r_p)
     const double log_rp = v.t64[2 + p];
-    const T log_r_t = (T)log_rp;
+    // the FP32 integrand takes log2 r
+    const T log_r_t = MODE == NLO_PREC_FP32 ? (T)(log_rp * kLog2e) : (T)log_rp;
     NloCoupling<float> cpl32;
     NloCoupling<double> cpl64;
     if constexpr (ANYC) {

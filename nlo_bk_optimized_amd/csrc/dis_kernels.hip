@@ -665,8 +665,16 @@ __device__ __forceinline__ bool vitem_active(const VItem& it, const VState& st) 
 }
 
 // One pass of every active integral of kind K: work entry (integral, block).
+// Occupancy per instance.  FP64: at most 256 VGPRs, two waves per SIMD (T2
+// and the qqbar-g terms hold 270-400 at their natural allocation, one wave;
+// held to 256 the qqbar-g terms spill 68-268 B per lane and still run 22-30 %
+// faster).  FP32 nested I2 / I3: 128 VGPRs, four waves instead of three (I2
+// spills 28 B; 4 % off the FP32 batch).  The others keep their allocation.
 template <typename T, int K>
-__global__ void __launch_bounds__(256) vegas_batch_kernel(BatchDev b, const int2* __restrict__ work) {
+constexpr int kDisWaves = std::is_same<T, double>::value ? 2 : (K >= NLO_DIS_QG_I2_NESTED ? 4 : 1);
+
+template <typename T, int K>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kDisWaves<T, K>))) vegas_batch_kernel(BatchDev b, const int2* __restrict__ work) {
     const int2 w = work[blockIdx.x];
     const VItem& it = b.vi[w.x];
     const VState& st = b.st[w.x];
