@@ -28,11 +28,13 @@ struct K01 {
 // K0(x) and K1(x) for x >= 0 (S/numerics/special.py:18-23 calls ATen's).
 // x <= 2: the ascending series (A&S 9.6.11/9.6.13) with t = x^2/4,
 //   K0 = -(ln(x/2) + gamma) I0 + sum H_k t^k / k!^2,
-//   K1 = 1/x + (x/2)(ln(x/2) + gamma) sum t^k/(k!(k+1)!) - (x/4) sum (H_k + H_{k+1}) t^k/(k!(k+1)!);
+//   K1 = 1/x + (x/2)(ln(x/2) + gamma) sum t^k/(k!(k+1)!) - (x/4) sum (H_k + H_{k+1}) t^k/(k!(k+1)!),
+// 15 terms in FP64 by the term recurrence, 9 in FP32 as polynomials in t by
+// Horner's rule;
 // x > 2: K = e^-x / sqrt(x) * sum c_m T_m(4/x - 1) (Chebyshev fit of the
-// scaled function, coefficients from gen_bessel_tables.py).  FP32 takes the
-// series' constant divisors as products and 1/x, 1/sqrt(x) and ln from the
-// hardware approximations; FP64 keeps IEEE divisions.
+// scaled function, coefficients from gen_bessel_tables.py; Clenshaw in FP64,
+// the power-basis form by Horner in FP32).  FP32 takes 1/x, 1/sqrt(x) and ln
+// from the hardware approximations; FP64 keeps IEEE divisions.
 // Reciprocal, reciprocal square root and natural log: IEEE operations in
 // FP64; in FP32 the hardware approximations (about 1 ulp; the log with an
 // absolute error ~1e-7), where no FP32 result is a small difference of them.
@@ -59,39 +61,48 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
         return {v, v};
     }
     if (x <= T(2)) {
-        constexpr int terms = f64 ? 15 : 9;
         const T t = T(0.25) * x * x;
-        T a = T(1), b = T(1);  // t^k/k!^2, t^k/(k!(k+1)!)
-        T h = T(0), hn = T(1);  // H_k, H_{k+1}
-        T i0 = T(1), s0 = T(0), i1 = T(1), s1 = T(1);
+        T i0, s0, i1, s1;
+        if constexpr (f64) {
+            // FP64: 15 terms by the term recurrence (Horner's 60 double constants would
+            // occupy registers in the FP64 integrands)
+            T a = T(1), b = T(1);   // t^k/k!^2, t^k/(k!(k+1)!)
+            T h = T(0), hn = T(1);  // H_k, H_{k+1}
+            i0 = T(1), s0 = T(0), i1 = T(1), s1 = T(1);
 #pragma unroll
-        for (int k = 1; k < terms; ++k) {
-            if constexpr (f64) {
+            for (int k = 1; k < 15; ++k) {
                 a *= t / T(k * k);
                 b *= t / T(k * (k + 1));
-            } else {
-                a *= t * T(1.0 / double(k * k));
-                b *= t * T(1.0 / double(k * (k + 1)));
+                h = hn;
+                hn += T(1) / T(k + 1);
+                i0 += a;
+                s0 += h * a;
+                i1 += b;
+                s1 += (h + hn) * b;
             }
-            h = hn;
-            hn += T(1) / T(k + 1);
-            i0 += a;
-            s0 += h * a;
-            i1 += b;
-            s1 += (h + hn) * b;
-        }
-        if constexpr (f64) {
             const T lg = log(T(0.5) * x) + T(kEulerGamma);
             return {s0 - lg * i0, T(1) / x + T(0.5) * x * lg * i1 - T(0.25) * x * s1};
         } else {
+            // the four series as polynomials in t (Horner, exact rational
+            // coefficients rounded once; every term is positive)
+            constexpr int n = kBesselSeriesTerms32;
+            i0 = kBesselSeriesI0_32[n - 1], s0 = kBesselSeriesS0_32[n - 1];
+            i1 = kBesselSeriesI1_32[n - 1], s1 = kBesselSeriesS1_32[n - 1];
+#pragma unroll
+            for (int k = n - 2; k >= 0; --k) {
+                i0 = fmaf(i0, t, kBesselSeriesI0_32[k]);
+                s0 = fmaf(s0, t, kBesselSeriesS0_32[k]);
+                i1 = fmaf(i1, t, kBesselSeriesI1_32[k]);
+                s1 = fmaf(s1, t, kBesselSeriesS1_32[k]);
+            }
             const T lg = __builtin_amdgcn_logf(T(0.5) * x) * T(0.69314718055994531) + T(kEulerGamma);
             return {s0 - lg * i0, dis_rcp(x) + T(0.5) * x * lg * i1 - T(0.25) * x * s1};
         }
     }
     const T s = (f64 ? T(4) / x : T(4) * dis_rcp(x)) - T(1);
-    const T s2 = s + s;
-    T b0k0 = T(0), b1k0 = T(0), b0k1 = T(0), b1k1 = T(0);
     if constexpr (f64) {
+        const T s2 = s + s;
+        T b0k0 = T(0), b1k0 = T(0), b0k1 = T(0), b1k1 = T(0);
 #pragma unroll
         for (int m = kBesselK0Deg64; m >= 1; --m) {
             const T t0 = s2 * b0k0 - b1k0 + T(kBesselK0Cheb64[m]);
@@ -106,17 +117,13 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
         const T sc = exp(-x) / sqrt(x);
         return {g0 * sc, g1 * sc};
     } else {
+        // the degree-8 interpolant in the power basis of s (Horner)
+        T g0 = kBesselK0Mono32[kBesselK0Deg32], g1 = kBesselK1Mono32[kBesselK1Deg32];
 #pragma unroll
-        for (int m = kBesselK0Deg32; m >= 1; --m) {
-            const T t0 = s2 * b0k0 - b1k0 + kBesselK0Cheb32[m];
-            b1k0 = b0k0;
-            b0k0 = t0;
-            const T t1 = s2 * b0k1 - b1k1 + kBesselK1Cheb32[m];
-            b1k1 = b0k1;
-            b0k1 = t1;
+        for (int m = kBesselK0Deg32 - 1; m >= 0; --m) {
+            g0 = fmaf(g0, s, kBesselK0Mono32[m]);
+            g1 = fmaf(g1, s, kBesselK1Mono32[m]);
         }
-        const T g0 = s * b0k0 - b1k0 + kBesselK0Cheb32[0];
-        const T g1 = s * b0k1 - b1k1 + kBesselK1Cheb32[0];
         const T sc = exp(-x) * dis_rsqrt(x);
         return {g0 * sc, g1 * sc};
     }
