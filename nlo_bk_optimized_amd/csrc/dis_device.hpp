@@ -52,6 +52,16 @@ __device__ __forceinline__ T dis_rsqrt(T x) {
     else
         return T(1) / sqrt(x);
 }
+// Square root: IEEE in FP64; in FP32 the hardware one (about 1 ulp, without
+// the correctly rounded expansion's denormal scaling: every FP32 argument here
+// is a product of squared physical scales, zero or far above 1e-38).
+template <typename T>
+__device__ __forceinline__ T dis_sqrt(T x) {
+    if constexpr (sizeof(T) == 4)
+        return __builtin_amdgcn_sqrtf(x);
+    else
+        return sqrt(x);
+}
 
 template <typename T>
 __device__ __forceinline__ K01<T> bessel_k01(T x) {
@@ -114,7 +124,7 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
         }
         const T g0 = s * b0k0 - b1k0 + T(kBesselK0Cheb64[0]);
         const T g1 = s * b0k1 - b1k1 + T(kBesselK1Cheb64[0]);
-        const T sc = exp(-x) / sqrt(x);
+        const T sc = exp(-x) / dis_sqrt(x);
         return {g0 * sc, g1 * sc};
     } else {
         // the degree-8 interpolant in the power basis of s (Horner)
@@ -400,7 +410,7 @@ __device__ __forceinline__ T dipole_amplitude(T r, T Y, const DisDipole& d) {
 // |Psi_{T,L}|^2 of one quark without the dipole (S/dis/lo.py:25-47).
 template <typename T>
 __device__ __forceinline__ T lo_wavefunction(T r, T z, T Q2, T mf, T charge, bool transverse) {
-    const T eps = sqrt(Q2 * z * (T(1) - z) + mf * mf);
+    const T eps = dis_sqrt(Q2 * z * (T(1) - z) + mf * mf);
     const T a = r * eps;
     if (!(a >= T(1e-7) && a <= T(5e2))) return T(0);
     const K01<T> k = bessel_k01(a);
@@ -414,7 +424,7 @@ __device__ __forceinline__ T lo_wavefunction(T r, T z, T Q2, T mf, T charge, boo
 
 template <typename T>
 __device__ __forceinline__ T omega_t_v_u(T Q, T z, T mf) {
-    const T g = sqrt(T(1) + T(4) * (mf / Q) * (mf / Q));
+    const T g = dis_sqrt(T(1) + T(4) * (mf / Q) * (mf / Q));
     return (T(1) + T(1) / (T(2) * z)) * (log(T(1) - z) + g * log((T(1) + g) / (T(1) + g - T(2) * z))) -
            T(1) / (T(2) * z) * ((z + T(0.5)) * (T(1) - g) + mf * mf / (Q * Q)) *
                log((z * (T(1) - z) * Q * Q + mf * mf) / (mf * mf));
@@ -422,7 +432,7 @@ __device__ __forceinline__ T omega_t_v_u(T Q, T z, T mf) {
 
 template <typename T>
 __device__ __forceinline__ T omega_t_n_u(T Q, T z, T mf) {
-    const T g = sqrt(T(1) + T(4) * (mf / Q) * (mf / Q));
+    const T g = dis_sqrt(T(1) + T(4) * (mf / Q) * (mf / Q));
     return (T(1) + z - T(2) * z * z) / z * (log(T(1) - z) + g * log((T(1) + g) / (T(1) + g - T(2) * z))) +
            (T(1) - z) / z * ((T(0.5) + z) * (g - T(1)) - mf * mf / (Q * Q)) *
                log((z * (T(1) - z) * Q * Q + mf * mf) / (mf * mf));
@@ -431,7 +441,7 @@ __device__ __forceinline__ T omega_t_n_u(T Q, T z, T mf) {
 // L_dip (S/dis/longitudinal.py:133-143).
 template <typename T>
 __device__ __forceinline__ T l_dip(T Q2, T z, T mf) {
-    const T g = sqrt(T(1) + T(4) * mf * mf / Q2);
+    const T g = dis_sqrt(T(1) + T(4) * mf * mf / Q2);
     const T s = dilog(T(1) / (T(1) - (T(1) - g) / (T(2) * z))) + dilog(T(1) / (T(1) - (T(1) + g) / (T(2) * z))) +
                 dilog(T(1) / (T(1) - (T(1) - g) / (T(2) * (T(1) - z)))) +
                 dilog(T(1) / (T(1) - (T(1) + g) / (T(2) * (T(1) - z))));
@@ -442,7 +452,7 @@ __device__ __forceinline__ T l_dip(T Q2, T z, T mf) {
 // OmegaL_V (S/dis/longitudinal.py:118-130).
 template <typename T>
 __device__ __forceinline__ T omega_l_v(T Q2, T z, T mf) {
-    const T g = sqrt(T(1) + T(4) * mf * mf / Q2);
+    const T g = dis_sqrt(T(1) + T(4) * mf * mf / Q2);
     return T(1) / (T(2) * z) * (log(T(1) - z) + g * log((T(1) + g) / (T(1) + g - T(2) * z))) +
            T(1) / (T(2) * (T(1) - z)) * (log(z) + g * log((T(1) + g) / (T(1) + g - T(2) * (T(1) - z)))) +
            T(1) / (T(4) * z * (T(1) - z)) * (g - T(1) + T(2) * mf * mf / Q2) *
@@ -452,8 +462,8 @@ __device__ __forceinline__ T omega_l_v(T Q2, T z, T mf) {
 // ITdip_massive_0 (S/dis/transverse.py:27-46).
 template <typename T>
 __device__ __forceinline__ T it_dip_0(T Q2, T z1, T x01, T mf) {
-    const T Q = sqrt(Q2);
-    const T kz = sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
+    const T Q = dis_sqrt(Q2);
+    const T kz = dis_sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
     const T lz = log(z1 / (T(1) - z1));
     const T common = T(-kPiD * kPiD / 3.0 + kPiD * kPiD / 6.0) + T(0.5) * lz * lz + omega_t_v_u(Q, z1, mf) +
                      omega_t_v_u(Q, T(1) - z1, mf) + l_dip(Q2, z1, mf);
@@ -467,9 +477,9 @@ __device__ __forceinline__ T it_dip_0(T Q2, T z1, T x01, T mf) {
 
 template <typename T>
 __device__ __forceinline__ void it_v1_vms1_u(T Q, T z, T mf, T r, T xi, T& v1, T& vms1) {
-    const T kz = sqrt(z * (T(1) - z) * Q * Q + mf * mf);
+    const T kz = dis_sqrt(z * (T(1) - z) * Q * Q + mf * mf);
     const T dk = xi / (T(1) - xi) * (T(1) - z) * mf * mf;
-    const T sh = sqrt(kz * kz + dk);
+    const T sh = dis_sqrt(kz * kz + dk);
     const K01<T> ks = bessel_k01(r * sh), kb = bessel_k01(r * kz);
     const T pre = T(1) / xi * (T(2) * log(xi) / (T(1) - xi) - (T(1) + xi) / T(2));
     v1 = pre * (sh * ks.k1 - kz * kb.k1) -
@@ -480,8 +490,8 @@ __device__ __forceinline__ void it_v1_vms1_u(T Q, T z, T mf, T r, T xi, T& v1, T
 // ITdip_massive_1 (S/dis/transverse.py:79-148).
 template <typename T>
 __device__ __forceinline__ T it_dip_1(T Q2, T z1, T x01, T mf, T xi) {
-    const T Q = sqrt(Q2);
-    const T kz = sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
+    const T Q = dis_sqrt(Q2);
+    const T kz = dis_sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
     T a1, a2, b1, b2;
     it_v1_vms1_u(Q, z1, mf, x01, xi, a1, b1);
     it_v1_vms1_u(Q, T(1) - z1, mf, x01, xi, a2, b2);
@@ -497,7 +507,7 @@ __device__ __forceinline__ T k1_difference_stable(T r, T kb, T dk2, const K01<T>
     if (!(s2 > T(0) && kb > T(0))) return T(0);
     const T rel = fabs(dk2) / (b2 + fabs(dk2) + T(1e-300));
     if (rel < T(1e-5)) return T(-0.5) * r * dk2 * kbase.k0;
-    const T sh = sqrt(fmax(s2, T(0)));
+    const T sh = dis_sqrt(fmax(s2, T(0)));
     return sh * bessel_k1(r * sh) - kb * kbase.k1;
 }
 
@@ -506,10 +516,10 @@ template <typename T>
 __device__ __forceinline__ void it_v2_n_vms2_u(T Q, T z, T mf, T r, T ych, T yu, T& v2, T& n, T& vms2) {
     const T chi = z * ych;
     const T u = (T(1) - yu) / yu;
-    const T kz = sqrt(z * (T(1) - z) * Q * Q + mf * mf);
-    const T kc = sqrt(chi * (T(1) - chi) * Q * Q + mf * mf);
+    const T kz = dis_sqrt(z * (T(1) - z) * Q * Q + mf * mf);
+    const T kc = dis_sqrt(chi * (T(1) - chi) * Q * Q + mf * mf);
     const T dk = u * (T(1) - z) / (T(1) - chi) * kc * kc;
-    const T sh = sqrt(kz * kz + dk);
+    const T sh = dis_sqrt(kz * kz + dk);
     const K01<T> kb = bessel_k01(r * kz), ks = bessel_k01(r * sh);
     const T m2 = mf * mf, kc2 = kc * kc;
     const T up1 = u + T(1), uu = u / up1;
@@ -538,8 +548,8 @@ __device__ __forceinline__ void it_v2_n_vms2_u(T Q, T z, T mf, T r, T ych, T yu,
 // ITdip_massive_2 (S/dis/transverse.py:151-168).
 template <typename T>
 __device__ __forceinline__ T it_dip_2(T Q2, T z1, T x01, T mf, T ych, T yu) {
-    const T Q = sqrt(Q2);
-    const T kz = sqrt(z1 * (T(1) - z1) * Q * Q + mf * mf);
+    const T Q = dis_sqrt(Q2);
+    const T kz = dis_sqrt(z1 * (T(1) - z1) * Q * Q + mf * mf);
     T v2a, na, sa, v2b, nb, sb;
     it_v2_n_vms2_u(Q, z1, mf, x01, ych, yu, v2a, na, sa);
     it_v2_n_vms2_u(Q, T(1) - z1, mf, x01, ych, yu, v2b, nb, sb);
@@ -551,7 +561,7 @@ __device__ __forceinline__ T it_dip_2(T Q2, T z1, T x01, T mf, T ych, T yu) {
 // ILdip_massive_Omega_L_Const (S/dis/longitudinal.py:95-115).
 template <typename T>
 __device__ __forceinline__ T il_dip_const(T Q2, T z, T r, T mf) {
-    const T a = sqrt(Q2 * z * (T(1) - z) + mf * mf) * r;
+    const T a = dis_sqrt(Q2 * z * (T(1) - z) + mf * mf) * r;
     if (!(a >= T(1e-7))) return T(0);
     const T lz = log(z / (T(1) - z));
     const T k0 = bessel_k0(a);
@@ -562,10 +572,10 @@ __device__ __forceinline__ T il_dip_const(T Q2, T z, T r, T mf) {
 // ILdip_massive_Iab (S/dis/longitudinal.py:74-92).
 template <typename T>
 __device__ __forceinline__ T il_dip_ab(T Q2, T z1, T r, T mf, T xi) {
-    const T kz = sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
+    const T kz = dis_sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
     const T b1 = bessel_k0(kz * r);
-    const T b2 = bessel_k0(sqrt(kz * kz + (T(1) - z1) * xi / (T(1) - xi) * mf * mf) * r);
-    const T b3 = bessel_k0(sqrt(kz * kz + z1 * xi / (T(1) - xi) * mf * mf) * r);
+    const T b2 = bessel_k0(dis_sqrt(kz * kz + (T(1) - z1) * xi / (T(1) - xi) * mf * mf) * r);
+    const T b3 = bessel_k0(dis_sqrt(kz * kz + z1 * xi / (T(1) - xi) * mf * mf) * r);
     return T(4) * Q2 * (z1 * (T(1) - z1)) * (z1 * (T(1) - z1)) *
            (b1 / xi * (T(-2) * log(xi) / (T(1) - xi) + (T(1) + xi) / T(2)) * (T(2) * b1 - b2 - b3));
 }
@@ -573,7 +583,7 @@ __device__ __forceinline__ T il_dip_ab(T Q2, T z1, T r, T mf, T xi) {
 // ILdip_massive_Icd (S/dis/longitudinal.py:12-71).
 template <typename T>
 __device__ __forceinline__ T il_dip_cd(T Q2, T z1, T r, T mf, T xi, T x) {
-    const T kz = sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
+    const T kz = dis_sqrt(z1 * (T(1) - z1) * Q2 + mf * mf);
     const T a = kz * r;
     if (!(a >= T(1e-7))) return T(0);
     const T m2 = mf * mf, omx = T(1) - x, omxi = T(1) - xi, omz = T(1) - z1;
@@ -583,9 +593,9 @@ __device__ __forceinline__ T il_dip_cd(T Q2, T z1, T r, T mf, T xi, T x) {
     const T kap2 = xi * m2 / (omxi * omx * (x * omxi + xi / z1)) * (xi * omx + x * (T(1) - omz * omxi / z1));
     const T b1 = bessel_k0(a);
     const T v = b1 * m2 *
-                ((b1 - bessel_k0(r * sqrt(kz * kz / omx + kap1))) * clm1 /
+                ((b1 - bessel_k0(r * dis_sqrt(kz * kz / omx + kap1))) * clm1 /
                      (omxi * omx * (x * omxi + xi / omz) * (x / omx * kz * kz + kap1)) +
-                 (b1 - bessel_k0(r * sqrt(kz * kz / omx + kap2))) * clm2 /
+                 (b1 - bessel_k0(r * dis_sqrt(kz * kz / omx + kap2))) * clm2 /
                      (omxi * omx * (x * omxi + xi / z1) * (x / omx * kz * kz + kap2)));
     return T(4) * Q2 * (z1 * omz) * (z1 * omz) * v;
 }
@@ -652,15 +662,15 @@ template <typename T>
 __device__ __forceinline__ GValues<T> g_values(T q2, T m2, T x22, T x32, T om, T lam, T y) {
     const T num = y * lam * m2 + q2;
     const T den = y * x32 + om * x22;
-    const T arg = sqrt(num * den / y);
+    const T arg = dis_sqrt(num * den / y);
     const K01<T> k = bessel_k01(arg);
     const T k2 = k.k0 + T(2) * k.k1 / arg;
-    const T ratio = sqrt(num / den);
+    const T ratio = dis_sqrt(num / den);
     const T iy = T(1) / y;
     GValues<T> g;
     g.g11 = T(2) * k.k0 * iy;                      // y^-1 2 K0
-    g.g12 = T(4) * om * sqrt(iy) * iy * ratio * k.k1;  // y^-3/2 4 omega (num/den)^1/2 K1
-    g.g21 = T(4) * sqrt(iy) * ratio * k.k1;         // y^-1/2 4 (num/den)^1/2 K1
+    g.g12 = T(4) * om * dis_sqrt(iy) * iy * ratio * k.k1;  // y^-3/2 4 omega (num/den)^1/2 K1
+    g.g21 = T(4) * dis_sqrt(iy) * ratio * k.k1;         // y^-1/2 4 (num/den)^1/2 K1
     g.g22 = T(8) * om * iy * (num / den) * k2;       // y^-1 8 omega (num/den) K2
     return g;
 }
@@ -674,13 +684,17 @@ struct TMoments {
 
 // FP32: g_values with 1/y, 1/sqrt(y) and 1/den shared by the lam and lam = 0
 // evaluations of a leg and no divisions (den does not depend on lam).
+// The argument's square root is the hardware one (about 1 ulp; the argument is
+// a product of squared scales, far from the denormal range) and
+// (num/den)^1/2 = num y^-1/2 / arg reuses its reciprocal.
 __device__ __forceinline__ GValues<float> g_values_f32(float num, float den, float iden, float om, float iy,
                                                        float siy) {
-    const float arg = sqrtf(num * den * iy);
+    const float arg = __builtin_amdgcn_sqrtf(num * den * iy);
     const K01<float> k = bessel_k01(arg);
-    const float k2 = k.k0 + 2.0f * k.k1 * dis_rcp(arg);
+    const float iarg = dis_rcp(arg);
+    const float k2 = k.k0 + 2.0f * k.k1 * iarg;
     const float nd = num * iden;
-    const float ratio = sqrtf(nd);
+    const float ratio = num * siy * iarg;
     GValues<float> g;
     g.g11 = 2.0f * k.k0 * iy;
     g.g12 = 4.0f * om * siy * iy * ratio * k.k1;
@@ -736,7 +750,7 @@ __device__ __forceinline__ TFixed<T> t_fixed(const Tripole<T>& v, T m2, T z1, T 
     const T z0 = v.z0;
     f.pj = (T(2) * z0 * (z0 + z2) + z2 * z2) / ((z0 + z2) * (z0 + z2));
     f.pk = (T(2) * z1 * (z1 + z2) + z2 * z2) / ((z1 + z2) * (z1 + z2));
-    const T aj = sqrt(v.qj2 * v.sj), ak = sqrt(v.qk2 * v.sk);
+    const T aj = dis_sqrt(v.qj2 * v.sj), ak = dis_sqrt(v.qk2 * v.sk);
     const K01<T> bj = bessel_k01(aj), bk = bessel_k01(ak);
     f.k0j = bj.k0;
     f.k1j = bj.k1;
@@ -744,11 +758,11 @@ __device__ __forceinline__ TFixed<T> t_fixed(const Tripole<T>& v, T m2, T z1, T 
     f.k1k = bk.k1;
     f.g12j = bj.k0 / v.x2j2;
     f.g12k = bk.k0 / v.x2k2;
-    f.g22j = sqrt(v.qj2 / v.sj) * bj.k1 / v.x2j2;
-    f.g22k = sqrt(v.qk2 / v.sk) * bk.k1 / v.x2k2;
+    f.g22j = dis_sqrt(v.qj2 / v.sj) * bj.k1 / v.x2j2;
+    f.g22k = dis_sqrt(v.qk2 / v.sk) * bk.k1 / v.x2k2;
     const T mj = v.qj2 + m2 * v.lam_j, mk = v.qk2 + m2 * v.lam_k;  // qbar^2 + mf^2 (1 + lam)
-    f.hj = T(4) * sqrt(mj / v.sj) * bessel_k1(sqrt(mj * v.sj));
-    f.hk = T(4) * sqrt(mk / v.sk) * bessel_k1(sqrt(mk * v.sk));
+    f.hj = T(4) * dis_sqrt(mj / v.sj) * bessel_k1(dis_sqrt(mj * v.sj));
+    f.hk = T(4) * dis_sqrt(mk / v.sk) * bessel_k1(dis_sqrt(mk * v.sk));
     f.cross = tripole_cross(v, z1, z2);
     return f;
 }
@@ -762,7 +776,7 @@ __device__ __forceinline__ T it_qg_uvsub(const Tripole<T>& v, T m2, T z1, T z2, 
     const T eg = T(1.7810724179901979852);  // e^gamma
     const T ej = -exp(-v.x2j2 / (x01sq * eg)) / v.x2j2;
     const T ek = -exp(-v.x2k2 / (x01sq * eg)) / v.x2k2;
-    const K01<T> bj = bessel_k01(sqrt(x01sq * v.qj2)), bk = bessel_k01(sqrt(x01sq * v.qk2));
+    const K01<T> bj = bessel_k01(dis_sqrt(x01sq * v.qj2)), bk = bessel_k01(dis_sqrt(x01sq * v.qk2));
     const T jk = pj * (T(1) - T(2) * z1 * (T(1) - z1)) * v.qj2 * ej * bj.k1 * bj.k1 +
                  pk * (T(1) - T(2) * z0 * (T(1) - z0)) * v.qk2 * ek * bk.k1 * bk.k1;
     const T jkm = m2 * (pj * ej * bj.k0 * bj.k0 + pk * ek * bk.k0 * bk.k0);
@@ -792,8 +806,8 @@ __device__ __forceinline__ T it_qg_i1(const Tripole<T>& v, const TFixed<T>& f, T
 template <typename T>
 __device__ __forceinline__ T it_qg_i2(const Tripole<T>& v, const TFixed<T>& f, T m2, T z1, T z2, const TMoments<T>& m) {
     const T z0 = v.z0, a = z0 + z2, b = z1 + z2;
-    const T jk = f.pj * (T(1) - T(2) * z1 * (T(1) - z1)) * m.b22j * v.x3j2 / T(8) * sqrt(v.qj2 / v.sj) * f.k1j +
-                 f.pk * (T(1) - T(2) * z0 * (T(1) - z0)) * m.b22k * v.x3k2 / T(8) * sqrt(v.qk2 / v.sk) * f.k1k;
+    const T jk = f.pj * (T(1) - T(2) * z1 * (T(1) - z1)) * m.b22j * v.x3j2 / T(8) * dis_sqrt(v.qj2 / v.sj) * f.k1j +
+                 f.pk * (T(1) - T(2) * z0 * (T(1) - z0)) * m.b22k * v.x3k2 / T(8) * dis_sqrt(v.qk2 / v.sk) * f.k1k;
     const T jkm = m2 * (f.pj * m.b12j / T(4) * f.k0j + f.pk * m.b12k / T(4) * f.k0k);
     const T F = T(0.5) * (f.cross / (T(4) * a * b) * (m.b22k * f.g22j + m.b22j * f.g22k) -
                           a * z1 * z2 / (T(16) * b * b) * v.x2j_x3j * f.hk * m.b22j +
@@ -851,7 +865,7 @@ template <typename T>
 __device__ __forceinline__ T il_qg_uvsub(const Tripole<T>& v, T Q2, T z1, T z2, T x01sq) {
     const T z0 = v.z0;
     const T eg = T(1.7810724179901979852);
-    const T bk = bessel_k0(sqrt(v.qj2) * sqrt(x01sq)), bl = bessel_k0(sqrt(v.qk2) * sqrt(x01sq));
+    const T bk = bessel_k0(dis_sqrt(v.qj2) * dis_sqrt(x01sq)), bl = bessel_k0(dis_sqrt(v.qk2) * dis_sqrt(x01sq));
     const T tk = -z1 * z1 * (T(2) * z0 * (z0 + z2) + z2 * z2) / v.x2j2 * exp(-v.x2j2 / x01sq / eg) * bk * bk;
     const T tl = -z0 * z0 * (T(2) * z1 * (z1 + z2) + z2 * z2) / v.x2k2 * exp(-v.x2k2 / x01sq / eg) * bl * bl;
     return T(4) * Q2 * (tk + tl);

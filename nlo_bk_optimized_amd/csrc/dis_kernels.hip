@@ -150,7 +150,7 @@ __device__ __forceinline__ T dis_value(const DisArgs& a, const double* u, const 
         const T phi = (T)(2.0 * kPiD * u[4]);
         const T x01sq = x01 * x01, x02sq = x02 * x02;
         const T x21sq = fmax(x01sq + x02sq - T(2) * x01 * x02 * cos(phi), T(0));
-        const T x21 = sqrt(x21sq);
+        const T x21 = dis_sqrt(x21sq);
         const T maxr = (T)a.maxr;
         const T jac = (T(1) - z2min) * (T(1) - z1 - z2min) * x01 * x02 * maxr * maxr * T(2.0 * kPiD);
         const T Y = log((Q2 / (T)a.xbj) * z2 / T(kQ0sqr));
