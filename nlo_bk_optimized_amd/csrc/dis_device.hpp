@@ -29,8 +29,7 @@ struct K01 {
 // x <= 2: the ascending series (A&S 9.6.11/9.6.13) with t = x^2/4,
 //   K0 = -(ln(x/2) + gamma) I0 + sum H_k t^k / k!^2,
 //   K1 = 1/x + (x/2)(ln(x/2) + gamma) sum t^k/(k!(k+1)!) - (x/4) sum (H_k + H_{k+1}) t^k/(k!(k+1)!),
-// 15 terms in FP64 by the term recurrence, 9 in FP32 as polynomials in t by
-// Horner's rule;
+// the four sums as polynomials in t (15 terms FP64, 9 FP32) by Horner's rule;
 // x > 2: K = e^-x / sqrt(x) * sum c_m T_m(4/x - 1) (Chebyshev fit of the
 // scaled function, coefficients from gen_bessel_tables.py; Clenshaw in FP64,
 // the power-basis form by Horner in FP32).  FP32 takes 1/x, 1/sqrt(x) and ln
@@ -74,21 +73,18 @@ __device__ __forceinline__ K01<T> bessel_k01(T x) {
         const T t = T(0.25) * x * x;
         T i0, s0, i1, s1;
         if constexpr (f64) {
-            // FP64: 15 terms by the term recurrence (Horner's 60 double constants would
-            // occupy registers in the FP64 integrands)
-            T a = T(1), b = T(1);   // t^k/k!^2, t^k/(k!(k+1)!)
-            T h = T(0), hn = T(1);  // H_k, H_{k+1}
-            i0 = T(1), s0 = T(0), i1 = T(1), s1 = T(1);
-#pragma unroll
-            for (int k = 1; k < 15; ++k) {
-                a *= t / T(k * k);
-                b *= t / T(k * (k + 1));
-                h = hn;
-                hn += T(1) / T(k + 1);
-                i0 += a;
-                s0 += h * a;
-                i1 += b;
-                s1 += (h + hn) * b;
+            // the four series as polynomials in t by Horner's rule, the loop
+            // kept rolled: its 60 double coefficients are read per use (scalar
+            // loads) instead of occupying registers across the integrand
+            constexpr int n = kBesselSeriesTerms64;
+            i0 = kBesselSeriesI0_64[n - 1], s0 = kBesselSeriesS0_64[n - 1];
+            i1 = kBesselSeriesI1_64[n - 1], s1 = kBesselSeriesS1_64[n - 1];
+#pragma unroll 1
+            for (int k = n - 2; k >= 0; --k) {
+                i0 = fma(i0, t, kBesselSeriesI0_64[k]);
+                s0 = fma(s0, t, kBesselSeriesS0_64[k]);
+                i1 = fma(i1, t, kBesselSeriesI1_64[k]);
+                s1 = fma(s1, t, kBesselSeriesS1_64[k]);
             }
             const T lg = log(T(0.5) * x) + T(kEulerGamma);
             return {s0 - lg * i0, T(1) / x + T(0.5) * x * lg * i1 - T(0.25) * x * s1};
