@@ -211,25 +211,39 @@ def main() -> int:
 
     # Dominant kernel: the K2 + Kf integrand launch as the evolver runs it
     # (sum w only) with its per-parent reduce, timed with HIP events on the
-    # stream it is launched on (torch's current stream).
+    # stream it is launched on (torch's current stream), on the state the
+    # timed steps ended at (the graph replays of the timed region cannot be
+    # bracketed one kernel at a time) and, for reference, at the initial
+    # condition.  The GPU clock drops in the pause before these launches and
+    # ramps back over ~20 ms of load (the kernel trace shows the first
+    # launches after a pause up to 25 % slower than the back-to-back launches
+    # of the timed steps), so 20 untimed launches precede each timed block.
     setup = fp32["setup"]
     config = fp32["config"]
-    t64, t32 = dev.spline_build(setup.r, setup.N0)
-    active0 = int(((torch.arange(setup.r.numel(), device=device) >= 1) & (setup.N0 <= 0.99999)).sum())
-    for _ in range(2):
-        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32",
-                   sum_only=True)
     stream = torch.cuda.current_stream()
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for _ in range(args.kernel_launches):
-        dev.bk_nlo(setup.r, setup.N0, t64, t32, setup.edges, setup.table, n, config, precision="fp32",
-                   sum_only=True)
-    stop.record(stream)
-    stop.synchronize()
-    launch_ms = start.elapsed_time(stop) / args.kernel_launches
-    achieved = FLOPS_PER_SAMPLE * active0 * n / (launch_ms * 1e-3) / 1e12
-    log(f"nlo fp32 launch: {launch_ms:.3f} ms for {active0} parents x {n} samples -> {achieved:.2f} TFLOP/s")
+    idx = torch.arange(setup.r.numel(), device=device)
+
+    def time_launches(N):
+        t64, t32 = dev.spline_build(setup.r, N)
+        active = int(((idx >= 1) & (N <= 0.99999)).sum())
+        for _ in range(20):
+            dev.bk_nlo(setup.r, N, t64, t32, setup.edges, setup.table, n, config, precision="fp32", sum_only=True)
+        launches = max(args.kernel_launches, 10)
+        start.record(stream)
+        for _ in range(launches):
+            dev.bk_nlo(setup.r, N, t64, t32, setup.edges, setup.table, n, config, precision="fp32", sum_only=True)
+        stop.record(stream)
+        stop.synchronize()
+        ms = start.elapsed_time(stop) / launches
+        return ms, active, FLOPS_PER_SAMPLE * active * n / (ms * 1e-3) / 1e12
+
+    launch_ms, active_end, achieved = time_launches(fp32["N"])
+    launch_ms0, active0, achieved0 = time_launches(setup.N0)
+    log(f"nlo fp32 launch at the end of the timed steps: {launch_ms:.3f} ms for {active_end} parents x {n} "
+        f"samples -> {achieved:.2f} TFLOP/s; at the initial condition {launch_ms0:.3f} ms for {active0} "
+        f"parents -> {achieved0:.2f} TFLOP/s")
+    t64, t32 = dev.spline_build(setup.r, setup.N0)
 
     # K1 per right-hand side on the initial condition: the fixed 8 x 24 rule
     # the evolution above uses, and the nested adaptive Gauss-Kronrod rule the
@@ -722,9 +736,14 @@ def main() -> int:
             "traffic": measured_traffic(args.rpoints, args.samples_log2),
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of nlo_kernel<FP32, sum only> on this "
                               "workload, calibrated on a device copy (profiles/nlo_kernel_fp32_256r_counters.md)",
-            "kernel": "nlo_kernel<FP32, sum only> + per-parent reduce (the evolver's instance)",
+            "kernel": "nlo_kernel<FP32, sum only> + per-parent reduce (the evolver's instance), on the state "
+                      "the timed steps ended at",
             "launch_ms": launch_ms,
+            "active_parents": active_end,
             "flops_per_sample": FLOPS_PER_SAMPLE,
+            "launch_ms_initial_condition": launch_ms0,
+            "achieved_initial_condition": achieved0,
+            "frac_initial_condition": achieved0 / FP32_PEAK_TFLOPS,
         },
         "cpu_baseline": cpu,
         "nonfinite": fp32["nonfinite"],
