@@ -29,7 +29,7 @@ struct K01 {
 // x <= 2: the ascending series (A&S 9.6.11/9.6.13) with t = x^2/4,
 //   K0 = -(ln(x/2) + gamma) I0 + sum H_k t^k / k!^2,
 //   K1 = 1/x + (x/2)(ln(x/2) + gamma) sum t^k/(k!(k+1)!) - (x/4) sum (H_k + H_{k+1}) t^k/(k!(k+1)!),
-// the four sums as polynomials in t (15 terms FP64, 9 FP32) by Horner's rule;
+// the four sums as polynomials in t (15 terms FP64, 8 FP32) by Horner's rule;
 // x > 2: K = e^-x / sqrt(x) * sum c_m T_m(4/x - 1) (Chebyshev fit of the
 // scaled function, coefficients from gen_bessel_tables.py; Clenshaw in FP64,
 // the power-basis form by Horner in FP32).  FP32 takes 1/x, 1/sqrt(x) and ln
