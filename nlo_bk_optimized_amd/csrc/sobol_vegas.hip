@@ -17,10 +17,17 @@ namespace {
 
 // ------------------------------------------------------------------ Sobol
 
-// One wave covers 64 consecutive sample indices n = hi * 64 + lane.  Then
-// gray(n) = (gray(hi) << 6) ^ (lane ^ (lane >> 1) ^ ((hi & 1) << 5)), so the
-// state splits into a wave-uniform part (direction bits 6..29, scalar ALU)
-// and a per-lane part (bits 0..5) fixed for the whole launch.
+// One wave covers 64 consecutive sample indices n = hi * 64 + lane per group.
+// Then gray(n) = (gray(hi) << 6) ^ (lane ^ (lane >> 1) ^ ((hi & 1) << 5)), so
+// the state splits into a wave-uniform part (direction bits 6..29, scalar
+// ALU) and a per-lane part (bits 0..5) that takes one of two values by the
+// parity of hi.  Each wave walks kSobolGroups consecutive groups: the
+// uniform part is built once from gray(hi) and then updated by one XOR per
+// group (gray(hi) ^ gray(hi - 1) = the lowest set bit of hi), so the work per
+// point is a select, the conversion and the store, and a wave writes one
+// contiguous span.
+constexpr int kSobolGroups = 16;
+
 template <int DIM>
 __global__ void __launch_bounds__(256) sobol_draw_kernel(nlo_sobol_table t, int64_t first,
                                                           int64_t n, int32_t kind, int vec, void* out) {
@@ -29,58 +36,77 @@ __global__ void __launch_bounds__(256) sobol_draw_kernel(nlo_sobol_table t, int6
     const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64);
     const int64_t hi_begin = first >> 6;
     const int64_t hi_end = (first + n + 63) >> 6;
-    for (int64_t hi = hi_begin + wave_global; hi < hi_end; hi += waves) {
-        const uint64_t idx = ((uint64_t)hi << 6) | (uint64_t)lane;
-        const int64_t row = (int64_t)idx - first;
-        const bool live = row >= 0 && row < n;
-        const uint32_t glo = (uint32_t)(lane ^ (lane >> 1)) ^ ((uint32_t)(hi & 1) << 5);
-        uint32_t q[DIM];
-        // wave-uniform high part: direction bits 6..29 of gray(hi)
+    // per-lane low part for even and odd hi
+    const uint32_t glo = (uint32_t)(lane ^ (lane >> 1));
+    uint32_t lo0[DIM], lo1[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) q[d] = t.shift[d];
-        uint64_t g = (uint64_t)hi ^ ((uint64_t)hi >> 1);
-        for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
-            if (g & 1u) {
+    for (int d = 0; d < DIM; ++d) {
+        uint32_t v = 0u;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) q[d] ^= t.direction[d][b];
-            }
+        for (int b = 0; b < 6; ++b) v ^= ((glo >> b) & 1u) ? t.direction[d][b] : 0u;
+        lo0[d] = v;
+        lo1[d] = v ^ t.direction[d][5];
+    }
+    for (int64_t base = hi_begin + wave_global * kSobolGroups; base < hi_end; base += waves * kSobolGroups) {
+        // wave-uniform high part: shift ^ direction bits 6..29 of gray(base)
+        uint32_t qh[DIM];
 #pragma unroll
-        for (int b = 0; b < 6; ++b) {
-            const bool on = (glo >> b) & 1u;
+        for (int d = 0; d < DIM; ++d) qh[d] = t.shift[d];
+        {
+            uint64_t g = (uint64_t)base ^ ((uint64_t)base >> 1);
+            for (int b = 6; g && b < NLO_SOBOL_BITS; ++b, g >>= 1)
+                if (g & 1u) {
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) q[d] ^= on ? t.direction[d][b] : 0u;
-        }
-        if (!live) continue;
-        const size_t o = (size_t)row * DIM;
-        if constexpr (DIM == 4) {
-            if (vec) {
-                // 16-byte aligned output: one 16- or 32-byte store per lane, so
-                // the wave writes one contiguous span
-                if (kind == NLO_SOBOL_U32) {
-                    reinterpret_cast<uint4*>(static_cast<uint32_t*>(out) + o)[0] = make_uint4(q[0], q[1], q[2], q[3]);
-                } else if (kind == NLO_SOBOL_F32) {
-                    reinterpret_cast<float4*>(static_cast<float*>(out) + o)[0] =
-                        make_float4((float)q[0] * 0x1p-30f, (float)q[1] * 0x1p-30f, (float)q[2] * 0x1p-30f,
-                                    (float)q[3] * 0x1p-30f);
-                } else {
-                    double v[4];
-                    for (int d = 0; d < 4; ++d) v[d] = (double)(idx == 0 ? t.first[d] : q[d]) * 0x1p-30;
-                    double2* po = reinterpret_cast<double2*>(static_cast<double*>(out) + o);
-                    po[0] = make_double2(v[0], v[1]);
-                    po[1] = make_double2(v[2], v[3]);
+                    for (int d = 0; d < DIM; ++d) qh[d] ^= t.direction[d][b];
                 }
-                continue;
-            }
         }
+        const int64_t stop = base + kSobolGroups < hi_end ? base + kSobolGroups : hi_end;
+        for (int64_t hi = base; hi < stop; ++hi) {
+            if (hi > base) {
+                const int b = 6 + __builtin_ctzll((unsigned long long)hi);
+                if (b < NLO_SOBOL_BITS) {
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-            if (kind == NLO_SOBOL_U32) {
-                static_cast<uint32_t*>(out)[o + d] = q[d];
-            } else if (kind == NLO_SOBOL_F32) {
-                static_cast<float*>(out)[o + d] = (float)q[d] * 0x1p-30f;
-            } else {
-                const uint32_t qq = idx == 0 ? t.first[d] : q[d];
-                static_cast<double*>(out)[o + d] = (double)qq * 0x1p-30;
+                    for (int d = 0; d < DIM; ++d) qh[d] ^= t.direction[d][b];
+                }
+            }
+            const uint64_t idx = ((uint64_t)hi << 6) | (uint64_t)lane;
+            const int64_t row = (int64_t)idx - first;
+            if (!(row >= 0 && row < n)) continue;
+            const bool odd = (hi & 1) != 0;
+            uint32_t q[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) q[d] = qh[d] ^ (odd ? lo1[d] : lo0[d]);
+            const size_t o = (size_t)row * DIM;
+            if constexpr (DIM == 4) {
+                if (vec) {
+                    // 16-byte aligned output: one 16- or 32-byte store per lane, so
+                    // the wave writes one contiguous span
+                    if (kind == NLO_SOBOL_U32) {
+                        reinterpret_cast<uint4*>(static_cast<uint32_t*>(out) + o)[0] = make_uint4(q[0], q[1], q[2], q[3]);
+                    } else if (kind == NLO_SOBOL_F32) {
+                        reinterpret_cast<float4*>(static_cast<float*>(out) + o)[0] =
+                            make_float4((float)q[0] * 0x1p-30f, (float)q[1] * 0x1p-30f, (float)q[2] * 0x1p-30f,
+                                        (float)q[3] * 0x1p-30f);
+                    } else {
+                        double v[4];
+                        for (int d = 0; d < 4; ++d) v[d] = (double)(idx == 0 ? t.first[d] : q[d]) * 0x1p-30;
+                        double2* po = reinterpret_cast<double2*>(static_cast<double*>(out) + o);
+                        po[0] = make_double2(v[0], v[1]);
+                        po[1] = make_double2(v[2], v[3]);
+                    }
+                    continue;
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                if (kind == NLO_SOBOL_U32) {
+                    static_cast<uint32_t*>(out)[o + d] = q[d];
+                } else if (kind == NLO_SOBOL_F32) {
+                    static_cast<float*>(out)[o + d] = (float)q[d] * 0x1p-30f;
+                } else {
+                    const uint32_t qq = idx == 0 ? t.first[d] : q[d];
+                    static_cast<double*>(out)[o + d] = (double)qq * 0x1p-30;
+                }
             }
         }
     }
@@ -345,7 +371,8 @@ int nlo_sobol_draw(const nlo_sobol_table* table, int64_t first_index, int64_t n,
     if (n == 0) return NLO_OK;
     const int64_t groups = ((first_index + n + 63) >> 6) - (first_index >> 6);
     const int64_t waves_per_block = 4;
-    int64_t blocks = (groups + waves_per_block - 1) / waves_per_block;
+    const int64_t chunks = (groups + nlo::kSobolGroups - 1) / nlo::kSobolGroups;
+    int64_t blocks = (chunks + waves_per_block - 1) / waves_per_block;
     if (blocks > 65536) blocks = 65536;
     const int vec = table->dim == 4 && (reinterpret_cast<uintptr_t>(d_out) & 15) == 0;
 #define NLO_SOBOL_DIM(D)                                                                                \
