@@ -569,6 +569,66 @@ size_t nlo_dis_batch_workspace(const nlo_dis_batch_desc* desc);
 int nlo_dis_batch_run(const nlo_dis_batch_desc* desc, double* d_results, void* d_workspace,
                       size_t workspace_bytes, hipStream_t stream);
 
+/* ---------------------------------------- GBW fixed-Sobol qqbar-g estimator */
+
+/* Paired-endpoint map of unit-cube points (dis_sobol_endpoint_map,
+ * K/dis/sampling/kernel.cu:15-81, and its Python form
+ * paired_endpoint_inputs, S/dis/fixed_sobol.py:116-158): each of n points
+ * (n x 5, row-major) is mapped through the importance grid d_edges
+ * (5 x (bins + 1)) and split into the sectors of the gluon plane nearer
+ * endpoint 0 (d_first) and endpoint 1 (d_second, re-expressed about endpoint
+ * 0), with weights inverse density x sector Jacobian.  Element type float
+ * (NLO_PREC_FP32, the reference's dtype) or double (NLO_PREC_FP64). */
+int nlo_dis_endpoint_map(int32_t precision, int64_t n, const void* d_points, const void* d_edges,
+                         int32_t bins, double maxr, int32_t fold_angle, void* d_first,
+                         void* d_first_weight, void* d_second, void* d_second_weight,
+                         hipStream_t stream);
+
+#define NLO_FIXED_SOBOL_MAX_ROUNDS 8
+#define NLO_FIXED_SOBOL_MAX_REPLICATES 32
+
+/* One estimator: one quark's qqbar-g term (I1 + I2 + I3) at one point and
+ * polarisation.  GBW dipole; quark 0 of desc; nested_points inner nodes. */
+typedef struct nlo_dis_fixed_sobol_item {
+    nlo_dis_desc desc;
+    int32_t table_set; /* its point sets: tables[table_set * (training_rounds + replicates) + k],
+                          training round k < training_rounds, then the replicates */
+    int32_t reserved;
+} nlo_dis_fixed_sobol_item;
+
+typedef struct nlo_dis_fixed_sobol_desc {
+    int32_t n_items;
+    int32_t n_tables;                     /* a multiple of training_rounds + replicates */
+    const nlo_dis_fixed_sobol_item* items; /* host array */
+    const nlo_sobol_table* tables;        /* host array of 5-D tables (make_sobol_points: seeds
+                                             seed + k for training, seed + 10000 + k for replicates) */
+    int32_t training_samples;             /* power of two (65 536) */
+    int32_t training_rounds;              /* 1..NLO_FIXED_SOBOL_MAX_ROUNDS (3); 0 with d_grid */
+    int32_t samples_per_replicate;        /* power of two (32 768) */
+    int32_t replicates;                   /* 2..NLO_FIXED_SOBOL_MAX_REPLICATES (7) */
+    int32_t bins;                         /* grid bins per dimension, 2..255 (32) */
+    int32_t fold_angle;                   /* 1: the angle folded onto [0, pi) */
+    int32_t precision;                    /* NLO_PREC_FP32 (the reference's float32 path; I1 in FP64)
+                                             or NLO_PREC_FP64 */
+    int32_t reserved;
+    double i1_weight;                     /* weight of I1 in the training score (1.0) */
+    double epsrel;                        /* converged: error <= epsrel |value| */
+    const double* d_grid;                 /* NULL, or device n_items x 5 x (bins + 1) frozen grids: no
+                                             training (training_rounds must be 0; evaluations count the
+                                             replicates only), integrate_triple_fixed_sobol alone */
+} nlo_dis_fixed_sobol_desc;
+
+/* gbw_qg_fixed_sobol (S/dis/fixed_sobol.py:274-368) for every item at once:
+ * learn_triple_importance_grid (:161-210) over the training sets, then
+ * integrate_triple_fixed_sobol (:213-271) over the replicates.  d_results:
+ * n_items x 4 x 4 doubles, rows I1, I2, I3 and their per-replicate sum, each
+ * (mean, standard error over replicates, integrand evaluations, converged
+ * 0 / 1).  d_edges (may be NULL): n_items x 5 x (bins + 1), the learned
+ * grids.  Returns after the stream has finished. */
+size_t nlo_dis_fixed_sobol_workspace(const nlo_dis_fixed_sobol_desc* desc);
+int nlo_dis_fixed_sobol(const nlo_dis_fixed_sobol_desc* desc, double* d_results, double* d_edges,
+                        void* d_workspace, size_t workspace_bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

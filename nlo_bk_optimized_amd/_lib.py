@@ -41,6 +41,8 @@ DIS_KINDS = {
 DIS_DIMENSIONS = {"LO": 2, "T0": 2, "T1": 3, "T2": 4, "Omega_L_const": 2, "ab": 3, "cd": 4,
                   "I1": 5, "I2": 6, "I3": 7, "I2_nested": 5, "I3_nested": 5}
 DIS_MAX_QUARKS = 8
+FIXED_SOBOL_MAX_ROUNDS = 8
+FIXED_SOBOL_MAX_REPLICATES = 32
 
 
 def spline_words(R: int) -> int:
@@ -212,6 +214,38 @@ class DisBatchDesc(ctypes.Structure):
     ]
 
 
+class FixedSobolItem(ctypes.Structure):
+    """nlo_dis_fixed_sobol_item."""
+
+    _fields_ = [
+        ("desc", DisDesc),
+        ("table_set", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class FixedSobolDesc(ctypes.Structure):
+    """nlo_dis_fixed_sobol_desc."""
+
+    _fields_ = [
+        ("n_items", ctypes.c_int32),
+        ("n_tables", ctypes.c_int32),
+        ("items", ctypes.POINTER(FixedSobolItem)),
+        ("tables", ctypes.POINTER(SobolTable)),
+        ("training_samples", ctypes.c_int32),
+        ("training_rounds", ctypes.c_int32),
+        ("samples_per_replicate", ctypes.c_int32),
+        ("replicates", ctypes.c_int32),
+        ("bins", ctypes.c_int32),
+        ("fold_angle", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("i1_weight", ctypes.c_double),
+        ("epsrel", ctypes.c_double),
+        ("d_grid", ctypes.c_void_p),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -314,6 +348,9 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_dis_bk_local_splines": (ctypes.c_int, [ctypes.POINTER(DisDipole), _vp, _vp]),
     "nlo_dis_batch_workspace": (_size, [ctypes.POINTER(DisBatchDesc)]),
     "nlo_dis_batch_run": (ctypes.c_int, [ctypes.POINTER(DisBatchDesc), _vp, _vp, _size, _vp]),
+    "nlo_dis_endpoint_map": (ctypes.c_int, [_i32, _i64, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "nlo_dis_fixed_sobol_workspace": (_size, [ctypes.POINTER(FixedSobolDesc)]),
+    "nlo_dis_fixed_sobol": (ctypes.c_int, [ctypes.POINTER(FixedSobolDesc), _vp, _vp, _vp, _size, _vp]),
 }
 
 _lib: ctypes.CDLL | None = None

@@ -14,186 +14,13 @@
 
 #include "common.hpp"
 #include "dis_device.hpp"
+#include "dis_integrands.hpp"
 #include "vegas_grid.hpp"
 #include "nlo_gfx950.h"
 
 namespace nlo {
 namespace dis {
 namespace {
-
-constexpr double kZmin = 1e-6;       // ZMIN (S/dis/observables.py:46)
-constexpr double kCF = 4.0 / 3.0;    // (Nc^2 - 1) / (2 Nc), Nc = 3
-constexpr double kNc = 3.0;
-constexpr double kQ0sqr = 1.0;       // Q0sqr (S/dis/config.py:13)
-constexpr int kMaxNested = 128;
-
-constexpr int kDims[NLO_DIS_KINDS] = {2, 2, 3, 4, 2, 3, 4, 5, 6, 7, 5, 5};
-
-// Gauss-Legendre node i (ascending) and weight of the n-point rule on [-1, 1]:
-// Newton on the three-term recurrence from the Chebyshev-like initial guess.
-__device__ inline void gl_node(int n, int i, double& x, double& w) {
-    double z = -cos(kPiD * (i + 0.75) / (n + 0.5));
-    double p0 = 1.0, p1 = z, dp = 1.0;
-    for (int it = 0; it < 100; ++it) {
-        p0 = 1.0;
-        p1 = z;
-        for (int k = 2; k <= n; ++k) {
-            const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
-            p0 = p1;
-            p1 = p2;
-        }
-        dp = n * (z * p1 - p0) / (z * z - 1.0);
-        const double dz = p1 / dp;
-        z -= dz;
-        if (fabs(dz) <= 1e-16) break;
-    }
-    p0 = 1.0;
-    p1 = z;
-    for (int k = 2; k <= n; ++k) {
-        const double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
-        p0 = p1;
-        p1 = p2;
-    }
-    dp = n * (z * p1 - p0) / (z * z - 1.0);
-    x = z;
-    w = 2.0 / ((1.0 - z * z) * dp * dp);
-}
-
-// The nested I2/I3 inner rule in LDS: y = t^3, t = (x + 1)/2, weight
-// w/2 * 3 t^2 (K/dis/i2_i3/extension.py:21-36).
-struct RuleView {
-    int n;
-    const double* y;
-    const double* w;
-};
-
-__device__ inline RuleView stage_nested_rule(int n, double* y, double* w) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double x, wt;
-        gl_node(n, i, x, wt);
-        const double t = 0.5 * (x + 1.0);
-        y[i] = t * t * t;
-        w[i] = 0.5 * wt * 3.0 * t * t;
-    }
-    __syncthreads();
-    return {n, y, w};
-}
-
-// Everything an integrand reads, by value in the kernel arguments.
-struct DisArgs {
-    double Q2, xbj;
-    int transverse, nq;
-    double mass[NLO_DIS_MAX_QUARKS], charge[NLO_DIS_MAX_QUARKS];
-    double maxr;
-    int finite_nc, parent_coupling;
-    DisCoupling coup;
-    double y_lo;    // LO rapidity max(ln(X0 / xbj), 0)
-    double y_dip;   // dipole-sector rapidity of quark 0 (S/dis/observables.py:55-68)
-    double z2min;   // Q0^2 / W^2 (S/dis/observables.py:71-73)
-    double min_r, max_r;  // the dipole's table range (inside_table, :145-147)
-    DisDipole dip;
-    int nested_points;
-};
-
-// ------------------------------------------------------------- integrands
-
-template <typename T>
-__device__ __forceinline__ T amp(const DisArgs& a, T r, T Y) {
-    return dipole_amplitude<T, false>(r, Y, a.dip);
-}
-
-// S/dis/observables.py:86-104
-template <typename T>
-__device__ __forceinline__ T tripole_amp(const DisArgs& a, T x01, T x02, T x21, T Y) {
-    const T S01 = T(1) - amp(a, x01, Y), S02 = T(1) - amp(a, x02, Y), S12 = T(1) - amp(a, x21, Y);
-    if (a.finite_nc) return T(1) - T(kNc / (2.0 * kCF)) * (S02 * S12 - S01 / T(kNc * kNc));
-    return T(1) - S02 * S12;
-}
-
-template <typename T, int K>
-__device__ __forceinline__ T dis_value(const DisArgs& a, const double* u, const RuleView& rule) {
-    const T Q2 = (T)a.Q2;
-    if constexpr (K == NLO_DIS_LO) {
-        // S/dis/observables.py:345-352 and S/dis/lo.py:12-51
-        const T z = (T)u[0];
-        const T r = (T)(a.maxr * u[1]);
-        T tot = T(0);
-        for (int q = 0; q < a.nq; ++q)
-            tot += lo_wavefunction<T>(r, z, Q2, (T)a.mass[q], (T)a.charge[q], a.transverse != 0);
-        return T(2.0 * kPiD) * r * (T)a.maxr * tot * amp(a, r, (T)a.y_lo);
-    } else if constexpr (K <= NLO_DIS_DIP_L_CD) {
-        // integrand_dip_massive (S/dis/observables.py:107-172)
-        const T z1 = (T)(kZmin + u[0] * (1.0 - 2.0 * kZmin));
-        const double x01d = a.maxr * u[1];
-        const bool inside = x01d >= a.min_r && x01d <= a.max_r;
-        const T x01 = (T)x01d;
-        const T x01e = (T)fmin(fmax(x01d, a.min_r), a.max_r);
-        const T mf = (T)a.mass[0];
-        const T N01 = amp(a, x01e, (T)a.y_dip);
-        T res;
-        if constexpr (K == NLO_DIS_DIP_T0) res = it_dip_0<T>(Q2, z1, x01e, mf);
-        else if constexpr (K == NLO_DIS_DIP_T1) res = it_dip_1<T>(Q2, z1, x01e, mf, (T)u[2]);
-        else if constexpr (K == NLO_DIS_DIP_T2) res = it_dip_2<T>(Q2, z1, x01e, mf, (T)u[2], (T)u[3]);
-        else if constexpr (K == NLO_DIS_DIP_L_CONST) res = il_dip_const<T>(Q2, z1, x01e, mf);
-        else if constexpr (K == NLO_DIS_DIP_L_AB) res = il_dip_ab<T>(Q2, z1, x01e, mf, (T)u[2]);
-        else res = il_dip_cd<T>(Q2, z1, x01e, mf, (T)u[2], (T)u[3]);
-        res = N01 * res * (x01 * (T)a.maxr * T(1.0 - 2.0 * kZmin)) * (dis_alpha<T>(x01e, a.coup) * T(kCF / kPiD));
-        return (inside && isfinite(res)) ? res : T(0);
-    } else {
-        // integrand_qgunsub_massive (S/dis/observables.py:175-250) and the nested
-        // form (:253-331)
-        if (a.z2min > 1.0) return T(0);
-        const T z2min = (T)a.z2min;
-        const T z1 = (T(1) - z2min) * (T)u[0];
-        const T z2 = (T(1) - z1 - z2min) * (T)u[1] + z2min;
-        const T x01 = (T)(a.maxr * u[2]), x02 = (T)(a.maxr * u[3]);
-        const T phi = (T)(2.0 * kPiD * u[4]);
-        const T x01sq = x01 * x01, x02sq = x02 * x02;
-        const T x21sq = fmax(x01sq + x02sq - T(2) * x01 * x02 * cos(phi), T(0));
-        const T x21 = dis_sqrt(x21sq);
-        const T maxr = (T)a.maxr;
-        const T jac = (T(1) - z2min) * (T(1) - z1 - z2min) * x01 * x02 * maxr * maxr * T(2.0 * kPiD);
-        const T Y = log((Q2 / (T)a.xbj) * z2 / T(kQ0sqr));
-        const T Ye = fmax(Y, T(0));
-        const T mf = (T)a.mass[0], m2 = mf * mf;
-        const T N012 = tripole_amp<T>(a, x01, x02, x21, Ye);
-        const T ra = a.parent_coupling ? x01 : fmin(x01, fmin(x02, x21));
-        const T af = dis_alpha<T>(ra, a.coup) * T(kCF / kPiD);
-        const Tripole<T> v = tripole_vars<T>(Q2, mf, z1, z2, x01sq, x02sq, x21sq);
-        T res;
-        if constexpr (K == NLO_DIS_QG_I1) {
-            const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
-            const T N01 = amp(a, x01, Ye);
-            if (a.transverse)
-                res = N01 * it_qg_uvsub<T>(v, m2, z1, z2, x01sq) + N012 * it_qg_i1<T>(v, f, m2, z1, z2);
-            else
-                res = N01 * il_qg_uvsub<T>(v, Q2, z1, z2, x01sq) + N012 * il_qg_i1<T>(v, f, Q2, z1, z2);
-        } else if constexpr (K == NLO_DIS_QG_I2 || K == NLO_DIS_QG_I3) {
-            const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
-            const TMoments<T> m1 = t_moments<T>(v, m2, (T)u[5]);
-            if constexpr (K == NLO_DIS_QG_I2) {
-                res = N012 * (a.transverse ? it_qg_i2<T>(v, f, m2, z1, z2, m1) : il_qg_i2<T>(v, f, Q2, z1, z2, m1));
-            } else {
-                const TMoments<T> mm2 = t_moments<T>(v, m2, (T)u[6]);
-                res = N012 * (a.transverse ? it_qg_i3<T>(v, f, m2, z1, z2, m1, mm2)
-                                           : il_qg_i3<T>(v, Q2, m2, z1, z2, m1, mm2));
-            }
-        } else {
-            // y_t integrals on the mapped inner rule: I2 is linear and I3
-            // bilinear in the G values (K/dis/i2_i3/kernel.cu:768-912)
-            const TFixed<T> f = t_fixed<T>(v, m2, z1, z2);
-            TMoments<T> acc{T(0), T(0), T(0), T(0), T(0), T(0), T(0), T(0)};
-            for (int i = 0; i < rule.n; ++i) t_moments_axpy(acc, (T)rule.w[i], t_moments<T>(v, m2, (T)rule.y[i]));
-            if constexpr (K == NLO_DIS_QG_I2_NESTED)
-                res = N012 * (a.transverse ? it_qg_i2<T>(v, f, m2, z1, z2, acc) : il_qg_i2<T>(v, f, Q2, z1, z2, acc));
-            else
-                res = N012 * (a.transverse ? it_qg_i3<T>(v, f, m2, z1, z2, acc, acc)
-                                           : il_qg_i3<T>(v, Q2, m2, z1, z2, acc, acc));
-        }
-        res = res * jac * af / z2;
-        return (Y >= T(0) && isfinite(res)) ? res : T(0);
-    }
-}
 
 // ------------------------------------------------------------------ kernels
 
@@ -867,6 +694,8 @@ __global__ void __launch_bounds__(256) batch_results_kernel(BatchDev b, const in
     }
 }
 
+}  // namespace
+
 // -------------------------------------------------------------------- host
 
 DisDipole make_dipole(const nlo_dis_dipole& d) {
@@ -904,9 +733,7 @@ int check_dipole(const nlo_dis_dipole* d) {
     return NLO_OK;
 }
 
-// ends: the BK table's first and last r when the caller has them (a batch
-// reads them once), else NULL and they are copied from the device.
-int make_args(const nlo_dis_desc* desc, DisArgs& a, const double* ends = nullptr) {
+int make_args(const nlo_dis_desc* desc, DisArgs& a, const double* ends) {
     NLO_REQUIRE(desc != nullptr, "desc is NULL");
     int rc = check_dipole(&desc->dipole);
     if (rc != NLO_OK) return rc;
@@ -964,6 +791,8 @@ int make_args(const nlo_dis_desc* desc, DisArgs& a, const double* ends = nullptr
     a.nested_points = desc->nested_points;
     return NLO_OK;
 }
+
+namespace {
 
 #define DIS_DISPATCH(MACRO)           \
     switch (contribution) {           \

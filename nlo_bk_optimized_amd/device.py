@@ -707,6 +707,99 @@ def dis_batch(items, tables, *, bins: int, precision: str, epsrel: float, epsabs
     return out
 
 
+def fixed_sobol_tables(seed: int, training_rounds: int, replicates: int) -> list[_lib.SobolTable]:
+    """The 5-D point sets of make_sobol_points (S/dis/fixed_sobol.py:87-97):
+    training round k scrambled with seed + k, replicate k with seed + 10000 + k."""
+
+    return ([sobol_table(5, seed + k) for k in range(training_rounds)]
+            + [sobol_table(5, seed + 10_000 + k) for k in range(replicates)])
+
+
+def dis_fixed_sobol(items, tables, *, training_samples: int, training_rounds: int, samples_per_replicate: int,
+                    replicates: int, bins: int, precision: str, epsrel: float, i1_weight: float = 1.0,
+                    fold_angle: bool = True, device: torch.device | str = "cuda",
+                    return_edges: bool = False, grid: torch.Tensor | None = None,
+                    workspace_budget: int = 8 << 30):
+    """nlo_dis_fixed_sobol over ``items`` (nlo_dis_fixed_sobol_item) with the
+    flat list of point-set ``tables``: [n_items, 4, 4] float64 on the device
+    (rows I1, I2, I3, total; columns value, error, evaluations, converged),
+    and with ``return_edges`` the learned grids [n_items, 5, bins + 1].  With
+    ``grid`` (float64 [n_items, 5, bins + 1] on the device) the grids are frozen
+    and no training runs (training_rounds must be 0).  Items are independent: a
+    batch whose workspace would exceed ``workspace_budget`` bytes runs as
+    consecutive sub-batches."""
+
+    device = torch.device(device)
+    tarr = (_lib.SobolTable * len(tables))(*tables)
+    if grid is not None:
+        require_cuda(grid, torch.float64, "grid")
+        if tuple(grid.shape) != (len(items), 5, bins + 1):
+            raise ValueError(f"grid must have shape [{len(items)}, 5, {bins + 1}]")
+
+    def describe(part, at: int = 0) -> _lib.FixedSobolDesc:
+        d = _lib.FixedSobolDesc()
+        d.n_items = len(part)
+        d.n_tables = len(tables)
+        d.items = (_lib.FixedSobolItem * len(part))(*part)
+        d.tables = tarr
+        d.training_samples = int(training_samples)
+        d.training_rounds = int(training_rounds)
+        d.samples_per_replicate = int(samples_per_replicate)
+        d.replicates = int(replicates)
+        d.bins = int(bins)
+        d.fold_angle = 1 if fold_angle else 0
+        d.precision = PRECISION_CODES[precision]
+        d.i1_weight = float(i1_weight)
+        d.epsrel = float(epsrel)
+        d.d_grid = ptr(grid[at:at + len(part)]) if grid is not None else None
+        return d
+
+    out = torch.empty((len(items), 4, 4), dtype=torch.float64, device=device)
+    edges = torch.empty((len(items), 5, bins + 1), dtype=torch.float64, device=device) if return_edges else None
+    start = 0
+    while start < len(items):
+        count = len(items) - start
+        while True:
+            d = describe(items[start:start + count], start)
+            nbytes = lib().nlo_dis_fixed_sobol_workspace(ctypes.byref(d))
+            if nbytes <= workspace_budget or count == 1:
+                break
+            count = max(1, count // 2)
+        ws = torch.empty(max(nbytes, 1), dtype=torch.uint8, device=device)
+        check(lib().nlo_dis_fixed_sobol(ctypes.byref(d), ptr(out[start:start + count]),
+                                        ptr(edges[start:start + count]) if edges is not None else None,
+                                        ptr(ws), nbytes, stream_handle(device)), "nlo_dis_fixed_sobol")
+        del ws
+        start += count
+    return (out, edges) if return_edges else out
+
+
+def dis_endpoint_map(points: torch.Tensor, edges: torch.Tensor, maxr: float, *, fold_angle: bool = True):
+    """dis_sobol_endpoint_map (K/dis/sampling/kernel.cu:15-81): (first,
+    first_weight, second, second_weight) for device points [n, 5] and a grid
+    [5, bins + 1] of one dtype (float32 or float64); the weights include the
+    grid's inverse density."""
+
+    if points.dtype not in (torch.float32, torch.float64):
+        raise TypeError("points must be float32 or float64")
+    require_cuda(points, points.dtype, "points")
+    require_cuda(edges, points.dtype, "edges")
+    if points.dim() != 2 or points.shape[1] != 5:
+        raise ValueError("points must have shape [n, 5]")
+    if edges.dim() != 2 or edges.shape[0] != 5 or edges.shape[1] < 3:
+        raise ValueError("edges must have shape [5, bins + 1] with at least two bins")
+    n = points.shape[0]
+    first = torch.empty_like(points)
+    second = torch.empty_like(points)
+    w1 = torch.empty(n, dtype=points.dtype, device=points.device)
+    w2 = torch.empty_like(w1)
+    code = _lib.NLO_PREC_FP32 if points.dtype == torch.float32 else _lib.NLO_PREC_FP64
+    check(lib().nlo_dis_endpoint_map(code, n, ptr(points), ptr(edges), edges.shape[1] - 1, float(maxr),
+                                     1 if fold_angle else 0, ptr(first), ptr(w1), ptr(second), ptr(w2),
+                                     stream_handle(points.device)), "nlo_dis_endpoint_map")
+    return first, w1, second, w2
+
+
 def dtype_epsilon(dtype: torch.dtype) -> float:
     return float(torch.finfo(dtype).eps)
 

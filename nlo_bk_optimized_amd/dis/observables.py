@@ -9,13 +9,20 @@ tensor Gauss-Legendre rule, the others through the device VEGAS driver.  The
 arithmetic follows the dtype of Q2 as in the reference: float32 -> the FP32
 integrands, float64 -> FP64 (sums are FP64 in both).
 
+The qqbar-g term takes the reference's GBW fixed-Sobol estimator where it
+does (float32, fused kernels, the 48-point nested rule, a GBW dipole and
+maxeval >= FIXED_SOBOL_EVALUATIONS; S/dis/observables.py:470-491) and falls
+back to VEGAS for every quark of the term when one quark's fixed-Sobol total
+misses epsrel; otherwise VEGAS over I1 plus, for float32 with cuda_nested, the
+nested I2/I3, else the 6-/7-dimensional I2/I3.  A batch runs every
+fixed-Sobol estimator in one device call first, then every VEGAS and
+Gauss-Legendre integral (fallbacks included) in one device batch.
+
 Differences from the reference, on purpose:
 * VEGAS draws scrambled Sobol points (seeded, deterministic) instead of
   torch.Generator samples;
-* the qqbar-g term always takes the VEGAS path (I1 plus, for float32 with
-  cuda_nested, the nested I2/I3, else the 6-/7-dimensional I2/I3); the
-  reference's GBW-only fixed-Sobol replicate estimator (S/dis/fixed_sobol.py)
-  is not reproduced.
+* the fixed-Sobol selection reads the dtype of Q2 only (every path here runs
+  on the GPU).
 """
 
 from __future__ import annotations
@@ -228,6 +235,8 @@ class _Plan:
         self.tables: list = []
         self._table_index: dict[tuple[int, int], int] = {}
         self._keep: list[torch.Tensor] = []  # device arrays the items point to (BK initialisations)
+        self.fixed_entries: list[tuple] = []  # fixed-Sobol estimators (Q2, xbj, polarization, quark, seed)
+        self._deferred: list = []  # called with the fixed-Sobol results before the batch runs
 
     def _desc(self, Q2, xbj, polarization: Polarization, quarks):
         desc = _desc(Q2, xbj, polarization, tuple(quarks), self.dipole, self.config)
@@ -284,18 +293,46 @@ class _Plan:
         item.first_index = first
         return self._add(item, int(seed))
 
-    def run(self, workspace_budget: int = 8 << 30) -> list[IntegralResult]:
-        """Every collected integral as an IntegralResult, in collection order."""
+    def fixed_sobol(self, Q2, xbj, polarization, quark, seed: int) -> int:
+        """gbw_qg_fixed_sobol (S/dis/fixed_sobol.py:274-368) for one quark."""
+
+        self.fixed_entries.append((Q2, xbj, polarization, quark, int(seed)))
+        return len(self.fixed_entries) - 1
+
+    def defer(self, decide) -> None:
+        """Call decide(fixed results) once the fixed-Sobol estimators have run,
+        before the batch; it may add integrals (the VEGAS fallback)."""
+
+        self._deferred.append(decide)
+
+    def run(self, workspace_budget: int = 8 << 30) -> "_Results":
+        """Every collected integral as an IntegralResult, in collection order,
+        with the fixed-Sobol results in ``.fixed``."""
 
         from nlo_bk_optimized_amd import device as dev
+        from nlo_bk_optimized_amd.dis.fixed_sobol import gbw_qg_fixed_sobol_batch
 
+        fixed = (gbw_qg_fixed_sobol_batch(self.fixed_entries, self.dipole, self.config, device=self.device)
+                 if self.fixed_entries else [])
+        for decide in self._deferred:
+            decide(fixed)
         if not self.items:
-            return []
+            return _Results([], fixed)
         rows = dev.dis_batch(self.items, self.tables, bins=self.config.bins, precision=self.precision,
                              epsrel=self.config.epsrel, epsabs=0.0, device=self.device,
                              workspace_budget=workspace_budget).cpu().tolist()
-        return [IntegralResult(torch.tensor(v, dtype=torch.float64), torch.tensor(e, dtype=torch.float64),
-                               int(n), bool(c), seed) for (v, e, n, c), seed in zip(rows, self.seeds)]
+        return _Results([IntegralResult(torch.tensor(v, dtype=torch.float64), torch.tensor(e, dtype=torch.float64),
+                                        int(n), bool(c), seed) for (v, e, n, c), seed in zip(rows, self.seeds)],
+                        fixed)
+
+
+class _Results(list):
+    """The batch's IntegralResults in collection order; ``fixed`` holds the
+    FixedSobolResults in the order the estimators were added."""
+
+    def __init__(self, results, fixed) -> None:
+        super().__init__(results)
+        self.fixed = fixed
 
 
 def _lo_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, quadrature_points: int):
@@ -339,11 +376,50 @@ def _dip_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, quadrature_poi
     return compose
 
 
+def _fixed_sobol_supported(Q2, dipole: Dipole, config: DISConfig) -> bool:
+    """S/dis/observables.py:644-658 (see fixed_sobol.fixed_sobol_supported)."""
+
+    from nlo_bk_optimized_amd.dis.fixed_sobol import fixed_sobol_supported
+
+    return fixed_sobol_supported(Q2, dipole, config)
+
+
 def _qg_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, seed: int):
-    """S/dis/observables.py:463-538 (VEGAS path)."""
+    """sigma_qg_d2b (S/dis/observables.py:459-532): the fixed-Sobol estimator
+    per quark (seed + quark index) where it applies, its totals scaled by the
+    squared charges and summed when every quark's total converged; else, and
+    where it does not apply, the VEGAS path."""
 
     config = plan.config
     _validate_kinematics(Q2, xbj)
+    if not _fixed_sobol_supported(Q2, plan.dipole, config):
+        return _qg_vegas_recipe(plan, Q2, xbj, polarization, seed)
+    handles = [plan.fixed_sobol(Q2, xbj, polarization, quark, seed + index)
+               for index, quark in enumerate(config.quarks)]
+    fallback = []
+
+    def decide(fixed) -> None:
+        # the reference stops at the first quark whose total misses epsrel and
+        # integrates every quark of the term with VEGAS instead
+        if not all(fixed[h].total.converged for h in handles):
+            fallback.append(_qg_vegas_recipe(plan, Q2, xbj, polarization, seed))
+
+    plan.defer(decide)
+    factor = 4 * NC * AlphaEM / (2 * math.pi) ** 3 * 2 * math.pi
+
+    def compose(R):
+        if fallback:
+            return fallback[0](R)
+        results = [_scale_result(R.fixed[h].total, quark.charge**2) for h, quark in zip(handles, config.quarks)]
+        return _scale_result(_sum_results(results, seed), factor)
+
+    return compose
+
+
+def _qg_vegas_recipe(plan: _Plan, Q2, xbj, polarization: Polarization, seed: int):
+    """S/dis/observables.py:493-532 (VEGAS path)."""
+
+    config = plan.config
     # the reference nests the I2 / I3 inner integral only on its float32 CUDA
     # path (use_nested, S/dis/observables.py:492); float64 integrates the
     # 6- / 7-dimensional forms
@@ -412,7 +488,7 @@ def sigma_dip_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: D
 
 def sigma_qg_d2b(Q2, xbj, polarization: Polarization, dipole: Dipole, config: DISConfig, *,
                  batch_size: int = 65_536, seed: int = 0) -> IntegralResult:
-    """S/dis/observables.py:463-538 (VEGAS path)."""
+    """S/dis/observables.py:459-532."""
 
     plan = _plan(Q2, dipole, config)
     recipe = _qg_recipe(plan, Q2, xbj, polarization, seed)
