@@ -94,6 +94,10 @@ def parse() -> argparse.Namespace:
     p.add_argument("--probe-dis", action="store_true",
                    help="with --kernel-probe: also one NLO DIS batch (8 points, GBW dipole, light + charm, "
                         "2^22 samples per integral) in the probe precision")
+    p.add_argument("--probe-fixed-sobol", action="store_true",
+                   help="with --kernel-probe: also the six-term fixed-Sobol workload (GBW, Q^2 = 9, x = 1e-3, "
+                        "charm, L and T), learned once and evaluated through the frozen grids "
+                        "--kernel-launches times")
     return p.parse_args()
 
 
@@ -184,6 +188,19 @@ def main() -> int:
             pts = [(torch.tensor(q2, dtype=dt), torch.tensor(xb, dtype=dt))
                    for q2 in (2.0, 30.0) for xb in (1e-5, 1e-4, 1e-3, 1e-2)]
             structure_functions_batch(pts, GBW(Qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0), dcfg, seed=0)
+        if args.probe_fixed_sobol:
+            from nlo_bk_optimized_amd.dipole import GBW
+            from nlo_bk_optimized_amd.dis import Polarization
+            from nlo_bk_optimized_amd.dis.config import DISConfig, Quark, QuarkType
+            from nlo_bk_optimized_amd.dis.fixed_sobol import gbw_qg_fixed_sobol_batch
+
+            gbw = GBW(Qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0)
+            fcfg = DISConfig(quarks=(Quark(QuarkType.C, 1.4),), maxr=5.0, nf_alphas=1, cuda_nested_points=48)
+            q2t, xt = torch.tensor(9.0, dtype=torch.float32), torch.tensor(1e-3, dtype=torch.float32)
+            entries = [(q2t, xt, pol, fcfg.quarks[0], 3100) for pol in (Polarization.L, Polarization.T)]
+            _, grids = gbw_qg_fixed_sobol_batch(entries, gbw, fcfg, return_edges=True)
+            for _ in range(args.kernel_launches):
+                gbw_qg_fixed_sobol_batch(entries, gbw, fcfg, grid=grids)
         src = torch.ones(1 << 25, dtype=torch.float64, device=device)  # 256 MiB
         dst = torch.empty_like(src)
         for _ in range(2):
@@ -316,6 +333,46 @@ def main() -> int:
                 ms = e0.elapsed_time(e1) / reps
                 extra[f"dis_{kind}_T_{prec}_samples_per_s"] = nd / (ms * 1e-3)
                 log(f"dis {kind} T {prec}: {ms:.3f} ms per 2^22-sample VEGAS pass")
+
+        # The reference's six-term fixed-Sobol workload at the same point
+        # (T/benchmarks/dis_fixed_sobol_all_qg.py): I1, I2 and I3 for L and T,
+        # seed 3100, 3 x 65 536 training and 7 x 32 768 replicate points per
+        # polarisation, FP32.  First use (grid learning and replicates) and the
+        # frozen-grid evaluation (the replicates through the learned grids), each
+        # the median of 7 runs as the reference times them, against its
+        # 20-worker C++ total of 3.625 s; the six values against the C++ ones.
+        from nlo_bk_optimized_amd.dis import Polarization
+        from nlo_bk_optimized_amd.dis.fixed_sobol import gbw_qg_fixed_sobol_batch
+
+        fcfg = DISConfig(quarks=(Quark(QuarkType.C, 1.4),), maxr=5.0, nf_alphas=1, cuda_nested_points=48)
+        q2t, xt = torch.tensor(9.0, dtype=torch.float32), torch.tensor(1e-3, dtype=torch.float32)
+        entries = [(q2t, xt, pol, fcfg.quarks[0], 3100) for pol in (Polarization.L, Polarization.T)]
+        fixed, grids = gbw_qg_fixed_sobol_batch(entries, gbw, fcfg, return_edges=True)
+        times = {"first_use": [], "frozen_grid": []}
+        for _ in range(7):
+            for key, grid in (("first_use", None), ("frozen_grid", grids)):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                gbw_qg_fixed_sobol_batch(entries, gbw, fcfg, grid=grid)
+                torch.cuda.synchronize()
+                times[key].append(time.perf_counter() - t0)
+        cpp = {"L": ((0.10206739088678346, 0.0002535000977528945), (-0.024439455902300525, 4.764427185133986e-05),
+                     (0.009414936380564324, 2.6852425497947778e-05)),
+               "T": ((0.40263591173620866, 0.0017956984452001683), (-0.08719375357937224, 0.00026850014906050766),
+                     (0.04848797141670969, 0.00013338145018266037))}
+        pulls = [(float(c.value) - v) / math.hypot(float(c.error), e)
+                 for pol, res in zip(("L", "T"), fixed) for c, (v, e) in zip(res.contributions, cpp[pol])]
+        med = {k: sorted(v)[len(v) // 2] for k, v in times.items()}
+        extra.update({
+            "fixed_sobol_six_terms_first_use_s": med["first_use"],
+            "fixed_sobol_six_terms_frozen_grid_s": med["frozen_grid"],
+            "fixed_sobol_cpp20_over_frozen_grid": 3.625 / med["frozen_grid"],
+            "fixed_sobol_cpp20_over_first_use": 3.625 / med["first_use"],
+            "fixed_sobol_max_abs_pull_vs_cpp": max(abs(x) for x in pulls),
+        })
+        log(f"fixed-Sobol six terms: first use {med['first_use'] * 1e3:.2f} ms, frozen-grid evaluation "
+            f"{med['frozen_grid'] * 1e3:.2f} ms ({3.625 / med['frozen_grid']:.0f}x the 20-worker C++ 3.625 s); "
+            f"pulls vs C++ " + " ".join(f"{x:+.2f}" for x in pulls))
 
     if args.dis_points > 0:
         # NLO DIS batch: (Q^2, x) on 32 log Q^2 in [1, 100] x 16 log x in

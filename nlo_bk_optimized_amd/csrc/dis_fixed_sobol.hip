@@ -338,46 +338,66 @@ __global__ void __launch_bounds__(256) fs_init_kernel(FsDev f, const double* __r
         g[i] = frozen ? frozen[(int64_t)item * E + i] : (double)(i % (f.bins + 1)) / (double)f.bins;
 }
 
-// Training partials of every estimator summed over its blocks in block order:
-// block (estimator, slice of 256 slots).
+// Training partials of every estimator summed over its blocks: block
+// (estimator, slice of 16 slots), thread (slot, group g < 16) sums blocks
+// g, g + 16, ... in order, then the 16 group sums of a slot are merged in
+// group order (a fixed order whatever the hardware schedule).
+constexpr int kFsReduceGroups = 16;
+constexpr int kFsReduceSlots = 256 / kFsReduceGroups;
+
 __global__ void __launch_bounds__(256) fs_train_reduce_kernel(FsDev f, int slices) {
+    __shared__ double gs[kFsReduceGroups][kFsReduceSlots], gc[kFsReduceGroups][kFsReduceSlots];
     const int item = blockIdx.x / slices, sl = blockIdx.x - item * slices;
     const int64_t L = fs_train_part(f.bins);
-    const int64_t k = (int64_t)sl * blockDim.x + threadIdx.x;
-    if (k >= L) return;
-    const double* p = f.part + (int64_t)item * f.part_stride + k;
+    const int j = threadIdx.x % kFsReduceSlots, g = threadIdx.x / kFsReduceSlots;
+    const int64_t k = (int64_t)sl * kFsReduceSlots + j;
     double s = 0.0, c = 0.0;
-    for (int b = 0; b < f.blocks_train; ++b) comp_add(s, c, p[(int64_t)b * L]);
-    f.stat[(int64_t)item * f.stat_stride + k] = s + c;
+    if (k < L) {
+        const double* p = f.part + (int64_t)item * f.part_stride + k;
+        for (int b = g; b < f.blocks_train; b += kFsReduceGroups) comp_add(s, c, p[(int64_t)b * L]);
+    }
+    gs[g][j] = s;
+    gc[g][j] = c;
+    __syncthreads();
+    if (g == 0 && k < L) {
+        for (int q = 1; q < kFsReduceGroups; ++q) comp_merge(s, c, gs[q][j], gc[q][j]);
+        f.stat[(int64_t)item * f.stat_stride + k] = s + c;
+    }
 }
 
 // The score histogram sum_k weight_k |w_k| / mean |w_k| per bin
 // (learn_triple_importance_grid, S/dis/fixed_sobol.py:193-208; the per-point
 // scores summed per bin equal the per-bin |w_k| sums so weighted, the sums
-// being linear) and the refined grid, one (estimator, dimension) per thread:
-// the current edge buffer into the other one.
-__global__ void __launch_bounds__(64) fs_adapt_kernel(FsDev f, int n_items, int round, double eps, double tiny) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n_items * kFsDim) return;
-    const int item = k / kFsDim, d = k - item * kFsDim;
+// being linear) and the refined grid: block (estimator, dimension) forms the
+// scores and their smoothed importances (vegas_smoothed) in parallel in LDS,
+// then one lane refines the edge row from the current buffer into the other
+// one with the same operations as vegas_adapt_row.
+__global__ void __launch_bounds__(64) fs_adapt_kernel(FsDev f, int round, double eps, double tiny) {
+    __shared__ double sc[256], cn[256], ed[256], sm[256];
+    const int item = blockIdx.x / kFsDim, d = blockIdx.x - item * kFsDim;
     const int bins = f.bins, cells = fs_cells(bins);
     double* st = f.stat + (int64_t)item * f.stat_stride;
     const double* H = st + kFsPart;
     const double* cnt = H + 3 * (int64_t)cells;
-    double* score = st + fs_train_part(bins) + (int64_t)d * bins;
     const double n = (double)f.n_train;
     const double wt[3] = {f.i1_weight, 1.0, 1.0};
     double inv_mean[3];
     for (int q = 0; q < 3; ++q) inv_mean[q] = 1.0 / fmax(st[q] / n, tiny);
-    for (int b = 0; b < bins; ++b) {
-        double v = 0.0;
-        for (int q = 0; q < 3; ++q) v += wt[q] * H[(int64_t)q * cells + d * bins + b] * inv_mean[q];
-        score[b] = v;
-    }
     double* g = f.edges + (int64_t)item * 2 * kFsDim * (bins + 1);
     const double* e = g + (int64_t)(round & 1) * kFsDim * (bins + 1) + (int64_t)d * (bins + 1);
     double* o = g + (int64_t)((round + 1) & 1) * kFsDim * (bins + 1) + (int64_t)d * (bins + 1);
-    vegas_adapt_row(e, score, cnt + (int64_t)d * bins, bins, eps, o);
+    for (int b = threadIdx.x; b < bins; b += blockDim.x) {
+        double v = 0.0;
+        for (int q = 0; q < 3; ++q) v += wt[q] * H[(int64_t)q * cells + d * bins + b] * inv_mean[q];
+        sc[b] = v;
+        cn[b] = cnt[(int64_t)d * bins + b];
+        st[fs_train_part(bins) + (int64_t)d * bins + b] = v;
+    }
+    for (int b = threadIdx.x; b <= bins; b += blockDim.x) ed[b] = e[b];
+    __syncthreads();
+    for (int b = threadIdx.x; b < bins; b += blockDim.x) sm[b] = vegas_smoothed(sc, cn, bins, b, eps);
+    __syncthreads();
+    if (threadIdx.x == 0) vegas_adapt_row_with(ed, [&](int i) { return sm[i]; }, bins, o);
 }
 
 // Replicate estimates: block (estimator, replicate), one wave: the block sums
@@ -551,7 +571,7 @@ int fs_enqueue(const nlo_dis_fixed_sobol_desc* d, const FsPlan& P, const FsDev& 
     const double tiny = std::is_same<T, float>::value ? (double)FLT_MIN : DBL_MIN;
     hipLaunchKernelGGL(fs_init_kernel, dim3(n), dim3(256), 0, s, f, d->d_grid);
     NLO_LAUNCH_CHECK();
-    const int slices = (int)((fs_train_part(d->bins) + 255) / 256);
+    const int slices = (int)((fs_train_part(d->bins) + kFsReduceSlots - 1) / kFsReduceSlots);
     for (int round = 0; round < d->training_rounds; ++round) {
         const dim3 grid((unsigned)((int64_t)n * P.blocks_train));
         hipLaunchKernelGGL((fs_train_kernel<T, kFsI1>), grid, dim3(256), P.lds_train, s, f, round);
@@ -560,7 +580,7 @@ int fs_enqueue(const nlo_dis_fixed_sobol_desc* d, const FsPlan& P, const FsDev& 
         NLO_LAUNCH_CHECK();
         hipLaunchKernelGGL(fs_train_reduce_kernel, dim3((unsigned)(n * slices)), dim3(256), 0, s, f, slices);
         NLO_LAUNCH_CHECK();
-        hipLaunchKernelGGL(fs_adapt_kernel, dim3((n * kFsDim + 63) / 64), dim3(64), 0, s, f, n, round, eps, tiny);
+        hipLaunchKernelGGL(fs_adapt_kernel, dim3(n * kFsDim), dim3(64), 0, s, f, round, eps, tiny);
         NLO_LAUNCH_CHECK();
     }
     const dim3 grid((unsigned)((int64_t)n * d->replicates * P.blocks_rep));

@@ -35,17 +35,18 @@ __device__ __forceinline__ double vegas_smoothed(const double* h, const double* 
 // edges o[bins + 1] out (o must not alias e).  The linspace targets
 // increase, so searchsorted(cumulative, target, right) - 1 is found by a
 // forward sweep over the cumulative sums: no scratch memory.
-__device__ inline void vegas_adapt_row(const double* e, const double* hh, const double* cc, int bins, double eps,
-                                       double* o) {
+// sm(i) gives the smoothed importance of bin i.
+template <typename Smoothed>
+__device__ inline void vegas_adapt_row_with(const double* e, const Smoothed& sm, int bins, double* o) {
 #pragma clang fp contract(off)
     double total = 0.0;
-    for (int i = 0; i < bins; ++i) total = total + vegas_smoothed(hh, cc, bins, i, eps);
+    for (int i = 0; i < bins; ++i) total = total + sm(i);
     const double step = total / (double)bins;
     // j: largest index with cumulative[j] <= target; cum_j = cumulative[j];
     // sm_j = smoothed[j] (cumulative[j + 1] = cum_j + sm_j).
     int j = 0;
     double cum_j = 0.0;
-    double sm_j = vegas_smoothed(hh, cc, bins, 0, eps);
+    double sm_j = sm(0);
     double cum_prev = 0.0, sm_prev = sm_j;  // values at j - 1 (valid when j == bins)
     for (int t = 0; t <= bins; ++t) {
         const double target =
@@ -57,7 +58,7 @@ __device__ inline void vegas_adapt_row(const double* e, const double* hh, const 
             sm_prev = sm_j;
             cum_j = cum_next;
             ++j;
-            if (j < bins) sm_j = vegas_smoothed(hh, cc, bins, j, eps);
+            if (j < bins) sm_j = sm(j);
         }
         const int b = j < bins ? j : bins - 1;
         const double cb = j < bins ? cum_j : cum_prev;
@@ -67,6 +68,11 @@ __device__ inline void vegas_adapt_row(const double* e, const double* hh, const 
     }
     o[0] = 0.0;
     o[bins] = 1.0;
+}
+
+__device__ inline void vegas_adapt_row(const double* e, const double* hh, const double* cc, int bins, double eps,
+                                       double* o) {
+    vegas_adapt_row_with(e, [&](int i) { return vegas_smoothed(hh, cc, bins, i, eps); }, bins, o);
 }
 
 }  // namespace nlo

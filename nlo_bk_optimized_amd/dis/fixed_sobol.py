@@ -34,6 +34,7 @@ Differences from the reference, on purpose:
 from __future__ import annotations
 
 from dataclasses import dataclass
+from functools import lru_cache
 
 import torch
 
@@ -118,6 +119,16 @@ def fixed_sobol_supported(Q2, dipole, config: DISConfig) -> bool:
             and config.maxeval >= FIXED_SOBOL_EVALUATIONS)
 
 
+@lru_cache(maxsize=64)
+def _point_set_tables(seed: int, training_rounds: int, replicates: int) -> tuple:
+    """The scrambled direction tables of one seed's point sets (torch's
+    SobolEngine scramble runs on the host once per seed)."""
+
+    from nlo_bk_optimized_amd import device as dev
+
+    return tuple(dev.fixed_sobol_tables(seed, training_rounds, replicates))
+
+
 def _scalar_pair(Q2, xbj) -> str:
     ok = (isinstance(Q2, torch.Tensor) and isinstance(xbj, torch.Tensor) and Q2.numel() == 1 and xbj.numel() == 1
           and Q2.dtype == xbj.dtype and Q2.dtype in (torch.float32, torch.float64))
@@ -170,7 +181,7 @@ def gbw_qg_fixed_sobol_batch(entries, dipole: GBW, config: DISConfig, *, trainin
         seed = int(seed)
         if seed not in sets:
             sets[seed] = len(sets)
-            tables.extend(dev.fixed_sobol_tables(seed, training_rounds, replicates))
+            tables.extend(_point_set_tables(seed, training_rounds, replicates))
         item = _lib.FixedSobolItem()
         item.desc = dev.dis_desc(float(Q2.item()), float(xbj.item()), polarization is Polarization.T, (quark,),
                                  dipole, config)
