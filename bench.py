@@ -72,6 +72,8 @@ def parse() -> argparse.Namespace:
     p.add_argument("--cpu-samples-log2", type=int, default=20,
                    help="Sobol points per parent in the CPU baseline sample")
     p.add_argument("--no-dis", action="store_true", help="skip the DIS VEGAS-pass measurements")
+    p.add_argument("--no-dropin", action="store_true",
+                   help="skip the end-to-end solve_bk run with the default K1_RULE (adaptive K1 at this size)")
     p.add_argument("--no-hbm", action="store_true", help="skip the Sobol / VEGAS map / histogram bandwidth runs")
     p.add_argument("--large-maxy", type=float, default=0.0,
                    help="run the 1024-node, 2^24-point RK4 evolution to this Y in FP32 and FP64 (0 skips it)")
@@ -283,7 +285,29 @@ def main() -> int:
         f"adaptive {k1_ms['adaptive']:.3f} ms (mean {float(k1_evals.mean()):.0f} evaluations per parent), "
         f"max rel difference {k1_rel:.2e}")
 
+    dropin = {}
+    if not args.no_dropin:
+        # The drop-in call itself: solve_bk with the same configuration and
+        # K1_RULE="auto", which at this grid size takes the nested adaptive K1
+        # as the reference does (RPOINTS != 100), end to end (grid adaptation,
+        # evolver set-up, every step, the table) to the same rapidity.
+        from nlo_bk_optimized_amd import solve_bk
+        from nlo_bk_optimized_amd.bk.config import resolve_k1_rule
+
+        cfg_d = BKConfig(PRECISION="fp32", **base)
+        solve_bk(ic, 0.2, cfg_d, device=device, seed=0)  # first-call costs
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        solve_bk(ic, args.steps * cfg_d.DE_SOLVER_STEP, cfg_d, device=device, seed=0)
+        torch.cuda.synchronize()
+        dropin = {"solve_bk_dropin_fp32_s": time.perf_counter() - t0,
+                  "solve_bk_dropin_k1_rule": resolve_k1_rule(cfg_d, torch.float32),
+                  "solve_bk_dropin_maxy": args.steps * cfg_d.DE_SOLVER_STEP}
+        log(f"solve_bk drop-in (K1 rule {dropin['solve_bk_dropin_k1_rule']}): "
+            f"{dropin['solve_bk_dropin_fp32_s']:.3f} s to Y = {dropin['solve_bk_dropin_maxy']:g}")
+
     extra: dict = {
+        **dropin,
         "k1_fixed_ms_per_rhs": k1_ms["fixed"],
         "k1_adaptive_ms_per_rhs": k1_ms["adaptive"],
         "k1_adaptive_mean_evaluations_per_parent": float(k1_evals.mean()),
