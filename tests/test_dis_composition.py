@@ -1,7 +1,8 @@
 """The DIS composition on the CPU with the device integrators replaced by
 fakes that return known integrals: normalisations, quark charges and
-budgets, seeds, the LO(x0) + dipole + qqbar-g sum, F2 = FT + FL, and the
-VEGAS iteration statistics of ``vegas_device``.
+budgets, seeds, the LO(x0) + dipole + qqbar-g sum, F2 = FT + FL, the
+VEGAS iteration statistics of ``vegas_device``, and the qqbar-g term's choice
+between the fixed-Sobol estimator and VEGAS (S/dis/observables.py:459-532).
 
 Known answers follow the reference's own tests
 (T/tests/test_dis_production.py:70-107 for the LO factor
@@ -255,3 +256,94 @@ def test_vegas_zero_variance_is_floored_at_tiny(monkeypatch):
                        epsrel=1e-3, seed=0, precision="fp32", device="cpu")
     assert float(res.value) == 3.0 and res.converged
     assert float(res.error) == pytest.approx(math.sqrt(torch.finfo(torch.float32).tiny / 3), rel=1e-12)
+
+
+def _fixed(total_value, total_error, converged, seed):
+    from nlo_bk_optimized_amd.dis.fixed_sobol import FIXED_SOBOL_EVALUATIONS, FixedSobolResult
+
+    t = _result(total_value, total_error, FIXED_SOBOL_EVALUATIONS, converged, seed)
+    return FixedSobolResult((t, t, t), t)
+
+
+def test_fixed_sobol_support_is_limited_to_the_validated_workload():
+    """T/tests/test_dis_production.py:110-116 and S/dis/observables.py:640-658."""
+
+    from nlo_bk_optimized_amd.dis.fixed_sobol import fixed_sobol_supported
+
+    f32, f64 = torch.tensor(9.0), torch.tensor(9.0, dtype=torch.float64)
+    assert fixed_sobol_supported(f32, GBW(), DISConfig())
+    assert not fixed_sobol_supported(f32, GBW(), DISConfig(maxeval=100_000))
+    assert not fixed_sobol_supported(f64, GBW(), DISConfig())
+    assert not fixed_sobol_supported(f32, GBW(), DISConfig(cuda_nested_points=32))
+    assert not fixed_sobol_supported(f32, GBW(), DISConfig(cuda_nested=False))
+    assert not fixed_sobol_supported(f32, GBW(), DISConfig(cuda_fusion=False))
+    assert not fixed_sobol_supported(f32, object(), DISConfig())
+
+
+def test_qg_dispatches_to_fixed_sobol(fakes, monkeypatch):
+    """T/tests/test_dis_production.py:195-231: the fixed-Sobol totals, scaled by
+    e_q^2 4 Nc alpha_em / (2 pi)^3 2 pi, with no VEGAS integral."""
+
+    from nlo_bk_optimized_amd.dis import fixed_sobol
+
+    seen = []
+
+    def fake(entries, dipole, config, **kw):
+        seen.extend(entries)
+        return [_fixed(2.0, 0.1, True, e[4]) for e in entries]
+
+    monkeypatch.setattr(fixed_sobol, "gbw_qg_fixed_sobol_batch", fake)
+    light, charm = Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)
+    res = obs.sigma_qg_d2b(torch.tensor(9.0), torch.tensor(1e-3), Polarization.T, GBW(),
+                           DISConfig(quarks=(light, charm)), seed=11)
+    want = (light.charge**2 + charm.charge**2) * 2.0 * QG_FACTOR
+    torch.testing.assert_close(res.value, torch.tensor(want, dtype=torch.float64))
+    err = math.sqrt(light.charge**4 + charm.charge**4) * 0.1 * QG_FACTOR
+    torch.testing.assert_close(res.error, torch.tensor(err, dtype=torch.float64))
+    assert [e[4] for e in seen] == [11, 12]  # seed + quark index
+    assert [e[3] for e in seen] == [light, charm]
+    assert not fakes.calls
+
+
+def test_qg_falls_back_to_vegas_when_a_total_misses_epsrel(fakes, monkeypatch):
+    """S/dis/observables.py:476-491: one unconverged quark sends every quark of
+    the term to VEGAS, seeded seed + 0, 1, ..."""
+
+    from nlo_bk_optimized_amd.dis import fixed_sobol
+
+    def fake(entries, dipole, config, **kw):
+        return [_fixed(2.0, 0.1, k == 0, e[4]) for k, e in enumerate(entries)]
+
+    monkeypatch.setattr(fixed_sobol, "gbw_qg_fixed_sobol_batch", fake)
+    fakes.vegas.update({"I1": (1.0, 0.0), "I2_nested": (0.5, 0.0), "I3_nested": (0.25, 0.0)})
+    light, charm = Quark(QuarkType.LIGHT, 0.14), Quark(QuarkType.C, 1.27)
+    res = obs.sigma_qg_d2b(torch.tensor(9.0), torch.tensor(1e-3), Polarization.L, GBW(),
+                           DISConfig(quarks=(light, charm)), seed=3)
+    want = (light.charge**2 + charm.charge**2) * 1.75 * QG_FACTOR
+    torch.testing.assert_close(res.value, torch.tensor(want, dtype=torch.float64))
+    assert [c[1] for c in fakes.calls] == ["I1", "I2_nested", "I3_nested"] * 2
+    assert [c[3]["seed"] for c in fakes.calls] == [3, 4, 5, 6, 7, 8]
+
+
+def test_fixed_sobol_arguments_are_checked_on_the_host():
+    """S/dis/fixed_sobol.py:79-85, 286-294: sizes and scalars are refused before
+    any device work."""
+
+    from nlo_bk_optimized_amd.dis.fixed_sobol import gbw_qg_fixed_sobol_batch, make_sobol_points
+
+    charm = Quark(QuarkType.C, 1.4)
+    q, x = torch.tensor(9.0), torch.tensor(1e-3)
+    entry = [(q, x, Polarization.T, charm, 0)]
+    with pytest.raises(TypeError):
+        gbw_qg_fixed_sobol_batch(entry, object(), DISConfig())
+    with pytest.raises(ValueError, match="powers of two"):
+        gbw_qg_fixed_sobol_batch(entry, GBW(), DISConfig(), samples_per_replicate=1000)
+    with pytest.raises(ValueError, match="replicates must exceed one"):
+        gbw_qg_fixed_sobol_batch(entry, GBW(), DISConfig(), replicates=1)
+    with pytest.raises(ValueError, match="matching"):
+        gbw_qg_fixed_sobol_batch([(q, x.double(), Polarization.T, charm, 0)], GBW(), DISConfig())
+    with pytest.raises(ValueError, match="matching"):
+        gbw_qg_fixed_sobol_batch([(9.0, 1e-3, Polarization.T, charm, 0)], GBW(), DISConfig())
+    with pytest.raises(ValueError, match="powers of two"):
+        make_sobol_points(5, training_samples=100, training_rounds=1, samples_per_replicate=64, replicates=2,
+                          device="cpu", dtype=torch.float32, seed=0)
