@@ -8,7 +8,9 @@ paired-endpoint map (nlo_dis_endpoint_map) on the GPU against the oracle
   float32 / float64 ulps of the value (the device's cos / sin / atan2 are not
   numpy's; angle fractions compared on the circle).
 * The whole estimator in FP64 at reduced sizes (two training sets of 2048
-  points, three replicates of 2048, 8 bins), both polarisations: the learned
+  points, three replicates of 2048, 8 bins), both polarisations, at the
+  benchmark point (charm) and a light-quark point (m = 0.14, Q^2 = 50,
+  x = 1e-2, maxr = 30): the learned
   grid, I1, I2, I3 and the total (value and error) against the oracle at 1e-9
   relative; in FP32 (the reference's path: float32 map and endpoint split,
   nested I2 / I3 in FP32, I1 in FP64) at the FP32 integrand tolerance.
@@ -90,37 +92,48 @@ def test_endpoint_map_matches_oracle(dev, dtype, fold):
 
 
 def _device_run(dev, precision: str, transverse: bool, *, seed=7, training=2048, rounds=2, reps_n=2048, reps=3,
-                bins=8, return_edges=True, entries=None):
+                bins=8, return_edges=True, entries=None, point=(Q2, XBJ, MASS, MAXR)):
     from nlo_bk_optimized_amd.dis import Polarization
+    from nlo_bk_optimized_amd.dis.config import DISConfig, Quark, QuarkType
     from nlo_bk_optimized_amd.dis.fixed_sobol import gbw_qg_fixed_sobol_batch
 
     dtype = torch.float32 if precision == "fp32" else torch.float64
-    cfg = _config()
+    q2, xbj, mass, maxr = point
+    quark = Quark(QuarkType.C if mass > 1 else QuarkType.LIGHT, mass)
+    cfg = DISConfig(quarks=(quark,), maxr=maxr, nf_alphas=1 if mass > 1 else 3, cuda_nested_points=48)
     if entries is None:
-        entries = [(torch.tensor(Q2, dtype=dtype), torch.tensor(XBJ, dtype=dtype),
-                    Polarization.T if transverse else Polarization.L, cfg.quarks[0], seed)]
+        entries = [(torch.tensor(q2, dtype=dtype), torch.tensor(xbj, dtype=dtype),
+                    Polarization.T if transverse else Polarization.L, quark, seed)]
     return gbw_qg_fixed_sobol_batch(entries, _gbw(), cfg, training_samples=training, training_rounds=rounds,
                                     samples_per_replicate=reps_n, replicates=reps, bins=bins,
                                     return_edges=return_edges)
 
 
-def _oracle_run(dtype, transverse: bool, *, seed=7, training=2048, rounds=2, reps_n=2048, reps=3, bins=8):
-    return F.gbw_qg_fixed_sobol(Q2, XBJ, transverse, D.QuarkSpec(MASS, 2 / 3),
+def _oracle_run(dtype, transverse: bool, *, seed=7, training=2048, rounds=2, reps_n=2048, reps=3, bins=8,
+                point=(Q2, XBJ, MASS, MAXR)):
+    q2, xbj, mass, maxr = point
+    charge = 2 / 3 if mass > 1 else math.sqrt(2 / 3)
+    return F.gbw_qg_fixed_sobol(q2, xbj, transverse, D.QuarkSpec(mass, charge),
                                 D.GBWParams(qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0),
-                                D.DisSetup(maxr=MAXR, nf_alphas=1), seed=seed, dtype=dtype,
+                                D.DisSetup(maxr=maxr, nf_alphas=1 if mass > 1 else 3), seed=seed, dtype=dtype,
                                 training_samples=training, training_rounds=rounds, samples_per_replicate=reps_n,
                                 replicates=reps, bins=bins, return_edges=True)
+
+
+# the benchmark point and a light-quark point at larger Q^2 and x with the default maxr
+POINTS = [(Q2, XBJ, MASS, MAXR), (50.0, 1e-2, 0.14, 30.0)]
 
 
 def _rows(result):
     return list(result.contributions) + [result.total]
 
 
+@pytest.mark.parametrize("point", POINTS)
 @pytest.mark.parametrize("transverse", [False, True])
-def test_fixed_sobol_fp64_matches_oracle(dev, transverse):
-    print(f"fixed-Sobol FP64 transverse={transverse}", flush=True)
-    (res,), edges = _device_run(dev, "fp64", transverse)
-    contributions, total, oedges = _oracle_run(np.float64, transverse)
+def test_fixed_sobol_fp64_matches_oracle(dev, transverse, point):
+    print(f"fixed-Sobol FP64 transverse={transverse} point={point}", flush=True)
+    (res,), edges = _device_run(dev, "fp64", transverse, point=point)
+    contributions, total, oedges = _oracle_run(np.float64, transverse, point=point)
     np.testing.assert_allclose(edges[0].cpu().numpy(), oedges, rtol=0, atol=1e-11)
     for got, want in zip(_rows(res), list(contributions) + [total]):
         print(f"  {float(got.value):+.12e} +- {float(got.error):.3e}  oracle {want.value:+.12e} +- {want.error:.3e}")
@@ -130,15 +143,16 @@ def test_fixed_sobol_fp64_matches_oracle(dev, transverse):
         assert got.converged == want.converged
 
 
+@pytest.mark.parametrize("point", POINTS)
 @pytest.mark.parametrize("transverse", [False, True])
-def test_fixed_sobol_fp32_matches_oracle(dev, transverse):
+def test_fixed_sobol_fp32_matches_oracle(dev, transverse, point):
     """FP32 path: the learned grid within 1e-4 (FP32 integrand values move the
     histogram), the estimates within 2e-4 relative of the oracle's
     (FP64-integrand) values or a tenth of their standard error."""
 
-    print(f"fixed-Sobol FP32 transverse={transverse}", flush=True)
-    (res,), edges = _device_run(dev, "fp32", transverse)
-    contributions, total, oedges = _oracle_run(np.float32, transverse)
+    print(f"fixed-Sobol FP32 transverse={transverse} point={point}", flush=True)
+    (res,), edges = _device_run(dev, "fp32", transverse, point=point)
+    contributions, total, oedges = _oracle_run(np.float32, transverse, point=point)
     np.testing.assert_allclose(edges[0].cpu().numpy(), oedges, rtol=0, atol=1e-4)
     for got, want in zip(_rows(res), list(contributions) + [total]):
         print(f"  {float(got.value):+.10e} +- {float(got.error):.3e}  oracle {want.value:+.10e} +- {want.error:.3e}")
