@@ -14,6 +14,11 @@ paired-endpoint map (nlo_dis_endpoint_map) on the GPU against the oracle
   grid, I1, I2, I3 and the total (value and error) against the oracle at 1e-9
   relative; in FP32 (the reference's path: float32 map and endpoint split,
   nested I2 / I3 in FP32, I1 in FP64) at the FP32 integrand tolerance.
+* Edge sizes in FP64 against the oracle: sets smaller than a block (64 and 32
+  points, 2 bins), two points per set, and the most replicates (32) with the
+  most bins (255) on 256 points (1e-8 on the edges, 1e-7 on the estimates:
+  bins holding one point or none amplify the summation order); the endpoint
+  map of no point and of one point.
 * An estimator batched with others equals it alone, bit for bit; the frozen
   grid of a run, passed back, reproduces the replicate estimates with the
   replicate evaluations only.
@@ -256,3 +261,34 @@ def test_sigma_qg_takes_the_fixed_sobol_path(dev):
     vegas = sigma_qg_d2b(q2, x, Polarization.T, _gbw(), small, seed=11)
     assert vegas.n_eval != FIXED_SOBOL_EVALUATIONS
     assert abs(float(vegas.value) - float(sigma.value)) <= 5 * math.hypot(float(vegas.error), float(sigma.error))
+
+
+@pytest.mark.parametrize("sizes", [
+    dict(training=64, rounds=1, reps_n=32, reps=2, bins=2),     # partial blocks, two bins
+    dict(training=256, rounds=3, reps_n=128, reps=32, bins=255),  # the most replicates and bins
+    dict(training=2, rounds=2, reps_n=2, reps=3, bins=4),       # two points per set
+])
+def test_fixed_sobol_edge_sizes_match_oracle(dev, sizes):
+    print(f"fixed-Sobol FP64 edge sizes {sizes}", flush=True)
+    (res,), edges = _device_run(dev, "fp64", True, **sizes)
+    contributions, total, oedges = _oracle_run(np.float64, True, **sizes)
+    # with 255 bins and 256 points most bins hold one point or none, and the
+    # refinement divides by their small smoothed importances: the summation
+    # order of the histograms (per block, then over blocks, against the
+    # oracle's point order) shows up at 1e-9 in the edges and in the estimates
+    np.testing.assert_allclose(edges[0].cpu().numpy(), oedges, rtol=0, atol=1e-8)
+    for got, want in zip(_rows(res), list(contributions) + [total]):
+        assert float(got.value) == pytest.approx(want.value, rel=1e-7, abs=1e-14)
+        assert float(got.error) == pytest.approx(want.error, rel=1e-5, abs=1e-14)
+        assert got.n_eval == want.n_eval
+
+
+def test_endpoint_map_empty_and_single_point(dev):
+    edges = torch.linspace(0, 1, 9, dtype=torch.float32).expand(5, -1).contiguous().cuda()
+    empty = dev.dis_endpoint_map(torch.empty((0, 5), dtype=torch.float32, device="cuda"), edges, MAXR)
+    assert all(t.shape[0] == 0 for t in empty)
+    one = torch.tensor([[0.3, 0.6, 0.5, 0.25, 0.2]], dtype=torch.float32)
+    got = [t.cpu().numpy() for t in dev.dis_endpoint_map(one.cuda(), edges, MAXR, fold_angle=True)]
+    want = F.endpoint_map(one.numpy(), edges.cpu().numpy(), MAXR, fold_angle=True)
+    for g, w in zip(got, want):
+        np.testing.assert_allclose(g, w, rtol=1e-5, atol=1e-6)
