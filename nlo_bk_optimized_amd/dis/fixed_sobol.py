@@ -24,9 +24,12 @@ Differences from the reference, on purpose:
   float32, as the reference's;
 * I1 is evaluated in FP64 at every point (the reference evaluates an FP32
   regular region and an FP64 sensitive region, K/dis/i1/kernel.cu:412-433);
-* the generic forms ``learn_triple_importance_grid`` and
-  ``integrate_triple_fixed_sobol`` over Python callables are not provided: the
-  estimator runs the device integrands;
+* the generic forms over Python callables (``map_importance_grid``,
+  ``paired_endpoint_inputs``, ``learn_triple_importance_grid``,
+  ``integrate_triple_fixed_sobol``) keep the reference's signatures and run
+  their map, endpoint split, histogram and refinement in the library's kernels,
+  the callables on the caller's device tensors; ``gbw_qg_fixed_sobol`` itself
+  does not go through them (its integrands are fused into the passes);
 * float64 scalars are accepted too (every step in FP64), beside the
   reference's float32.
 """
@@ -107,6 +110,109 @@ def dis_sobol_endpoint_map(points: torch.Tensor, edges: torch.Tensor, maxr: floa
     if maxr <= 0 or maxr != maxr or maxr == float("inf"):
         raise ValueError("DIS Sobol maxr must be positive and finite")
     return dev.dis_endpoint_map(points.contiguous(), edges.contiguous(), maxr, fold_angle=fold_angle)
+
+
+def map_importance_grid(points: torch.Tensor, edges: torch.Tensor):
+    """S/dis/fixed_sobol.py:100-113 on the device (nlo_vegas_map): (source,
+    inverse density, bin index) of points [n, dim] through edges
+    [dim, bins + 1], in the points' dtype (the map runs in FP64)."""
+
+    from nlo_bk_optimized_amd import device as dev
+
+    u = points.to(torch.float64).contiguous()
+    e = edges.to(device=u.device, dtype=torch.float64).contiguous()
+    dim = u.shape[1]
+    zero = torch.zeros(dim, dtype=torch.float64, device=u.device)
+    x, jac, b = dev.vegas_map(u, e, zero, torch.ones_like(zero))
+    return x.to(points.dtype), jac.to(points.dtype), b.to(torch.long)
+
+
+def _unit_grid(dtype: torch.dtype, device) -> torch.Tensor:
+    # two bins on [0, 1]: the map is the identity bit for bit and the inverse
+    # density is 1
+    return torch.tensor([0.0, 0.5, 1.0], dtype=dtype, device=device).expand(5, -1).contiguous()
+
+
+def paired_endpoint_inputs(source: torch.Tensor, maxr: float, *, fold_angle: bool = False):
+    """S/dis/fixed_sobol.py:116-158 on the device: (first, first weight,
+    second, second weight) of mapped points [n, 5]."""
+
+    return dis_sobol_endpoint_map(source, _unit_grid(source.dtype, source.device), maxr, fold_angle=fold_angle)
+
+
+@torch.no_grad()
+def learn_triple_importance_grid(function, training_points, *, maxr: float, bins: int = GRID_BINS,
+                                 fold_angle: bool = True, i1_weight: float = 1.0) -> torch.Tensor:
+    """S/dis/fixed_sobol.py:161-210 with a caller's ``function(x) -> (regular
+    I1, sensitive I1, I2, I3)``: one grid learned from the normalised
+    magnitudes over the training sets; the histograms (nlo_vegas_histogram,
+    fixed-order sums) and the refinement (nlo_vegas_adapt, the points' epsilon)
+    in the library."""
+
+    from nlo_bk_optimized_amd import device as dev
+
+    if not training_points:
+        raise ValueError("at least one independent training point set is required")
+    reference = training_points[0]
+    dims = reference.shape[1]
+    edges = torch.linspace(0, 1, bins + 1, device=reference.device, dtype=torch.float64).expand(dims, -1).contiguous()
+    eps = float(torch.finfo(reference.dtype).eps)
+    tiny = torch.finfo(reference.dtype).tiny
+    for points in training_points:
+        source, inverse_density, bin_index = map_importance_grid(points, edges)
+        first, first_weight, second, second_weight = paired_endpoint_inputs(source, maxr, fold_angle=fold_angle)
+        first_regular, first_sensitive, first_I2, first_I3 = function(first)
+        second_regular, second_sensitive, second_I2, second_I3 = function(second)
+        first_values = first_regular + first_sensitive.float(), first_I2, first_I3
+        second_values = second_regular + second_sensitive.float(), second_I2, second_I3
+        weighted = tuple((a * first_weight + b * second_weight) * inverse_density
+                         for a, b in zip(first_values, second_values, strict=True))
+        score = sum(weight * value.abs() / value.abs().mean().clamp_min(tiny)
+                    for weight, value in zip((i1_weight, 1.0, 1.0), weighted, strict=True))
+        histogram, bin_count = dev.vegas_histogram(bin_index.to(torch.uint8).contiguous(),
+                                                   score.to(torch.float64).contiguous(), bins)
+        edges = dev.vegas_adapt(edges, histogram, bin_count, eps)
+    return edges.to(reference.dtype)
+
+
+@torch.no_grad()
+def integrate_triple_fixed_sobol(i1_sum_function, nested_function, edges: torch.Tensor, replicate_points, *,
+                                 maxr: float, epsrel: float, seed: int, training_evaluations: int = 0):
+    """S/dis/fixed_sobol.py:213-271 with a caller's ``i1_sum_function(x,
+    weights) -> (regular sums, sensitive sums)`` and ``nested_function(x) ->
+    (I2, I3)``: the replicate estimates of I1, I2, I3 and of their correlated
+    sum; the endpoint map of every replicate in the library."""
+
+    import math
+
+    if len(replicate_points) < 2:
+        raise ValueError("at least two scrambled Sobol replicates are required")
+    estimates = []
+    for points in replicate_points:
+        first, first_weight, second, second_weight = dis_sobol_endpoint_map(points.contiguous(), edges, maxr,
+                                                                            fold_angle=True)
+        first_regular_sums, first_sensitive_sums = i1_sum_function(first, first_weight)
+        second_regular_sums, second_sensitive_sums = i1_sum_function(second, second_weight)
+        first_I2, first_I3 = nested_function(first)
+        second_I2, second_I3 = nested_function(second)
+        sample_count = points.shape[0]
+        regular_I1 = (first_regular_sums.sum() + second_regular_sums.sum()) / sample_count
+        sensitive_I1 = ((first_sensitive_sums.sum() + second_sensitive_sums.sum()) / sample_count).float()
+        estimates.append(torch.stack((
+            (regular_I1 + sensitive_I1).double(),
+            (first_I2 * first_weight + second_I2 * second_weight).double().mean(),
+            (first_I3 * first_weight + second_I3 * second_weight).double().mean(),
+        )))
+    stacked = torch.stack(estimates)
+    samples = training_evaluations + 2 * sum(p.shape[0] for p in replicate_points)
+
+    def result(values: torch.Tensor) -> IntegralResult:
+        value = values.mean()
+        error = values.std(unbiased=True) / math.sqrt(values.numel())
+        return IntegralResult(value, error, samples, bool(error <= epsrel * value.abs()), seed)
+
+    contributions = tuple(result(stacked[:, k]) for k in range(3))
+    return FixedSobolResult(contributions, result(stacked.sum(dim=1)))
 
 
 def fixed_sobol_supported(Q2, dipole, config: DISConfig) -> bool:
@@ -223,5 +329,9 @@ __all__ = [
     "fixed_sobol_supported",
     "gbw_qg_fixed_sobol",
     "gbw_qg_fixed_sobol_batch",
+    "integrate_triple_fixed_sobol",
+    "learn_triple_importance_grid",
     "make_sobol_points",
+    "map_importance_grid",
+    "paired_endpoint_inputs",
 ]

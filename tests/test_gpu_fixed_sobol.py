@@ -27,6 +27,10 @@ paired-endpoint map (nlo_dis_endpoint_map) on the GPU against the oracle
   C++ values with their errors (T/benchmarks/dis_fixed_sobol_all_qg.py:69-76).
 * sigma_qg_d2b takes the fixed-Sobol path where the reference does
   (S/dis/observables.py:470-491) and VEGAS below the evaluation budget.
+* The generic forms over Python callables: learn_triple_importance_grid on
+  the device against the reference's own grids (golden vectors, 1e-11),
+  paired_endpoint_inputs against the oracle, integrate_triple_fixed_sobol's
+  known answer (T/tests/test_dis_production.py:153-192).
 """
 
 from __future__ import annotations
@@ -292,3 +296,71 @@ def test_endpoint_map_empty_and_single_point(dev):
     want = F.endpoint_map(one.numpy(), edges.cpu().numpy(), MAXR, fold_angle=True)
     for g, w in zip(got, want):
         np.testing.assert_allclose(g, w, rtol=1e-5, atol=1e-6)
+
+
+def _synthetic_torch(x):
+    """tests/golden/make_fixed_sobol_golden.py synthetic() as the reference's
+    function(x) returns it (regular, sensitive in float64, I2, I3)."""
+
+    phi = 2 * math.pi * x[:, 4]
+    regular = x[:, 0] * (1 + x[:, 3]) * torch.cos(phi) + 0.1
+    sensitive = 0.01 * x[:, 1] * x[:, 2]
+    i2 = torch.exp(-3 * x[:, 1]) * x[:, 2] - 0.05
+    i3 = (x[:, 3] - 0.4) * x[:, 0] ** 2 * (1 + torch.cos(phi) ** 2)
+    return regular, sensitive.double(), i2, i3
+
+
+def test_generic_grid_learning_matches_reference_vectors(dev):
+    """learn_triple_importance_grid over a Python callable, on the device,
+    against the reference's own function's grids
+    (tests/golden/fixed_sobol_reference_vectors.npz)."""
+
+    from pathlib import Path
+
+    from nlo_bk_optimized_amd.dis.fixed_sobol import learn_triple_importance_grid, make_sobol_points
+
+    golden = np.load(Path(__file__).resolve().parent / "golden" / "fixed_sobol_reference_vectors.npz")
+    for k in range(3):
+        seed, sets, count, bins, fold, i1_weight, maxr = golden[f"case{k}_params"]
+        print(f"generic grid learning case {k}", flush=True)
+        pts = make_sobol_points(5, training_samples=int(count), training_rounds=int(sets), samples_per_replicate=2,
+                                replicates=2, device="cuda", dtype=torch.float64, seed=int(seed))
+        edges = learn_triple_importance_grid(_synthetic_torch, pts.training, maxr=float(maxr), bins=int(bins),
+                                             fold_angle=bool(fold), i1_weight=float(i1_weight))
+        np.testing.assert_allclose(edges.cpu().numpy(), golden[f"case{k}_edges"], rtol=0, atol=1e-11)
+
+
+def test_generic_paired_endpoint_inputs_match_oracle(dev):
+    from nlo_bk_optimized_amd.dis.fixed_sobol import paired_endpoint_inputs
+
+    src = F.sobol_points(31, 2048, np.float64)
+    for fold in (False, True):
+        got = [t.cpu().numpy() for t in paired_endpoint_inputs(torch.from_numpy(src).cuda(), MAXR, fold_angle=fold)]
+        want = F.paired_endpoint_inputs(src, MAXR, fold_angle=fold)
+        for g, w in zip(got, want):
+            np.testing.assert_allclose(g, w, rtol=1e-13, atol=1e-14)
+
+
+def test_generic_integrate_known_answer(dev, monkeypatch):
+    """T/tests/test_dis_production.py:153-192 on device tensors."""
+
+    from nlo_bk_optimized_amd.dis import fixed_sobol
+
+    monkeypatch.setattr(fixed_sobol, "dis_sobol_endpoint_map", lambda points, edges, maxr, fold_angle: (
+        points, torch.ones(points.shape[0], device=points.device), points,
+        torch.zeros(points.shape[0], device=points.device)))
+
+    def i1_sums(x, weights):
+        return x[:, 0] * weights, torch.zeros(1, dtype=torch.float64, device=x.device)
+
+    def i2_i3(x):
+        return x[:, 1], -0.5 * x[:, 0]
+
+    replicates = tuple(torch.tensor(v, device="cuda") for v in
+                       ([[0.1, 0.8], [0.3, 0.4]], [[0.5, 0.2], [0.7, 0.6]], [[0.2, 0.9], [0.4, 0.1]]))
+    result = fixed_sobol.integrate_triple_fixed_sobol(i1_sums, i2_i3, torch.empty(0), replicates, maxr=5.0,
+                                                      epsrel=1.0, seed=13, training_evaluations=10)
+    totals = torch.tensor([0.7, 0.7, 0.65], dtype=torch.float64)
+    torch.testing.assert_close(result.total.value.cpu(), totals.mean())
+    torch.testing.assert_close(result.total.error.cpu(), totals.std(unbiased=True) / math.sqrt(3))
+    assert result.total.n_eval == 22 and result.total.seed == 13
