@@ -499,6 +499,14 @@ int nlo_dis_dimensions(int32_t contribution);
 int nlo_dis_integrand(const nlo_dis_desc* desc, int32_t contribution, int64_t n, const double* d_x,
                       int32_t precision, double* d_out, hipStream_t stream);
 
+/* The nested I2 and I3 at n device points x (n x 5 doubles) from one pass
+ * over the inner rule, into d_i2 and d_i3 (n doubles each); quark 0 of desc.
+ * Replaces dis_nested_gbw_i2_i3_integrand_cuda / dis_nested_i2_i3_cuda
+ * (K/dis/i2_i3/extension.py), the pair integrand_qg_nested_massive
+ * (S/dis/observables.py:253-331) evaluates one at a time. */
+int nlo_dis_qg_i23(const nlo_dis_desc* desc, int64_t n, const double* d_x, int32_t precision,
+                   double* d_i2, double* d_i3, hipStream_t stream);
+
 /* One VEGAS pass: n scrambled-Sobol points (indices first_index ..
  * first_index + n - 1 of `sobol`, dimension = the contribution's) mapped
  * through d_edges (dim x (bins + 1), on [0, 1]), the integrand evaluated and

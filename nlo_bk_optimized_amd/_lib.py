@@ -348,6 +348,7 @@ SIGNATURES: dict[str, tuple] = {
     "nlo_dis_bk_local_splines": (ctypes.c_int, [ctypes.POINTER(DisDipole), _vp, _vp]),
     "nlo_dis_batch_workspace": (_size, [ctypes.POINTER(DisBatchDesc)]),
     "nlo_dis_batch_run": (ctypes.c_int, [ctypes.POINTER(DisBatchDesc), _vp, _vp, _size, _vp]),
+    "nlo_dis_qg_i23": (ctypes.c_int, [ctypes.POINTER(DisDesc), _i64, _vp, _i32, _vp, _vp, _vp]),
     "nlo_dis_endpoint_map": (ctypes.c_int, [_i32, _i64, _vp, _vp, _i32, _f64, _i32, _vp, _vp, _vp, _vp, _vp]),
     "nlo_dis_fixed_sobol_workspace": (_size, [ctypes.POINTER(FixedSobolDesc)]),
     "nlo_dis_fixed_sobol": (ctypes.c_int, [ctypes.POINTER(FixedSobolDesc), _vp, _vp, _vp, _size, _vp]),

@@ -145,6 +145,23 @@ __global__ void __launch_bounds__(256) integrand_kernel(DisArgs a, int64_t n, co
     out[i] = (double)dis_value<T, K>(a, u, rule);
 }
 
+// The nested I2 and I3 at each point from one pass over the inner rule.
+template <typename T>
+__global__ void __launch_bounds__(256) i23_kernel(DisArgs a, int64_t n, const double* __restrict__ x,
+                                                  double* __restrict__ i2, double* __restrict__ i3) {
+    __shared__ double ry[kMaxNested], rw[kMaxNested];
+    const RuleView rule = stage_nested_rule(a.nested_points, ry, rw);
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[5];
+#pragma unroll
+    for (int d = 0; d < 5; ++d) u[d] = x[i * 5 + d];
+    T v2, v3;
+    qg_nested_pair<T>(a, u, rule, v2, v3);
+    i2[i] = (double)v2;
+    i3[i] = (double)v3;
+}
+
 struct VegasArgs {
     int bins, spt, blocks;
     int64_t n, first;
@@ -1140,6 +1157,25 @@ int nlo_dis_integrand(const nlo_dis_desc* desc, int32_t contribution, int64_t n,
         break;
     DIS_DISPATCH(DIS_POINT)
 #undef DIS_POINT
+    NLO_LAUNCH_CHECK();
+    return NLO_OK;
+}
+
+int nlo_dis_qg_i23(const nlo_dis_desc* desc, int64_t n, const double* d_x, int32_t precision, double* d_i2,
+                   double* d_i3, hipStream_t stream) {
+    clear_error();
+    NLO_REQUIRE(precision == NLO_PREC_FP64 || precision == NLO_PREC_FP32, "precision must be FP64 or FP32");
+    NLO_REQUIRE(n >= 0, "n must be >= 0");
+    DisArgs a;
+    int rc = make_args(desc, a);
+    if (rc != NLO_OK) return rc;
+    if (n == 0) return NLO_OK;
+    NLO_REQUIRE(d_x && d_i2 && d_i3, "NULL array");
+    const dim3 grid((unsigned)((n + 255) / 256));
+    if (precision == NLO_PREC_FP64)
+        hipLaunchKernelGGL(i23_kernel<double>, grid, dim3(256), 0, stream, a, n, d_x, d_i2, d_i3);
+    else
+        hipLaunchKernelGGL(i23_kernel<float>, grid, dim3(256), 0, stream, a, n, d_x, d_i2, d_i3);
     NLO_LAUNCH_CHECK();
     return NLO_OK;
 }

@@ -630,6 +630,20 @@ def dis_integrand(desc: _lib.DisDesc, kind: str, x: torch.Tensor, *, precision: 
     return out
 
 
+def dis_qg_i23(desc: _lib.DisDesc, x: torch.Tensor, *, precision: str = "fp64") -> tuple[torch.Tensor, torch.Tensor]:
+    """The nested I2 and I3 at unit-cube points x[n, 5] (float64 device) from
+    one pass over the inner rule."""
+
+    require_cuda(x, torch.float64, "x")
+    if x.ndim != 2 or x.shape[1] != 5:
+        raise ValueError("the nested I2 / I3 need points of shape (n, 5)")
+    i2 = torch.empty(x.shape[0], dtype=torch.float64, device=x.device)
+    i3 = torch.empty_like(i2)
+    check(lib().nlo_dis_qg_i23(ctypes.byref(desc), x.shape[0], ptr(x), PRECISION_CODES[precision], ptr(i2),
+                               ptr(i3), stream_handle(x.device)), "nlo_dis_qg_i23")
+    return i2, i3
+
+
 def dis_vegas(desc: _lib.DisDesc, kind: str, edges: torch.Tensor, table: _lib.SobolTable, n: int, *,
               first_index: int = 0, precision: str = "fp64", histogram: bool = True):
     """One VEGAS pass: (sums[2] = (sum w, sum w^2), hist[dim, B], count[dim, B])."""

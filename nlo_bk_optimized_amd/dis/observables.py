@@ -187,6 +187,23 @@ def integrand_qg_nested_massive(x: torch.Tensor, Q2, xbj, polarization: Polariza
     return _evaluate(contribution + "_nested", x, Q2, xbj, polarization, (quark,), dipole, config)
 
 
+def integrand_qg_nested_pair(x: torch.Tensor, Q2, xbj, polarization: Polarization, quark: Quark, dipole: Dipole,
+                             config: DISConfig) -> tuple[torch.Tensor, torch.Tensor]:
+    """(I2, I3) of integrand_qg_nested_massive at the same points from one pass
+    over the inner rule (the pair dis_nested_gbw_i2_i3_integrand_cuda returns,
+    K/dis/i2_i3/extension.py)."""
+
+    from nlo_bk_optimized_amd import device as dev
+
+    if x.ndim != 2 or x.shape[1] != 5:
+        raise ValueError(f"the nested I2 / I3 need points of shape (n, 5), got {tuple(x.shape)}")
+    device = _device_of(dipole, x)
+    i2, i3 = dev.dis_qg_i23(_desc(Q2, xbj, polarization, (quark,), dipole, config), _points(x, device),
+                            precision=_precision(Q2))
+    dtype = x.dtype if x.is_floating_point() else torch.float64
+    return i2.to(dtype), i3.to(dtype)
+
+
 # --------------------------------------------------------- cross sections
 
 
@@ -602,6 +619,7 @@ __all__ = [
     "integrand_dip_massive",
     "integrand_photon_target_LO_unit",
     "integrand_qg_nested_massive",
+    "integrand_qg_nested_pair",
     "integrand_qgunsub_massive",
     "photon_proton_cross_section_LO_d2b",
     "photon_proton_cross_section_d2b",

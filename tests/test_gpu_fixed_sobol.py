@@ -364,3 +364,37 @@ def test_generic_integrate_known_answer(dev, monkeypatch):
     torch.testing.assert_close(result.total.value.cpu(), totals.mean())
     torch.testing.assert_close(result.total.error.cpu(), totals.std(unbiased=True) / math.sqrt(3))
     assert result.total.n_eval == 22 and result.total.seed == 13
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("transverse", [False, True])
+def test_nested_pair_equals_the_single_terms(dev, precision, transverse):
+    """nlo_dis_qg_i23 (both nested terms from one pass over the inner rule)
+    against the nested I2 and I3 kinds of nlo_dis_integrand at the same points
+    (the same operations: FP64 to 1e-13, FP32 to 1e-6 relative) and, in FP64,
+    against the oracle."""
+
+    from nlo_bk_optimized_amd.dis import Polarization
+    from nlo_bk_optimized_amd.dis.observables import integrand_qg_nested_massive, integrand_qg_nested_pair
+
+    cfg = _config()
+    pol = Polarization.T if transverse else Polarization.L
+    dtype = torch.float32 if precision == "fp32" else torch.float64
+    q2, x = torch.tensor(Q2, dtype=dtype), torch.tensor(XBJ, dtype=dtype)
+    pts = torch.from_numpy(F.sobol_points(41, 4096, np.float64))
+    print(f"nested pair {precision} transverse={transverse}", flush=True)
+    i2, i3 = integrand_qg_nested_pair(pts, q2, x, pol, cfg.quarks[0], _gbw(), cfg)
+    s2 = integrand_qg_nested_massive(pts, q2, x, pol, cfg.quarks[0], "I2", _gbw(), cfg)
+    s3 = integrand_qg_nested_massive(pts, q2, x, pol, cfg.quarks[0], "I3", _gbw(), cfg)
+    rtol = 1e-13 if precision == "fp64" else 1e-6
+    for a, b in ((i2, s2), (i3, s3)):
+        scale = float(b.abs().max())
+        np.testing.assert_allclose(a.cpu().numpy(), b.cpu().numpy(), rtol=rtol, atol=rtol * 1e-3 * scale)
+    if precision == "fp64":
+        amp = lambda r, Y: D.gbw(r, Y, D.GBWParams(qs0sqr=1.0, lambda_=0.3, gamma=1.0, x0=1.0))  # noqa: E731
+        setup = D.DisSetup(maxr=MAXR, nf_alphas=1)
+        for c, got in (("I2", i2), ("I3", i3)):
+            want = D.qg_integrand(pts.numpy(), Q2, XBJ, transverse, D.QuarkSpec(MASS, 2 / 3), c, amp, setup,
+                                  nested_points=48)
+            scale = float(np.max(np.abs(want)))
+            np.testing.assert_allclose(got.cpu().numpy(), want, rtol=1e-9, atol=1e-12 * scale)
