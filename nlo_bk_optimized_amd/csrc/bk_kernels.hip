@@ -466,25 +466,6 @@ __device__ __forceinline__ void gk_insert_wave(const GkList& L, int count, int s
     }
 }
 
-// The same by one thread.
-__device__ __forceinline__ void gk_insert_serial(const GkList& L, int count, int split, double mid,
-                                                 double e0, double r0, double e1, double r1) {
-    const double u = L.hi[split];
-    for (int i = count; i > split + 1; --i) {
-        L.lo[i] = L.lo[i - 1];
-        L.hi[i] = L.hi[i - 1];
-        L.est[i] = L.est[i - 1];
-        L.err[i] = L.err[i - 1];
-    }
-    L.hi[split] = mid;
-    L.est[split] = e0;
-    L.err[split] = r0;
-    L.lo[split + 1] = mid;
-    L.hi[split + 1] = u;
-    L.est[split + 1] = e1;
-    L.err[split + 1] = r1;
-}
-
 // The per-daughter terms of the K1 integrand at z = e^{log_z} (K1Z).
 template <bool ANY>
 __device__ __forceinline__ K1Z k1_daughter(const K1Parent& P, double log_z, const LdsSpline<double>& s,
@@ -622,7 +603,7 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     const int rows = (ANY && H.rows) ? min(max(*H.rows, 1), H.capacity) : 0;
     const double rapidity = rows > 0 ? H.y[rows - 1] : 0.0;
 
-    int count = 0;  // outer intervals (thread 0's copy is authoritative)
+    int count = 0;  // outer intervals (wave 0 keeps the outer list; its lanes hold equal copies)
     int split = 0;
     double value = 0.0, error = 0.0;
     long long evals = 0;
@@ -647,34 +628,39 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
             if (lane == 0) o_val[t] = inner * exp(2.0 * lz) * 2.0;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (wave == 0) {
             if (count == 0) {
                 double e, rr;
                 gk_from_values(a0, b0, o_val, e, rr);
-                O.lo[0] = a0;
-                O.hi[0] = b0;
-                O.est[0] = e;
-                O.err[0] = rr;
+                if (lane == 0) {
+                    O.lo[0] = a0;
+                    O.hi[0] = b0;
+                    O.est[0] = e;
+                    O.err[0] = rr;
+                }
                 count = 1;
             } else {
                 const double a = O.lo[split], b = O.hi[split], mid = (a + b) / 2.0;
                 double e0, r0, e1, r1;
                 gk_from_values(a, mid, o_val, e0, r0);
                 gk_from_values(mid, b, o_val + 21, e1, r1);
-                gk_insert_serial(O, count, split, mid, e0, r0, e1, r1);
+                gk_insert_wave(O, count, split, mid, e0, r0, e1, r1, lane);
                 ++count;
             }
+            wave_sync();
             gk_scan(O, count, value, error, split);
             if (count >= rint || error <= epsrel * fabs(value)) {
-                o_done = 1;
+                if (lane == 0) o_done = 1;
             } else {
                 const double a = O.lo[split], b = O.hi[split], mid = (a + b) / 2.0;
-                for (int k = 0; k < 21; ++k) {
-                    o_node[k] = gk_node(a, mid, k);
-                    o_node[21 + k] = gk_node(mid, b, k);
+                if (lane < 21) {
+                    o_node[lane] = gk_node(a, mid, lane);
+                    o_node[21 + lane] = gk_node(mid, b, lane);
                 }
-                n_tasks = 42;
-                task_next = 0;
+                if (lane == 0) {
+                    n_tasks = 42;
+                    task_next = 0;
+                }
             }
         }
         __syncthreads();
