@@ -15,6 +15,7 @@ import torch
 LIBRARY = Path(__file__).resolve().parent / "libnlo_gfx950.so"
 
 NLO_OK = 0
+NLO_API_VERSION = 3  # include/nlo_gfx950.h
 NLO_SOBOL_U32, NLO_SOBOL_F32, NLO_SOBOL_F64 = 0, 1, 2
 NLO_PREC_FP64, NLO_PREC_FP32, NLO_PREC_FP64_STABLE = 0, 1, 2
 NLO_BK_SUM_ONLY = 0x100
@@ -372,6 +373,10 @@ def lib() -> ctypes.CDLL:
             fn = getattr(handle, name)
             fn.restype = restype
             fn.argtypes = argtypes
+        version = handle.nlo_api_version()
+        if version != NLO_API_VERSION:
+            raise RuntimeError(f"{LIBRARY} has interface version {version}, these bindings expect "
+                               f"{NLO_API_VERSION}: rebuild it with `python -m nlo_bk_optimized_amd.build`")
         _lib = handle
     return _lib
 

@@ -32,7 +32,9 @@ def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(str(_lib.LIBRARY))
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert not missing, missing
-    assert _lib.lib().nlo_api_version() == 3
+    header_version = int(re.search(r"#define NLO_API_VERSION (\d+)", HEADER.read_text()).group(1))
+    assert _lib.NLO_API_VERSION == header_version
+    assert _lib.lib().nlo_api_version() == header_version
 
 
 def _gauss_legendre_mp(n: int):
