@@ -527,9 +527,13 @@ __device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const doubl
             v[lane] = k1_theta_value<ANY>(P, Z, x, s, c, H, rows, rapidity);
         }
         wave_sync();
-        double e0, r0, e1, r1;
-        gk_from_values(a, mid, v, e0, r0);
-        gk_from_values(mid, b, v + 21, e1, r1);
+        // lanes 0-31 sum the rule on [a, mid], lanes 32-63 on [mid, b]; lanes
+        // 0 and 32 hand both results to the whole wave
+        const bool upper = lane >= 32;
+        double eh, rh;
+        gk_from_values(upper ? mid : a, upper ? b : mid, upper ? v + 21 : v, eh, rh);
+        const double e0 = __shfl(eh, 0, 64), r0 = __shfl(rh, 0, 64);
+        const double e1 = __shfl(eh, 32, 64), r1 = __shfl(rh, 32, 64);
         gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
         ++count;
         evals += 42;
