@@ -413,17 +413,27 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Summed estimate and error in list order; first interval of largest error.
-__device__ __forceinline__ void gk_scan(const GkList& L, int count, double& value, double& error,
-                                        int& split) {
-    value = 0.0;
-    error = 0.0;
-    split = 0;
+// The summed estimate and error of a list in list order, and its first
+// interval of largest error, by a whole wave: lanes 0-31 sum the estimates,
+// lanes 32-63 the errors and find the first largest one; lanes 0 and 32 hand
+// the results to the wave.
+__device__ __forceinline__ void gk_scan_wave(const GkList& L, int count, double& value, double& error,
+                                             int& split, int lane) {
+    const bool upper = lane >= 32;
+    const double* x = upper ? L.err : L.est;
+    double acc = 0.0, best = L.err[0];
+    int arg = 0;
     for (int i = 0; i < count; ++i) {
-        value += L.est[i];
-        error += L.err[i];
-        if (L.err[i] > L.err[split]) split = i;
+        const double xi = x[i];
+        acc += xi;
+        if (xi > best) {  // err[i] > err[arg] on the upper half
+            best = xi;
+            arg = i;
+        }
     }
+    value = __shfl(acc, 0, 64);
+    error = __shfl(acc, 32, 64);
+    split = __shfl(arg, 32, 64);
 }
 
 // Replace entry `split` of a list of `count` (< cap <= 128) entries by its
@@ -519,7 +529,7 @@ __device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const doubl
     for (;;) {
         wave_sync();
         double error;
-        gk_scan(L, count, value, error, split);
+        gk_scan_wave(L, count, value, error, split, lane);
         if (count >= tint || error <= epsrel * fabs(value)) break;
         const double a = L.lo[split], b = L.hi[split], mid = (a + b) / 2.0;
         if (lane < 42) {
@@ -652,7 +662,7 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
                 ++count;
             }
             wave_sync();
-            gk_scan(O, count, value, error, split);
+            gk_scan_wave(O, count, value, error, split, lane);
             if (count >= rint || error <= epsrel * fabs(value)) {
                 if (lane == 0) o_done = 1;
             } else {
