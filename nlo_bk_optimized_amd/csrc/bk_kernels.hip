@@ -505,16 +505,15 @@ __device__ __forceinline__ double k1_theta_value(const K1Parent& P, const K1Z& Z
         return k1_theta_term(P, Z, sn * sn, s, c);
 }
 
-// The theta integral at one daughter size z for parent P, by one wave, from
-// the 21 values `first` of the block's first pass on [0, pi]; each bisection
+// The theta integral at one daughter size z (terms Z) for parent P, by one
+// wave, from the 21 values `first` of the block's first pass on [0, pi]; each bisection
 // of [a, b] evaluates the 21 nodes of [a, mid] (lanes 0-20) and of [mid, b]
 // (lanes 21-41).
 template <bool ANY>
-__device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const double* first,
+__device__ double k1_theta_adaptive(const K1Parent& P, const K1Z& Z, const double* first,
                                     const LdsSpline<double>& s, const BkConsts& c, const K1History& H,
                                     int rows, double rapidity, int tint, double epsrel, const GkList& L,
                                     double* v, int lane, long long& evals) {
-    const K1Z Z = k1_daughter<ANY>(P, log_z, s, c);
     double e, r;
     gk_from_values(0.0, kPi, first, e, r);
     if (lane == 0) {
@@ -554,11 +553,14 @@ __device__ double k1_theta_adaptive(const K1Parent& P, double log_z, const doubl
 
 // LDS of the adaptive K1 kernel (doubles): spline (R nodes + 4 (R - 1)
 // coefficients), J1 table, outer list (4 cap), outer nodes and values (2 x
-// 42), the first-pass values of the pending inner integrals (42 x 21), then
-// per wave a list (4 cap) and 42 node values.
+// 42), the first-pass values of the pending inner integrals (42 x 21) and
+// their daughter terms (42 K1Z), then per wave a list (4 cap) and 42 node
+// values.
+constexpr int kK1ZWords = sizeof(K1Z) / sizeof(double);
+static_assert(sizeof(K1Z) == kK1ZWords * sizeof(double), "K1Z is a record of doubles");
 __host__ __device__ inline size_t k1a_lds_doubles(int R, int cap) {
     return (size_t)R + 4 * (size_t)(R - 1) + kJ1Pieces * (kJ1Deg + 1) + 4 * (size_t)cap + 84 + 42 * 21 +
-           (size_t)kK1aWaves * (4 * (size_t)cap + 42);
+           42 * kK1ZWords + (size_t)kK1aWaves * (4 * (size_t)cap + 42);
 }
 
 // Parent p of one evolution (arrays of that evolution).
@@ -587,8 +589,9 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     double* o_node = o_base + 4 * cap;  // 42 ln z nodes
     double* o_val = o_node + 42;        // 42 outer integrand values
     double* f_val = o_val + 42;         // 42 x 21 first-pass theta values
+    K1Z* z_tab = reinterpret_cast<K1Z*>(f_val + 42 * 21);  // daughter terms of the 42 nodes
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double* w_base = f_val + 42 * 21 + (size_t)wave * (4 * cap + 42);
+    double* w_base = f_val + 42 * 21 + 42 * kK1ZWords + (size_t)wave * (4 * cap + 42);
     const GkList W{w_base, w_base + cap, w_base + 2 * cap, w_base + 3 * cap};
     double* w_val = w_base + 4 * cap;
 
@@ -622,11 +625,14 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     double value = 0.0, error = 0.0;
     long long evals = 0;
     for (int step = 0; step < rint; ++step) {
+        // the daughter terms of every pending inner integral, once
+        for (int t = threadIdx.x; t < n_tasks; t += blockDim.x) z_tab[t] = k1_daughter<ANY>(P, o_node[t], s, c);
+        __syncthreads();
         // first pass on [0, pi] of every pending inner integral, over the
         // whole block (n_tasks x 21 theta nodes)
         for (int i = threadIdx.x; i < n_tasks * 21; i += blockDim.x) {
             const int t = i / 21, k = i - 21 * t;
-            const K1Z Z = k1_daughter<ANY>(P, o_node[t], s, c);
+            const K1Z Z = z_tab[t];
             f_val[i] = k1_theta_value<ANY>(P, Z, gk_node(0.0, kPi, k), s, c, H, rows, rapidity);
         }
         __syncthreads();
@@ -637,7 +643,8 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
             t = __shfl(t, 0, 64);
             if (t >= n_tasks) break;
             const double lz = o_node[t];
-            const double inner = k1_theta_adaptive<ANY>(P, lz, f_val + 21 * t, s, c, H, rows, rapidity, tint,
+            const K1Z Z = z_tab[t];
+            const double inner = k1_theta_adaptive<ANY>(P, Z, f_val + 21 * t, s, c, H, rows, rapidity, tint,
                                                         epsrel, W, w_val, lane, evals);
             if (lane == 0) o_val[t] = inner * exp(2.0 * lz) * 2.0;
         }
