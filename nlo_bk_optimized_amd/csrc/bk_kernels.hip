@@ -436,6 +436,20 @@ __device__ __forceinline__ void gk_scan_wave(const GkList& L, int count, double&
     split = __shfl(arg, 32, 64);
 }
 
+// The rule on [a, mid] (values v[0..20]) and on [mid, b] (v[21..41]) by a
+// whole wave: lanes 0-31 sum the first, lanes 32-63 the second; lanes 0 and
+// 32 hand the results to the wave.
+__device__ __forceinline__ void gk_halves_wave(double a, double mid, double b, const double* v, double& e0,
+                                               double& r0, double& e1, double& r1, int lane) {
+    const bool upper = lane >= 32;
+    double eh, rh;
+    gk_from_values(upper ? mid : a, upper ? b : mid, upper ? v + 21 : v, eh, rh);
+    e0 = __shfl(eh, 0, 64);
+    r0 = __shfl(rh, 0, 64);
+    e1 = __shfl(eh, 32, 64);
+    r1 = __shfl(rh, 32, 64);
+}
+
 // Replace entry `split` of a list of `count` (< cap <= 128) entries by its
 // halves [lo, mid] and [mid, hi] (estimates e0, e1; errors r0, r1), moving the
 // entries after it one place up.  Every lane of one wave calls it: each lane
@@ -506,16 +520,15 @@ __device__ __forceinline__ double k1_theta_value(const K1Parent& P, const K1Z& Z
 }
 
 // The theta integral at one daughter size z (terms Z) for parent P, by one
-// wave, from the 21 values `first` of the block's first pass on [0, pi]; each bisection
+// wave, from the rule on [0, pi] of the block's first pass (estimate e,
+// error r); each bisection
 // of [a, b] evaluates the 21 nodes of [a, mid] (lanes 0-20) and of [mid, b]
 // (lanes 21-41).
 template <bool ANY>
-__device__ double k1_theta_adaptive(const K1Parent& P, const K1Z& Z, const double* first,
+__device__ double k1_theta_adaptive(const K1Parent& P, const K1Z& Z, double e, double r,
                                     const LdsSpline<double>& s, const BkConsts& c, const K1History& H,
                                     int rows, double rapidity, int tint, double epsrel, const GkList& L,
                                     double* v, int lane, long long& evals) {
-    double e, r;
-    gk_from_values(0.0, kPi, first, e, r);
     if (lane == 0) {
         L.lo[0] = 0.0;
         L.hi[0] = kPi;
@@ -536,13 +549,8 @@ __device__ double k1_theta_adaptive(const K1Parent& P, const K1Z& Z, const doubl
             v[lane] = k1_theta_value<ANY>(P, Z, x, s, c, H, rows, rapidity);
         }
         wave_sync();
-        // lanes 0-31 sum the rule on [a, mid], lanes 32-63 on [mid, b]; lanes
-        // 0 and 32 hand both results to the whole wave
-        const bool upper = lane >= 32;
-        double eh, rh;
-        gk_from_values(upper ? mid : a, upper ? b : mid, upper ? v + 21 : v, eh, rh);
-        const double e0 = __shfl(eh, 0, 64), r0 = __shfl(rh, 0, 64);
-        const double e1 = __shfl(eh, 32, 64), r1 = __shfl(rh, 32, 64);
+        double e0, r0, e1, r1;
+        gk_halves_wave(a, mid, b, v, e0, r0, e1, r1, lane);
         gk_insert_wave(L, count, split, mid, e0, r0, e1, r1, lane);
         ++count;
         evals += 42;
@@ -553,14 +561,14 @@ __device__ double k1_theta_adaptive(const K1Parent& P, const K1Z& Z, const doubl
 
 // LDS of the adaptive K1 kernel (doubles): spline (R nodes + 4 (R - 1)
 // coefficients), J1 table, outer list (4 cap), outer nodes and values (2 x
-// 42), the first-pass values of the pending inner integrals (42 x 21) and
-// their daughter terms (42 K1Z), then per wave a list (4 cap) and 42 node
-// values.
+// 42), the first-pass values of the pending inner integrals (42 x 21), their
+// daughter terms (42 K1Z) and first-pass estimates and errors (2 x 42), then
+// per wave a list (4 cap) and 42 node values.
 constexpr int kK1ZWords = sizeof(K1Z) / sizeof(double);
 static_assert(sizeof(K1Z) == kK1ZWords * sizeof(double), "K1Z is a record of doubles");
 __host__ __device__ inline size_t k1a_lds_doubles(int R, int cap) {
     return (size_t)R + 4 * (size_t)(R - 1) + kJ1Pieces * (kJ1Deg + 1) + 4 * (size_t)cap + 84 + 42 * 21 +
-           42 * kK1ZWords + (size_t)kK1aWaves * (4 * (size_t)cap + 42);
+           42 * kK1ZWords + 84 + (size_t)kK1aWaves * (4 * (size_t)cap + 42);
 }
 
 // Parent p of one evolution (arrays of that evolution).
@@ -590,8 +598,10 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     double* o_val = o_node + 42;        // 42 outer integrand values
     double* f_val = o_val + 42;         // 42 x 21 first-pass theta values
     K1Z* z_tab = reinterpret_cast<K1Z*>(f_val + 42 * 21);  // daughter terms of the 42 nodes
+    double* f_est = f_val + 42 * 21 + 42 * kK1ZWords;       // first-pass estimates
+    double* f_err = f_est + 42;                              // and errors
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double* w_base = f_val + 42 * 21 + 42 * kK1ZWords + (size_t)wave * (4 * cap + 42);
+    double* w_base = f_err + 42 + (size_t)wave * (4 * cap + 42);
     const GkList W{w_base, w_base + cap, w_base + 2 * cap, w_base + 3 * cap};
     double* w_val = w_base + 4 * cap;
 
@@ -636,6 +646,10 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
             f_val[i] = k1_theta_value<ANY>(P, Z, gk_node(0.0, kPi, k), s, c, H, rows, rapidity);
         }
         __syncthreads();
+        // the rule on [0, pi] of every pending inner integral, one thread each
+        for (int t = threadIdx.x; t < n_tasks; t += blockDim.x)
+            gk_from_values(0.0, kPi, f_val + 21 * t, f_est[t], f_err[t]);
+        __syncthreads();
         // the bisections of each inner integral, one wave each
         for (;;) {
             int t = 0;
@@ -644,8 +658,8 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
             if (t >= n_tasks) break;
             const double lz = o_node[t];
             const K1Z Z = z_tab[t];
-            const double inner = k1_theta_adaptive<ANY>(P, Z, f_val + 21 * t, s, c, H, rows, rapidity, tint,
-                                                        epsrel, W, w_val, lane, evals);
+            const double inner = k1_theta_adaptive<ANY>(P, Z, f_est[t], f_err[t], s, c, H, rows, rapidity,
+                                                        tint, epsrel, W, w_val, lane, evals);
             if (lane == 0) o_val[t] = inner * exp(2.0 * lz) * 2.0;
         }
         __syncthreads();
@@ -663,8 +677,7 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
             } else {
                 const double a = O.lo[split], b = O.hi[split], mid = (a + b) / 2.0;
                 double e0, r0, e1, r1;
-                gk_from_values(a, mid, o_val, e0, r0);
-                gk_from_values(mid, b, o_val + 21, e1, r1);
+                gk_halves_wave(a, mid, b, o_val, e0, r0, e1, r1, lane);
                 gk_insert_wave(O, count, split, mid, e0, r0, e1, r1, lane);
                 ++count;
             }
