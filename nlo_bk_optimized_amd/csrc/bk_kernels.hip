@@ -581,7 +581,8 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
                                                  double* __restrict__ k1_err,
                                                  long long* __restrict__ k1_evals) {
     extern __shared__ double sh[];
-    __shared__ int task_next, n_tasks, o_done;
+    __shared__ int task_next, n_tasks, o_done, n_open;
+    __shared__ int open_task[42];  // the inner integrals the first pass leaves open
     __shared__ unsigned long long ev_total;
     if (p == 0 || !(N[p] <= 0.99999)) {  // reference active set: index >= 1, N <= 0.99999
         if (threadIdx.x == 0) {
@@ -637,6 +638,7 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
     for (int step = 0; step < rint; ++step) {
         // the daughter terms of every pending inner integral, once
         for (int t = threadIdx.x; t < n_tasks; t += blockDim.x) z_tab[t] = k1_daughter<ANY>(P, o_node[t], s, c);
+        if (threadIdx.x == 0) n_open = 0;
         __syncthreads();
         // first pass on [0, pi] of every pending inner integral, over the
         // whole block (n_tasks x 21 theta nodes)
@@ -646,16 +648,31 @@ __device__ __forceinline__ void k1_adaptive_body(int R, const double* __restrict
             f_val[i] = k1_theta_value<ANY>(P, Z, gk_node(0.0, kPi, k), s, c, H, rows, rapidity);
         }
         __syncthreads();
-        // the rule on [0, pi] of every pending inner integral, one thread each
-        for (int t = threadIdx.x; t < n_tasks; t += blockDim.x)
-            gk_from_values(0.0, kPi, f_val + 21 * t, f_est[t], f_err[t]);
+        // the rule on [0, pi] of every pending inner integral, one thread each;
+        // an integral that meets its tolerance there (the first test of
+        // k1_theta_adaptive, on the list's sums 0 + e and 0 + r) is done, the
+        // others go to the bisections
+        for (int t = threadIdx.x; t < n_tasks; t += blockDim.x) {
+            double e, r;
+            gk_from_values(0.0, kPi, f_val + 21 * t, e, r);
+            const double value = 0.0 + e, err = 0.0 + r;
+            if (1 >= tint || err <= epsrel * fabs(value)) {
+                o_val[t] = value * exp(2.0 * o_node[t]) * 2.0;
+                atomicAdd(&ev_total, 21ull);
+            } else {
+                f_est[t] = e;
+                f_err[t] = r;
+                open_task[atomicAdd(&n_open, 1)] = t;
+            }
+        }
         __syncthreads();
         // the bisections of each inner integral, one wave each
         for (;;) {
             int t = 0;
             if (lane == 0) t = atomicAdd(&task_next, 1);
             t = __shfl(t, 0, 64);
-            if (t >= n_tasks) break;
+            if (t >= n_open) break;
+            t = open_task[t];
             const double lz = o_node[t];
             const K1Z Z = z_tab[t];
             const double inner = k1_theta_adaptive<ANY>(P, Z, f_est[t], f_err[t], s, c, H, rows, rapidity,
